@@ -1,0 +1,192 @@
+/*
+ * MaBoSS MI355X engine: C-ABI of the simulate/statistics path.
+ *
+ * The reference (MaBoSS.GPU) has no library API; its host/device cut is the
+ * kernel_compiler / kernel_wrapper / stats trio plus the CLI.  Every entry
+ * point below names the reference interface it replaces.  Conventions:
+ *   - every function returns an int status (MBG_OK = 0) and never exits the
+ *     process; mbg_last_error() describes the last failure of this thread;
+ *   - handles are opaque; objects created by the library are released by the
+ *     matching mbg_*_free;
+ *   - device buffers passed in are owned by the caller;
+ *   - a handle must not be used from two threads at once;
+ *   - streams are hipStream_t passed as void* (NULL = the default stream).
+ */
+#ifndef MABOSS_GPU_H
+#define MABOSS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBG_OK 0
+#define MBG_ERR_ARG 1      /* invalid argument */
+#define MBG_ERR_IO 2       /* file could not be read or written */
+#define MBG_ERR_PARSE 3    /* .bnd/.cfg syntax error */
+#define MBG_ERR_MODEL 4    /* unknown node, missing rate, undefined $variable, ... */
+#define MBG_ERR_LIMIT 5    /* model or run exceeds a supported size */
+#define MBG_ERR_COMPILE 6  /* hipRTC compile or link failed */
+#define MBG_ERR_HIP 7      /* HIP runtime error (no device, out of memory, ...) */
+#define MBG_ERR_OVERFLOW 8 /* a device hash table filled up */
+#define MBG_ERR_INTERNAL 9
+
+typedef struct mbg_model mbg_model;
+typedef struct mbg_program mbg_program;
+typedef struct mbg_result mbg_result;
+
+/* Last error message of the calling thread ("" if none). */
+const char* mbg_last_error(void);
+/* Library version string. */
+const char* mbg_version(void);
+
+/* ---- model (replaces driver::parse, src/parser/driver.cpp:16-31, and the
+ *      mask / istate / limit logic of src/main.cpp:15-36, :128-152) ---- */
+int mbg_model_load(const char* bnd_path, const char* cfg_path, mbg_model** out);
+int mbg_model_load_strings(const char* bnd_text, const char* cfg_text, mbg_model** out);
+void mbg_model_free(mbg_model* m);
+
+typedef struct mbg_model_info
+{
+	int32_t n_nodes;      /* N, in .bnd declaration order = bit order */
+	int32_t n_words;      /* ceil(N / 32) 32-bit state words */
+	int32_t n_observed;   /* non-internal nodes */
+	int32_t windows;      /* ceil(max_time / time_tick) */
+	int32_t discrete_time;
+	float max_time;
+	float time_tick;
+	int64_t sample_count;
+	uint64_t seed;        /* seed_pseudorandom, exact for integer literals */
+	int32_t n_params;     /* $variables used by rate expressions */
+	int32_t total_literals;
+} mbg_model_info;
+
+int mbg_model_get_info(const mbg_model* m, mbg_model_info* out);
+/* Name of node i (valid while m lives). */
+const char* mbg_model_node_name(const mbg_model* m, int32_t i);
+/* 1 if node i is internal (src/parser/parse_types.cpp:20-22). */
+int mbg_model_node_internal(const mbg_model* m, int32_t i);
+/* Override a .cfg constant (sample_count, max_time, time_tick, discrete_time,
+ * seed_pseudorandom, ...), as if written last in the .cfg. */
+int mbg_model_set_constant(mbg_model* m, const char* name, double value);
+/* Override a $variable ("$u_A" or "u_A"). */
+int mbg_model_set_variable(mbg_model* m, const char* name, double value);
+/* Name of rate parameter k (a $variable, with '$'). */
+const char* mbg_model_param_name(const mbg_model* m, int32_t k);
+/* Value of rate parameter k as the model defines it. */
+float mbg_model_param_value(const mbg_model* m, int32_t k);
+/* Evaluate node i's rate in a state with the host interpreter; used by tests. */
+int mbg_model_eval_rate(const mbg_model* m, int32_t node, const uint32_t* state_words, float* rate);
+
+/* ---- code generation + JIT (replaces generator::generate_code,
+ *      src/generator.cpp:11-55, and kernel_compiler::compile_simulation,
+ *      src/kernel_compiler.cpp:61-164) ---- */
+typedef struct mbg_compile_opts
+{
+	int32_t entropy_formula; /* 0 = log-sum form (default), 1 = reference form */
+	int32_t no_cache;        /* 1: bypass the code-object cache */
+	int32_t reserved[6];
+} mbg_compile_opts;
+
+/* Generated HIP source of the model (caller frees with mbg_free_string). */
+int mbg_generate_code(const mbg_model* m, const mbg_compile_opts* opts, char** source);
+void mbg_free_string(char* s);
+/* Generate, compile with hipRTC for gfx950 and link against the precompiled
+ * kernels.  Needs a GPU (the link targets the current device). */
+int mbg_program_compile(const mbg_model* m, const mbg_compile_opts* opts, mbg_program** out);
+void mbg_program_free(mbg_program* p);
+/* Seconds spent generating + compiling + linking (0 on a cache hit path is
+ * not guaranteed; it includes the cache lookup). */
+double mbg_program_compile_seconds(const mbg_program* p);
+
+/* ---- simulation (replaces simulation_runner::run_simulation,
+ *      src/simulation_runner.cu:54-180, and the stats_composite pipeline,
+ *      src/statistics/stats_composite.cpp:3-30) ---- */
+typedef struct mbg_sim_params
+{
+	int64_t n_trajectories;  /* <0: sample_count of the model */
+	uint64_t first_id;       /* global id of the first trajectory (prefix runs) */
+	int64_t seed;            /* <0: seed_pseudorandom of the model */
+	uint32_t variant;        /* stream tag (batched variant index) */
+	uint32_t max_steps;      /* 0: default cap on SSA steps per trajectory */
+	int32_t window_mode;     /* -1 auto, 0 dense HBM, 1 hash, 2 dense LDS */
+	int32_t final_mode;      /* -1 auto, 0 dense HBM, 1 hash, 2 dense LDS */
+	uint64_t window_capacity;/* hash slots (power of two), 0 = auto */
+	uint64_t final_capacity;
+	uint64_t fixed_capacity;
+	int32_t block_size;      /* 0 = 256 */
+	int32_t blocks_per_cu;   /* 0 = occupancy limit */
+	double normalization;    /* <=0: n_trajectories (reference divides by sample_count) */
+	int32_t reserved[8];
+} mbg_sim_params;
+
+/* Fill p with the defaults above. */
+void mbg_sim_params_default(mbg_sim_params* p);
+/* Simulate and collect window, final-state and fixed-point statistics.
+ * Inputs live in HBM for the whole run; the result is copied to the host
+ * before returning (the stream is synchronised). */
+int mbg_simulate(mbg_program* p, const mbg_sim_params* params, void* stream, mbg_result** out);
+
+/* ---- results (replaces stats::finalize / visualize / write_csv,
+ *      src/statistics/stats.h:15-29) ---- */
+void mbg_result_free(mbg_result* r);
+
+typedef struct mbg_result_info
+{
+	int32_t windows;
+	int32_t n_observed;
+	int32_t n_words;
+	int32_t discrete_time;
+	float time_tick;
+	float max_time;
+	double normalization;            /* trajectories the probabilities divide by */
+	uint64_t n_trajectories;
+	uint64_t transitions;            /* SSA iterations (records) */
+	uint64_t finished;               /* trajectories that reached max_time or a fixed point */
+	uint64_t truncated;              /* trajectories stopped by the step cap */
+	uint64_t n_fixed_points;         /* distinct fixed points */
+	uint64_t n_final_states;         /* distinct final observed states */
+	uint64_t n_window_cells;         /* nonzero (window, state) cells */
+	double kernel_ms;                /* device time of the simulate kernel(s) */
+	double total_ms;                 /* host wall time of mbg_simulate */
+	int32_t window_mode, final_mode;
+	uint64_t window_capacity;
+} mbg_result_info;
+
+int mbg_result_get_info(const mbg_result* r, mbg_result_info* out);
+/* Number of nonzero observed states in window w. */
+int64_t mbg_result_window_size(const mbg_result* r, int32_t w);
+/* Window w: observed-state indices (bit j = j-th non-internal node) and raw
+ * fixed-point dwell sums (continuous: dwell*2^32/tick, discrete: counts),
+ * sorted by index.  Arrays hold mbg_result_window_size(r, w) entries. */
+int mbg_result_window(const mbg_result* r, int32_t w, uint64_t* states, uint64_t* raw);
+/* Window w: probabilities (as the reference normalises them). */
+int mbg_result_window_probs(const mbg_result* r, int32_t w, uint64_t* states, double* probs);
+/* Per-window transition entropy TH and state entropy H. */
+int mbg_result_entropies(const mbg_result* r, double* th, double* h);
+/* Final observed states and counts, sorted by index. */
+int64_t mbg_result_final_size(const mbg_result* r);
+int mbg_result_final(const mbg_result* r, uint64_t* states, uint64_t* counts);
+/* Fixed points: n_words words per state, in the reference's order (most
+ * significant word first, src/state.h:43-49), and their counts. */
+int64_t mbg_result_fixed_size(const mbg_result* r);
+int mbg_result_fixed(const mbg_result* r, uint32_t* states, uint64_t* counts);
+
+/* <prefix>_probtraj.csv and <prefix>_fp.csv (src/statistics/window_average_small.cpp:181-250,
+ * src/statistics/fixed_states.cu:154-187); with_finals adds <prefix>_finalstates.csv. */
+int mbg_result_write_csv(const mbg_result* r, const char* prefix, int with_finals);
+/* The stdout report of the reference (final points, fixed points, windows)
+ * to a file path, or to stdout when path is NULL
+ * (src/statistics/final_states.cpp:53-66, fixed_states.cu:140-152,
+ * window_average_small.cpp:142-179). */
+int mbg_result_print(const mbg_result* r, const char* path);
+/* Exact integer dump used by the parity tests (one record per line). */
+int mbg_result_write_raw(const mbg_result* r, const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MABOSS_GPU_H */

@@ -1,0 +1,232 @@
+"""MaBoSS on MI355X: Python face of the engine's C-ABI.
+
+The reference (MaBoSS.GPU) is driven from its command line
+(``MaBoSS.GPU [-o prefix] model.bnd model.cfg``, src/main.cpp:101-169).  This
+module exposes the same pipeline -- parse, generate + JIT, simulate, write
+``_probtraj.csv`` / ``_fp.csv`` -- as objects, for tests, benchmarks and the
+differentiable form.  All computation happens in the HIP kernels behind
+``libmaboss_gpu.so``; there is no Python or CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._capi import (CLI_PATH, CompileOpts, MbgError, ModelInfo, ResultInfo,
+                    SimParams, check, lib)
+
+__all__ = ["Model", "Program", "Result", "simulate", "MbgError", "run_cli"]
+
+WINDOW_MODES = {"auto": -1, "dense": 0, "hash": 1, "lds": 2}
+
+
+class Model:
+    """A parsed .bnd/.cfg model (the reference's ``driver``)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def load(cls, bnd_path: str, cfg_path: str) -> "Model":
+        h = ctypes.c_void_p()
+        check(lib().mbg_model_load(os.fsencode(bnd_path), os.fsencode(cfg_path), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_strings(cls, bnd: str, cfg: str) -> "Model":
+        h = ctypes.c_void_p()
+        check(lib().mbg_model_load_strings(bnd.encode(), cfg.encode(), ctypes.byref(h)))
+        return cls(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().mbg_model_free(h)
+            self._h = None
+
+    @property
+    def info(self) -> ModelInfo:
+        out = ModelInfo()
+        check(lib().mbg_model_get_info(self._h, ctypes.byref(out)))
+        return out
+
+    @property
+    def node_names(self) -> list[str]:
+        return [lib().mbg_model_node_name(self._h, i).decode() for i in range(self.info.n_nodes)]
+
+    @property
+    def internal(self) -> list[bool]:
+        return [lib().mbg_model_node_internal(self._h, i) == 1 for i in range(self.info.n_nodes)]
+
+    @property
+    def observed(self) -> list[int]:
+        return [i for i, x in enumerate(self.internal) if not x]
+
+    @property
+    def param_names(self) -> list[str]:
+        return [lib().mbg_model_param_name(self._h, k).decode() for k in range(self.info.n_params)]
+
+    @property
+    def param_values(self) -> np.ndarray:
+        return np.array([lib().mbg_model_param_value(self._h, k) for k in range(self.info.n_params)],
+                        dtype=np.float32)
+
+    def set_constant(self, name: str, value: float):
+        check(lib().mbg_model_set_constant(self._h, name.encode(), float(value)))
+
+    def set_variable(self, name: str, value: float):
+        check(lib().mbg_model_set_variable(self._h, name.encode(), float(value)))
+
+    def rate(self, node: int, state_words) -> float:
+        """Rate of ``node`` in a state, by the host expression interpreter."""
+        arr = (ctypes.c_uint32 * len(state_words))(*state_words)
+        out = ctypes.c_float()
+        check(lib().mbg_model_eval_rate(self._h, node, arr, ctypes.byref(out)))
+        return out.value
+
+    def generate_code(self, entropy_formula: int = 0) -> str:
+        opts = CompileOpts(entropy_formula=entropy_formula)
+        p = ctypes.c_void_p()
+        check(lib().mbg_generate_code(self._h, ctypes.byref(opts), ctypes.byref(p)))
+        try:
+            return ctypes.string_at(p).decode()
+        finally:
+            lib().mbg_free_string(p)
+
+    def compile(self, entropy_formula: int = 0, no_cache: bool = False) -> "Program":
+        return Program(self, entropy_formula=entropy_formula, no_cache=no_cache)
+
+
+class Program:
+    """Generated + JIT-compiled kernels of a model (the reference's kernel_compiler)."""
+
+    def __init__(self, model: Model, entropy_formula: int = 0, no_cache: bool = False):
+        opts = CompileOpts(entropy_formula=entropy_formula, no_cache=int(no_cache))
+        h = ctypes.c_void_p()
+        check(lib().mbg_program_compile(model._h, ctypes.byref(opts), ctypes.byref(h)))
+        self._h = h
+        self.model = model
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().mbg_program_free(h)
+            self._h = None
+
+    @property
+    def compile_seconds(self) -> float:
+        return lib().mbg_program_compile_seconds(self._h)
+
+    def simulate(self, n_trajectories: int = -1, first_id: int = 0, seed: int = -1, variant: int = 0,
+                 max_steps: int = 0, window_mode: str = "auto", final_mode: str = "auto",
+                 window_capacity: int = 0, final_capacity: int = 0, fixed_capacity: int = 0,
+                 block_size: int = 0, blocks_per_cu: int = 0, normalization: float = 0.0,
+                 stream=None) -> "Result":
+        p = SimParams()
+        lib().mbg_sim_params_default(ctypes.byref(p))
+        p.n_trajectories = n_trajectories
+        p.first_id = first_id
+        p.seed = seed
+        p.variant = variant
+        p.max_steps = max_steps
+        p.window_mode = WINDOW_MODES[window_mode]
+        p.final_mode = WINDOW_MODES[final_mode]
+        p.window_capacity = window_capacity
+        p.final_capacity = final_capacity
+        p.fixed_capacity = fixed_capacity
+        p.block_size = block_size
+        p.blocks_per_cu = blocks_per_cu
+        p.normalization = normalization
+        h = ctypes.c_void_p()
+        s = ctypes.c_void_p(stream) if stream else None
+        check(lib().mbg_simulate(self._h, ctypes.byref(p), s, ctypes.byref(h)))
+        return Result(h, self.model)
+
+
+@dataclass
+class Window:
+    states: np.ndarray  # uint64 observed-state indices
+    raw: np.ndarray     # uint64 fixed-point sums
+    probs: np.ndarray   # float64
+
+
+class Result:
+    """Statistics of one run: window probabilities, entropies, final states, fixed points."""
+
+    def __init__(self, handle, model: Model):
+        self._h = handle
+        self.model = model
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().mbg_result_free(h)
+            self._h = None
+
+    @property
+    def info(self) -> ResultInfo:
+        out = ResultInfo()
+        check(lib().mbg_result_get_info(self._h, ctypes.byref(out)))
+        return out
+
+    def window(self, w: int) -> Window:
+        n = lib().mbg_result_window_size(self._h, w)
+        if n < 0:
+            raise IndexError(w)
+        states = np.zeros(n, dtype=np.uint64)
+        raw = np.zeros(n, dtype=np.uint64)
+        probs = np.zeros(n, dtype=np.float64)
+        check(lib().mbg_result_window(self._h, w, states.ctypes.data, raw.ctypes.data))
+        check(lib().mbg_result_window_probs(self._h, w, None, probs.ctypes.data))
+        return Window(states, raw, probs)
+
+    def windows(self) -> list[Window]:
+        return [self.window(w) for w in range(self.info.windows)]
+
+    def entropies(self) -> tuple[np.ndarray, np.ndarray]:
+        nw = self.info.windows
+        th = np.zeros(nw)
+        h = np.zeros(nw)
+        check(lib().mbg_result_entropies(self._h, th.ctypes.data, h.ctypes.data))
+        return th, h
+
+    def final_states(self) -> tuple[np.ndarray, np.ndarray]:
+        n = lib().mbg_result_final_size(self._h)
+        s = np.zeros(n, dtype=np.uint64)
+        c = np.zeros(n, dtype=np.uint64)
+        check(lib().mbg_result_final(self._h, s.ctypes.data, c.ctypes.data))
+        return s, c
+
+    def fixed_points(self) -> tuple[np.ndarray, np.ndarray]:
+        n = lib().mbg_result_fixed_size(self._h)
+        nw = self.info.n_words
+        s = np.zeros((n, nw), dtype=np.uint32)
+        c = np.zeros(n, dtype=np.uint64)
+        check(lib().mbg_result_fixed(self._h, s.ctypes.data, c.ctypes.data))
+        return s, c
+
+    def write_csv(self, prefix: str, with_finals: bool = False):
+        check(lib().mbg_result_write_csv(self._h, os.fsencode(prefix), int(with_finals)))
+
+    def print(self, path: str | None = None):
+        check(lib().mbg_result_print(self._h, os.fsencode(path) if path else None))
+
+    def write_raw(self, path: str):
+        check(lib().mbg_result_write_raw(self._h, os.fsencode(path)))
+
+
+def simulate(bnd_path: str, cfg_path: str, **kw) -> Result:
+    """Parse, compile and simulate in one call."""
+    m = Model.load(bnd_path, cfg_path)
+    return m.compile().simulate(**kw)
+
+
+def run_cli(*args: str, timeout: float = 600) -> subprocess.CompletedProcess:
+    """Run the MaBoSS.GPU command line built next to the library."""
+    if not os.path.exists(CLI_PATH):
+        raise RuntimeError(f"{CLI_PATH} is missing: build first")
+    return subprocess.run([CLI_PATH, *args], capture_output=True, text=True, timeout=timeout)

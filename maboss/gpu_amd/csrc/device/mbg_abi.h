@@ -1,0 +1,42 @@
+// Device-side ABI between the per-model generated code (emitted by the host
+// generator and compiled by hipRTC) and the hand-written kernels that are
+// precompiled to LLVM bitcode at build time.  hipRTC links the two bitcode
+// modules (with whole-program optimisation), so every call below is inlined
+// into the kernel.
+//
+// The generated functions replace the reference's generated
+// compute_transition_rates / compute_transition_entropy /
+// get_non_internal_index (src/generator.cpp:124-175, :207-235) and the
+// flip selection of src/jit_kernels/simulation.cu:9-21.
+#pragma once
+
+#include "mbg_numeric.h"
+
+// Result of evaluating every node's rate in one state.
+struct mbg_eval_t
+{
+	float total; // sum of all node rates, sequential in node order
+	float H;     // transition entropy of the observed (non-internal) nodes
+	int flip;    // node to flip for the given u2 (first i with prefix_i >= u2*total)
+	int pad;
+};
+
+// Per-variant runtime tables for the batched and differentiated forms.
+struct mbg_param_view
+{
+	const float* rate;      // [n_params] value of each $variable used by a rate
+	const float* rate_lr;   // [n_params] rate * log2(rate), precomputed
+	const uint32_t* up_en;  // [W] bit i clear: node i can never switch on
+	const uint32_t* dn_en;  // [W] bit i clear: node i can never switch off
+};
+
+#if defined(__HIPCC__) || defined(__HIPCC_RTC__)
+// Rates with the model's $variables baked in as exact float constants.
+extern "C" __device__ void mbg_eval(const uint32_t* __restrict__ s, float u2, mbg_eval_t* __restrict__ out);
+// Rates read from runtime tables (batched variants, gradients).
+extern "C" __device__ void mbg_eval_p(const uint32_t* __restrict__ s, float u2, const mbg_param_view* __restrict__ pv,
+									  mbg_eval_t* __restrict__ out);
+// Parallel bit-extract of the observed nodes, in node order (bit j of the
+// result is the j-th non-internal node).
+extern "C" __device__ uint64_t mbg_obs_index(const uint32_t* __restrict__ s);
+#endif

@@ -1,0 +1,50 @@
+// Argument block of the fused simulate kernels; shared by the host runner and
+// the device code (plain C layout, passed by value as the only kernel argument).
+#pragma once
+
+#include "mbg_abi.h"
+
+// Histogram storage modes.
+#define MBG_HIST_DENSE 0  // cell (w << n_obs) | obs, in HBM
+#define MBG_HIST_HASH 1   // open-addressing table keyed by (w << n_obs) | obs
+#define MBG_HIST_LDS 2    // dense, privatised in LDS and flushed once per block
+
+// Counter slots (u64) written by the kernels.
+#define MBG_CTR_TRANSITIONS 0    // SSA iterations = records of the reference
+#define MBG_CTR_WIN_OVERFLOW 1   // window hash table full
+#define MBG_CTR_FIN_OVERFLOW 2   // final-state hash table full
+#define MBG_CTR_FP_OVERFLOW 3    // fixed-point table full
+#define MBG_CTR_TRUNCATED 4      // trajectories stopped by the step cap
+#define MBG_CTR_FP_TIMEOUT 5     // fixed-point slot never published (bounded wait ended)
+#define MBG_CTR_QUEUE 6          // trajectory work queue
+#define MBG_CTR_FINISHED 7       // trajectories ended at max_time or a fixed point
+#define MBG_NCOUNTERS 8
+
+#define MBG_EMPTY_KEY 0xffffffffffffffffull
+
+struct mbg_sim_args
+{
+	unsigned long long n_traj;  // trajectories in this launch
+	unsigned long long id_base; // global id of the launch's first trajectory
+	unsigned int seed_lo, seed_hi, variant, max_steps;
+	int n_nodes, n_obs, windows, discrete;
+	float max_time, tick;
+	float qscale;   // 2^32 / tick (continuous) or 2^32 (discrete entropy)
+	float pad0;
+	const float* istate; // [n_nodes] probability that node starts on
+
+	int win_mode, fin_mode, ent_lds, lds_bytes; // storage modes, LDS size in bytes
+	int win_lds_off, fin_lds_off, ent_lds_off, pad1; // u64 offsets into dynamic LDS
+
+	unsigned long long* win_vals;   // dense cells or hash values
+	unsigned long long* win_keys;   // hash keys (MBG_HIST_HASH)
+	unsigned long long win_mask;    // hash capacity - 1
+	unsigned long long* win_ent;    // [windows] entropy sums
+	unsigned long long* fin_vals;
+	unsigned long long* fin_keys;
+	unsigned long long fin_mask;
+	unsigned int* fp_table;         // slots of (2 + W) u32: tag, count, state words
+	unsigned long long fp_mask;
+	unsigned long long* counters;   // [MBG_NCOUNTERS]
+	const mbg_param_view* pv;       // per-variant tables (batched/gradient); null for plain
+};

@@ -1,0 +1,504 @@
+// C-ABI of the engine (declared in include/maboss_gpu.h).  Every entry point
+// catches exceptions and turns them into a status code plus a thread-local
+// message; nothing here calls exit() (the reference exits on any CUDA error,
+// src/utils.cpp:3-30).
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <iostream>
+
+#include "../../../../include/maboss_gpu.h"
+#include "engine.h"
+
+struct mbg_model
+{
+	std::shared_ptr<mbg::model> m;
+	std::vector<std::string> params; // rate parameters, as the generator orders them
+};
+
+struct mbg_program
+{
+	std::unique_ptr<mbg::program> p;
+};
+
+struct mbg_result
+{
+	std::unique_ptr<mbg::result> r;
+};
+
+namespace
+{
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f)
+{
+	try
+	{
+		g_last_error.clear();
+		f();
+		return MBG_OK;
+	}
+	catch (const mbg::error& e)
+	{
+		g_last_error = e.what();
+		return e.code;
+	}
+	catch (const std::bad_alloc&)
+	{
+		g_last_error = "host out of memory";
+		return MBG_ERR_LIMIT;
+	}
+	catch (const std::exception& e)
+	{
+		g_last_error = e.what();
+		return MBG_ERR_INTERNAL;
+	}
+}
+
+void refresh_params(mbg_model* mm)
+{
+	mbg::gen_options o;
+	mm->params = mbg::generate(*mm->m, o).param_names;
+}
+
+mbg::gen_options to_gen(const mbg_compile_opts* o)
+{
+	mbg::gen_options g;
+	if (o && o->entropy_formula == 1)
+		g.entropy = mbg::entropy_formula::reference;
+	return g;
+}
+
+float interp(const mbg::model& m, const mbg::node_t& cur, const mbg::expr& e, const uint32_t* s, int depth)
+{
+	using mbg::op;
+	if (depth > 64)
+		throw mbg::error(mbg::MBG_E_MODEL, "alias nesting too deep");
+	auto f = [&](const mbg::expr& x) { return interp(m, cur, x, s, depth); };
+	switch (e.kind)
+	{
+		case op::node: {
+			int i = m.node_index(e.name);
+			return (float)((s[i / 32] >> (i % 32)) & 1u);
+		}
+		case op::alias: return interp(m, cur, *cur.attr(e.name.substr(1)), s, depth + 1);
+		case op::lit: return e.value;
+		case op::var: return m.variables.at(e.name);
+		case op::paren:
+		case op::pos: return f(*e.a);
+		case op::neg: return -f(*e.a);
+		case op::lnot: return !f(*e.a);
+		case op::add: return f(*e.a) + f(*e.b);
+		case op::sub: return f(*e.a) - f(*e.b);
+		case op::mul: return f(*e.a) * f(*e.b);
+		case op::div: return f(*e.a) / f(*e.b);
+		case op::land: return f(*e.a) && f(*e.b);
+		case op::lor: return f(*e.a) || f(*e.b);
+		case op::lxor: return (f(*e.a) != 0) != (f(*e.b) != 0);
+		case op::eq: return f(*e.a) == f(*e.b);
+		case op::ne: return f(*e.a) != f(*e.b);
+		case op::lt: return f(*e.a) < f(*e.b);
+		case op::le: return f(*e.a) <= f(*e.b);
+		case op::gt: return f(*e.a) > f(*e.b);
+		case op::ge: return f(*e.a) >= f(*e.b);
+		case op::ternary: return f(*e.a) ? f(*e.b) : f(*e.c);
+	}
+	return 0.f;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* mbg_last_error(void) { return g_last_error.c_str(); }
+
+const char* mbg_version(void) { return "maboss-gpu-amd 0.1 (gfx950)"; }
+
+int mbg_model_load(const char* bnd_path, const char* cfg_path, mbg_model** out)
+{
+	return guarded([&] {
+		if (!bnd_path || !cfg_path || !out)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		auto mm = std::make_unique<mbg_model>();
+		mm->m = std::shared_ptr<mbg::model>(mbg::load_model_files(bnd_path, cfg_path).release());
+		refresh_params(mm.get());
+		*out = mm.release();
+	});
+}
+
+int mbg_model_load_strings(const char* bnd_text, const char* cfg_text, mbg_model** out)
+{
+	return guarded([&] {
+		if (!bnd_text || !cfg_text || !out)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		auto mm = std::make_unique<mbg_model>();
+		mm->m = std::shared_ptr<mbg::model>(mbg::load_model_strings(bnd_text, cfg_text).release());
+		refresh_params(mm.get());
+		*out = mm.release();
+	});
+}
+
+void mbg_model_free(mbg_model* m) { delete m; }
+
+int mbg_model_get_info(const mbg_model* mm, mbg_model_info* out)
+{
+	return guarded([&] {
+		if (!mm || !out)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		const mbg::model& m = *mm->m;
+		std::memset(out, 0, sizeof(*out));
+		out->n_nodes = m.n_nodes();
+		out->n_words = m.n_words();
+		out->n_observed = m.n_observed();
+		out->windows = m.windows();
+		out->discrete_time = m.discrete_time();
+		out->max_time = m.max_time();
+		out->time_tick = m.time_tick();
+		out->sample_count = m.sample_count();
+		out->seed = m.seed();
+		out->n_params = (int32_t)mm->params.size();
+		mbg::gen_options o;
+		out->total_literals = mbg::generate(m, o).total_literals;
+	});
+}
+
+const char* mbg_model_node_name(const mbg_model* mm, int32_t i)
+{
+	if (!mm || i < 0 || i >= mm->m->n_nodes())
+		return nullptr;
+	return mm->m->nodes[i].name.c_str();
+}
+
+int mbg_model_node_internal(const mbg_model* mm, int32_t i)
+{
+	if (!mm || i < 0 || i >= mm->m->n_nodes())
+		return -1;
+	try
+	{
+		return mm->m->is_internal((size_t)i) ? 1 : 0;
+	}
+	catch (...)
+	{
+		return -1;
+	}
+}
+
+int mbg_model_set_constant(mbg_model* mm, const char* name, double value)
+{
+	return guarded([&] {
+		if (!mm || !name)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		mm->m->constants[name] = (float)value;
+		mm->m->constants_exact[name] = value;
+		mbg::validate(*mm->m);
+	});
+}
+
+int mbg_model_set_variable(mbg_model* mm, const char* name, double value)
+{
+	return guarded([&] {
+		if (!mm || !name)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		std::string n = name[0] == '$' ? name : std::string("$") + name;
+		mm->m->variables[n] = (float)value;
+		refresh_params(mm);
+	});
+}
+
+const char* mbg_model_param_name(const mbg_model* mm, int32_t k)
+{
+	if (!mm || k < 0 || k >= (int32_t)mm->params.size())
+		return nullptr;
+	return mm->params[k].c_str();
+}
+
+float mbg_model_param_value(const mbg_model* mm, int32_t k)
+{
+	if (!mm || k < 0 || k >= (int32_t)mm->params.size())
+		return 0.f;
+	return mm->m->variables.at(mm->params[k]);
+}
+
+int mbg_model_eval_rate(const mbg_model* mm, int32_t node, const uint32_t* s, float* rate)
+{
+	return guarded([&] {
+		if (!mm || !s || !rate || node < 0 || node >= mm->m->n_nodes())
+			throw mbg::error(mbg::MBG_E_ARG, "bad argument");
+		const mbg::model& m = *mm->m;
+		const mbg::node_t& n = m.nodes[node];
+		const bool on = (s[node / 32] >> (node % 32)) & 1u;
+		*rate = interp(m, n, *n.attr(on ? "rate_down" : "rate_up"), s, 0);
+	});
+}
+
+int mbg_generate_code(const mbg_model* mm, const mbg_compile_opts* opts, char** source)
+{
+	return guarded([&] {
+		if (!mm || !source)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		std::string s = mbg::generate(*mm->m, to_gen(opts)).source;
+		char* c = (char*)std::malloc(s.size() + 1);
+		if (!c)
+			throw std::bad_alloc();
+		std::memcpy(c, s.c_str(), s.size() + 1);
+		*source = c;
+	});
+}
+
+void mbg_free_string(char* s) { std::free(s); }
+
+int mbg_program_compile(const mbg_model* mm, const mbg_compile_opts* opts, mbg_program** out)
+{
+	return guarded([&] {
+		if (!mm || !out)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		// the program keeps its own copy of the model
+		auto copy = std::make_shared<mbg::model>();
+		{
+			// re-parse is not possible (text not kept); deep-copy the IR
+			std::function<mbg::expr_ptr(const mbg::expr*)> clone = [&](const mbg::expr* e) -> mbg::expr_ptr {
+				if (!e)
+					return nullptr;
+				auto c = mbg::expr::make(e->kind);
+				c->value = e->value;
+				c->exact = e->exact;
+				c->name = e->name;
+				c->a = clone(e->a.get());
+				c->b = clone(e->b.get());
+				c->c = clone(e->c.get());
+				return c;
+			};
+			copy->variables = mm->m->variables;
+			copy->constants = mm->m->constants;
+			copy->constants_exact = mm->m->constants_exact;
+			for (auto& n : mm->m->nodes)
+			{
+				mbg::node_t nn;
+				nn.name = n.name;
+				nn.istate = n.istate;
+				for (auto& a : n.attrs)
+					nn.attrs.emplace_back(a.first, clone(a.second.get()));
+				copy->nodes.push_back(std::move(nn));
+			}
+		}
+		auto pp = std::make_unique<mbg_program>();
+		pp->p = mbg::compile_program(copy, to_gen(opts), opts && opts->no_cache);
+		*out = pp.release();
+	});
+}
+
+void mbg_program_free(mbg_program* p) { delete p; }
+
+double mbg_program_compile_seconds(const mbg_program* p) { return p ? p->p->compile_seconds : 0.0; }
+
+void mbg_sim_params_default(mbg_sim_params* p)
+{
+	if (!p)
+		return;
+	std::memset(p, 0, sizeof(*p));
+	p->n_trajectories = -1;
+	p->seed = -1;
+	p->window_mode = -1;
+	p->final_mode = -1;
+}
+
+int mbg_simulate(mbg_program* p, const mbg_sim_params* params, void* stream, mbg_result** out)
+{
+	return guarded([&] {
+		if (!p || !out)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		mbg::sim_params sp;
+		if (params)
+		{
+			sp.n_traj = params->n_trajectories;
+			sp.first_id = params->first_id;
+			sp.seed = params->seed;
+			sp.variant = params->variant;
+			sp.max_steps = params->max_steps;
+			sp.window_mode = params->window_mode;
+			sp.final_mode = params->final_mode;
+			sp.window_capacity = params->window_capacity;
+			sp.final_capacity = params->final_capacity;
+			sp.fixed_capacity = params->fixed_capacity;
+			sp.block_size = params->block_size;
+			sp.blocks_per_cu = params->blocks_per_cu;
+			sp.normalization = params->normalization;
+		}
+		if (sp.block_size < 0 || sp.block_size > 1024 || (sp.block_size % 64) != 0)
+			throw mbg::error(mbg::MBG_E_ARG, "block_size must be a multiple of 64 up to 1024");
+		auto rr = std::make_unique<mbg_result>();
+		rr->r = mbg::simulate(*p->p, sp, (hipStream_t)stream);
+		*out = rr.release();
+	});
+}
+
+void mbg_result_free(mbg_result* r) { delete r; }
+
+int mbg_result_get_info(const mbg_result* rr, mbg_result_info* out)
+{
+	return guarded([&] {
+		if (!rr || !out)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		const mbg::result& r = *rr->r;
+		std::memset(out, 0, sizeof(*out));
+		out->windows = r.windows;
+		out->n_observed = r.n_obs;
+		out->n_words = r.n_words;
+		out->discrete_time = r.discrete;
+		out->time_tick = r.tick;
+		out->max_time = r.max_time;
+		out->normalization = r.norm;
+		out->n_trajectories = r.n_traj;
+		out->transitions = r.transitions;
+		out->finished = r.finished;
+		out->truncated = r.truncated;
+		out->n_fixed_points = r.fixed.size();
+		out->n_final_states = r.fin.size();
+		uint64_t cells = 0;
+		for (auto& w : r.win)
+			cells += w.size();
+		out->n_window_cells = cells;
+		out->kernel_ms = r.kernel_ms;
+		out->total_ms = r.total_ms;
+		out->window_mode = r.window_mode;
+		out->final_mode = r.final_mode;
+		out->window_capacity = r.window_capacity;
+	});
+}
+
+int64_t mbg_result_window_size(const mbg_result* rr, int32_t w)
+{
+	if (!rr || w < 0 || w >= rr->r->windows)
+		return -1;
+	return (int64_t)rr->r->win[w].size();
+}
+
+int mbg_result_window(const mbg_result* rr, int32_t w, uint64_t* states, uint64_t* raw)
+{
+	return guarded([&] {
+		if (!rr || w < 0 || w >= rr->r->windows)
+			throw mbg::error(mbg::MBG_E_ARG, "bad window");
+		size_t i = 0;
+		for (auto& c : rr->r->win[w])
+		{
+			if (states)
+				states[i] = c.first;
+			if (raw)
+				raw[i] = c.second;
+			i++;
+		}
+	});
+}
+
+int mbg_result_window_probs(const mbg_result* rr, int32_t w, uint64_t* states, double* probs)
+{
+	return guarded([&] {
+		if (!rr || w < 0 || w >= rr->r->windows)
+			throw mbg::error(mbg::MBG_E_ARG, "bad window");
+		size_t i = 0;
+		for (auto& c : rr->r->win[w])
+		{
+			if (states)
+				states[i] = c.first;
+			if (probs)
+				probs[i] = rr->r->prob(c.second);
+			i++;
+		}
+	});
+}
+
+int mbg_result_entropies(const mbg_result* rr, double* th, double* h)
+{
+	return guarded([&] {
+		if (!rr)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		for (int w = 0; w < rr->r->windows; w++)
+		{
+			if (th)
+				th[w] = rr->r->th(w);
+			if (h)
+				h[w] = rr->r->h(w);
+		}
+	});
+}
+
+int64_t mbg_result_final_size(const mbg_result* rr) { return rr ? (int64_t)rr->r->fin.size() : -1; }
+
+int mbg_result_final(const mbg_result* rr, uint64_t* states, uint64_t* counts)
+{
+	return guarded([&] {
+		if (!rr)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		size_t i = 0;
+		for (auto& f : rr->r->fin)
+		{
+			if (states)
+				states[i] = f.first;
+			if (counts)
+				counts[i] = f.second;
+			i++;
+		}
+	});
+}
+
+int64_t mbg_result_fixed_size(const mbg_result* rr) { return rr ? (int64_t)rr->r->fixed.size() : -1; }
+
+int mbg_result_fixed(const mbg_result* rr, uint32_t* states, uint64_t* counts)
+{
+	return guarded([&] {
+		if (!rr)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		const size_t W = (size_t)rr->r->n_words;
+		size_t i = 0;
+		for (auto& f : rr->r->fixed)
+		{
+			if (states)
+				std::memcpy(states + i * W, f.first.data(), W * 4);
+			if (counts)
+				counts[i] = f.second;
+			i++;
+		}
+	});
+}
+
+int mbg_result_write_csv(const mbg_result* rr, const char* prefix, int with_finals)
+{
+	return guarded([&] {
+		if (!rr || !prefix)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		rr->r->write_csv(prefix, with_finals != 0);
+	});
+}
+
+int mbg_result_print(const mbg_result* rr, const char* path)
+{
+	return guarded([&] {
+		if (!rr)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		if (!path)
+		{
+			rr->r->print(std::cout);
+			return;
+		}
+		std::ofstream f(path);
+		if (!f)
+			throw mbg::error(mbg::MBG_E_IO, std::string("cannot write ") + path);
+		rr->r->print(f);
+	});
+}
+
+int mbg_result_write_raw(const mbg_result* rr, const char* path)
+{
+	return guarded([&] {
+		if (!rr || !path)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		std::ofstream f(path);
+		if (!f)
+			throw mbg::error(mbg::MBG_E_IO, std::string("cannot write ") + path);
+		rr->r->write_raw(f);
+	});
+}
+
+} // extern "C"

@@ -1,0 +1,107 @@
+// Host engine: kernel compiler (hipRTC), simulation runner and results.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "generator.h"
+#include "model.h"
+
+namespace mbg
+{
+
+// Throws mbg::error(MBG_E_HIP) with the HIP error string.
+void hip_check(hipError_t e, const char* what);
+
+// Device allocation that only grows; reused by successive runs of a program.
+struct device_buffer
+{
+	void* ptr = nullptr;
+	size_t bytes = 0;
+	void* ensure(size_t n);
+	void release();
+	~device_buffer() { release(); }
+};
+
+// The reference's kernel_compiler (src/kernel_compiler.{h,cpp}): generated
+// source -> hipRTC bitcode -> link with the precompiled kernel bitcode ->
+// code object -> module and kernel handles.
+struct program
+{
+	// run workspace (kept between runs so repeated launches do not reallocate)
+	device_buffer b_counters, b_win_vals, b_win_keys, b_ent, b_fin_vals, b_fin_keys, b_fp, b_istate, b_pv;
+
+	std::shared_ptr<const model> mdl;
+	generated gen;
+	gen_options opts;
+	int W = 1;
+	hipModule_t mod_plain = nullptr;
+	hipFunction_t fn_plain = nullptr;
+	hipModule_t mod_param = nullptr;
+	hipFunction_t fn_param = nullptr;
+	double compile_seconds = 0;
+	bool no_cache = false;
+	int device = 0;
+	int num_cus = 0;
+
+	~program();
+	hipFunction_t function(bool param);
+};
+
+std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const gen_options& opt, bool no_cache);
+
+// Largest state-word count served by the lane-per-trajectory kernels.
+constexpr int max_lane_words = 8;
+
+struct sim_params
+{
+	int64_t n_traj = -1;
+	uint64_t first_id = 0;
+	int64_t seed = -1;
+	uint32_t variant = 0;
+	uint32_t max_steps = 0;
+	int window_mode = -1, final_mode = -1;
+	uint64_t window_capacity = 0, final_capacity = 0, fixed_capacity = 0;
+	int block_size = 0, blocks_per_cu = 0;
+	double normalization = 0;
+};
+
+struct result
+{
+	// run description
+	int windows = 0, n_obs = 0, n_words = 0, discrete = 0;
+	float tick = 0, max_time = 0;
+	double norm = 0;
+	uint64_t n_traj = 0;
+	std::vector<std::string> names;
+	std::vector<int> observed;
+
+	// statistics; raw values are fixed point (see simulate_lane.hip)
+	std::vector<std::vector<std::pair<uint64_t, uint64_t>>> win; // per window (state, raw), sorted
+	std::vector<uint64_t> ent;                                    // per window
+	std::vector<std::pair<uint64_t, uint64_t>> fin;               // (state, count), sorted
+	std::vector<std::pair<std::vector<uint32_t>, uint64_t>> fixed; // reference order
+
+	uint64_t transitions = 0, finished = 0, truncated = 0;
+	double kernel_ms = 0, total_ms = 0;
+	int window_mode = 0, final_mode = 0;
+	uint64_t window_capacity = 0;
+
+	double prob(uint64_t raw) const;     // window cell -> probability
+	double th(int w) const;              // transition entropy of window w
+	double h(int w) const;               // state entropy of window w (float, as the reference)
+	std::string state_name(uint64_t obs) const;
+	std::string full_state_name(const std::vector<uint32_t>& s) const;
+
+	void write_csv(const std::string& prefix, bool with_finals) const;
+	void print(std::ostream& os) const;
+	void write_raw(std::ostream& os) const;
+};
+
+std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t stream);
+
+} // namespace mbg

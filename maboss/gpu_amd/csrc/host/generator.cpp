@@ -1,0 +1,421 @@
+// Per-model HIP source generator.
+//
+// Follows src/generator.cpp: one rate per node (generate_node_transitions
+// :84-122, including the "fast form" of is_logic_ternary_expression :57-82),
+// the sequential total (generate_aggregate_function :124-144), the transition
+// entropy of the observed nodes (:146-175) and the observed-state index
+// (generate_non_internal_index :207-235).  Differences, all deliberate:
+//   * constants are emitted as exact hexadecimal floats instead of 6
+//     significant digits (src/expressions.cpp:185, :213; src/generator.cpp:23);
+//   * expressions are typed: booleans stay booleans, arithmetic is float, so
+//     every operator has the float semantics of the reference's evaluate()
+//     (src/parser/expressions.cpp:10-81) rather than C++ int/double promotion;
+//   * XOR is supported (bool inequality);
+//   * the flip selection (src/jit_kernels/simulation.cu:9-21) is emitted with
+//     the rates, reusing the same prefix sums as the total;
+//   * a second flavour reads the $variables from a runtime table and applies
+//     per-variant mutant masks (batched and differentiated forms).
+#include "generator.h"
+
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <set>
+#include <sstream>
+
+#include "../device/mbg_numeric.h"
+
+namespace mbg
+{
+
+float rate_log2_rate(float r) { return mbg_rlog2r(r); }
+
+std::string float_literal(float v)
+{
+	if (v == 0.0f)
+		return std::signbit(v) ? "(-0.0f)" : "0.0f";
+	char buf[64];
+	std::snprintf(buf, sizeof(buf), "%af", (double)v);
+	std::string s(buf);
+	if (s[0] == '-')
+		return "(" + s + ")";
+	return s;
+}
+
+namespace
+{
+
+enum class ty
+{
+	b,
+	f
+};
+
+struct code
+{
+	std::string s;
+	ty t;
+};
+
+const expr* strip_parens(const expr* e)
+{
+	while (e && e->kind == op::paren)
+		e = e->a.get();
+	return e;
+}
+
+bool node_free(const model& m, const node_t& cur, const expr& e, int depth = 0)
+{
+	if (depth > 64)
+		return false;
+	switch (e.kind)
+	{
+		case op::node: return false;
+		case op::alias: {
+			const expr* t = cur.attr(e.name.substr(1));
+			return t && node_free(m, cur, *t, depth + 1);
+		}
+		default: break;
+	}
+	for (const expr* c : { e.a.get(), e.b.get(), e.c.get() })
+		if (c && !node_free(m, cur, *c, depth))
+			return false;
+	return true;
+}
+
+class emitter
+{
+public:
+	emitter(const model& m, const node_t& cur, bool param, const std::map<std::string, int>& pidx)
+		: m_(m), cur_(cur), param_(param), pidx_(pidx)
+	{}
+
+	int literals = 0;
+
+	code nat(const expr& e)
+	{
+		switch (e.kind)
+		{
+			case op::lit: return { float_literal(e.value), ty::f };
+			case op::node: {
+				int i = m_.node_index(e.name);
+				if (i < 0)
+					throw error(MBG_E_MODEL, "unknown node name: " + e.name);
+				literals++;
+				return { "x" + std::to_string(i), ty::b };
+			}
+			case op::var: {
+				auto it = m_.variables.find(e.name);
+				if (it == m_.variables.end())
+					throw error(MBG_E_MODEL, "undefined variable " + e.name);
+				if (param_)
+					return { "pv->rate[" + std::to_string(pidx_.at(e.name)) + "]", ty::f };
+				return { float_literal(it->second), ty::f };
+			}
+			case op::alias: {
+				std::string an = e.name.substr(1);
+				const expr* t = cur_.attr(an);
+				if (!t)
+					throw error(MBG_E_MODEL, "Node " + cur_.name + " does not have attribute " + an);
+				if (stack_.count(an))
+					throw error(MBG_E_MODEL, "cyclic alias @" + an + " in node " + cur_.name);
+				stack_.insert(an);
+				code c = nat(*t);
+				stack_.erase(an);
+				return { "(" + c.s + ")", c.t };
+			}
+			case op::paren: {
+				code c = nat(*e.a);
+				return { "(" + c.s + ")", c.t };
+			}
+			case op::neg: return { "(-" + as_f(*e.a) + ")", ty::f };
+			case op::pos: return { as_f(*e.a), ty::f };
+			case op::lnot: return { "(!" + as_b(*e.a) + ")", ty::b };
+			case op::land: return { "((" + as_b(*e.a) + " & " + as_b(*e.b) + ") != 0)", ty::b };
+			case op::lor: return { "((" + as_b(*e.a) + " | " + as_b(*e.b) + ") != 0)", ty::b };
+			case op::lxor: return { "(" + as_b(*e.a) + " != " + as_b(*e.b) + ")", ty::b };
+			case op::eq:
+			case op::ne: {
+				code l = nat(*e.a), r = nat(*e.b);
+				const char* o = e.kind == op::eq ? " == " : " != ";
+				if (l.t == ty::b && r.t == ty::b)
+					return { "(" + l.s + o + r.s + ")", ty::b };
+				return { "(" + to_f(l) + o + to_f(r) + ")", ty::b };
+			}
+			case op::lt: return cmp(e, " < ");
+			case op::le: return cmp(e, " <= ");
+			case op::gt: return cmp(e, " > ");
+			case op::ge: return cmp(e, " >= ");
+			case op::add: return arith(e, " + ");
+			case op::sub: return arith(e, " - ");
+			case op::mul: return arith(e, " * ");
+			case op::div: return arith(e, " / ");
+			case op::ternary: {
+				std::string c = as_b(*e.a);
+				code l = nat(*e.b), r = nat(*e.c);
+				if (l.t == ty::b && r.t == ty::b)
+					return { "(" + c + " ? " + l.s + " : " + r.s + ")", ty::b };
+				return { "(" + c + " ? " + to_f(l) + " : " + to_f(r) + ")", ty::f };
+			}
+		}
+		throw error(MBG_E_INTERNAL, "bad expression");
+	}
+
+	std::string as_b(const expr& e) { return to_b(nat(e)); }
+	std::string as_f(const expr& e) { return to_f(nat(e)); }
+
+private:
+	static std::string to_b(const code& c) { return c.t == ty::b ? c.s : "(" + c.s + " != 0.0f)"; }
+	static std::string to_f(const code& c) { return c.t == ty::f ? c.s : "(" + c.s + " ? 1.0f : 0.0f)"; }
+
+	code cmp(const expr& e, const char* o) { return { "(" + as_f(*e.a) + o + as_f(*e.b) + ")", ty::b }; }
+	code arith(const expr& e, const char* o) { return { "(" + as_f(*e.a) + o + as_f(*e.b) + ")", ty::f }; }
+
+	const model& m_;
+	const node_t& cur_;
+	bool param_;
+	const std::map<std::string, int>& pidx_;
+	std::set<std::string> stack_;
+};
+
+// src/generator.cpp:57-82: `@logic ? V : 0` (up) or `@logic ? 0 : V` (down).
+bool logic_ternary(const model& m, const node_t& n, const expr* e, bool up, const expr*& val)
+{
+	if (!e || e->kind != op::ternary)
+		return false;
+	const expr* c = e->a.get();
+	if (c->kind != op::alias || c->name != "@logic" || !n.attr("logic"))
+		return false;
+	const expr* zero = up ? e->c.get() : e->b.get();
+	val = up ? e->b.get() : e->c.get();
+	if (!node_free(m, n, *zero) || !node_free(m, n, *val))
+		return false;
+	try
+	{
+		if (m.evaluate(*zero) != 0.0f)
+			return false;
+	}
+	catch (const error&)
+	{
+		return false;
+	}
+	return true;
+}
+
+int bare_param(const expr* e, const std::map<std::string, int>& pidx)
+{
+	e = strip_parens(e);
+	if (e && e->kind == op::var)
+	{
+		auto it = pidx.find(e->name);
+		if (it != pidx.end())
+			return it->second;
+	}
+	return -1;
+}
+
+void collect_vars(const model& m, const node_t& n, const expr& e, std::set<std::string>& out, int depth = 0)
+{
+	if (depth > 64)
+		return;
+	if (e.kind == op::var)
+		out.insert(e.name);
+	if (e.kind == op::alias)
+	{
+		if (const expr* t = n.attr(e.name.substr(1)))
+			collect_vars(m, n, *t, out, depth + 1);
+		return;
+	}
+	for (const expr* c : { e.a.get(), e.b.get(), e.c.get() })
+		if (c)
+			collect_vars(m, n, *c, out, depth);
+}
+
+void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, bool param,
+			   const std::map<std::string, int>& pidx, generated& g)
+{
+	const int N = m.n_nodes();
+	const auto obs = m.observed_nodes();
+	std::vector<bool> is_obs(N, false);
+	for (int i : obs)
+		is_obs[i] = true;
+
+	if (param)
+		os << "extern \"C\" __device__ void mbg_eval_p(const uint32_t* __restrict__ s, float u2, "
+			  "const mbg_param_view* __restrict__ pv, mbg_eval_t* __restrict__ out)\n{\n";
+	else
+		os << "extern \"C\" __device__ void mbg_eval(const uint32_t* __restrict__ s, float u2, "
+			  "mbg_eval_t* __restrict__ out)\n{\n";
+
+	for (int i = 0; i < N; i++)
+		os << "\tconst bool x" << i << " = ((s[" << i / 32 << "] >> " << i % 32 << ") & 1u) != 0u;\n";
+	if (param)
+	{
+		for (int w = 0; w < m.n_words(); w++)
+			os << "\tconst uint32_t ue" << w << " = pv->up_en[" << w << "], de" << w << " = pv->dn_en[" << w << "];\n";
+	}
+
+	for (int i = 0; i < N; i++)
+	{
+		const node_t& n = m.nodes[i];
+		emitter em(m, n, param, pidx);
+		const expr *U = nullptr, *D = nullptr;
+		const bool fast = logic_ternary(m, n, n.attr("rate_up"), true, U)
+						  && logic_ternary(m, n, n.attr("rate_down"), false, D);
+		node_info& ni = g.nodes[i];
+		ni.fast_form = fast;
+		ni.up_param = fast ? bare_param(U, pidx) : -1;
+		ni.down_param = fast ? bare_param(D, pidx) : -1;
+		os << "\t// node " << i << ": " << n.name << (is_obs[i] ? " (observed)" : "") << "\n";
+		os << "\tfloat r" << i << ";\n\t{\n";
+		std::string u_code, d_code;
+		if (fast)
+		{
+			std::string logic = em.as_b(*n.attr("logic"));
+			u_code = em.as_f(*U);
+			d_code = em.as_f(*D);
+			os << "\t\tconst bool L = " << logic << ";\n";
+			os << "\t\tr" << i << " = (x" << i << " == L) ? 0.0f : (x" << i << " ? " << d_code << " : " << u_code
+			   << ");\n";
+		}
+		else
+		{
+			std::string rd = em.as_f(*n.attr("rate_down"));
+			std::string ru = em.as_f(*n.attr("rate_up"));
+			os << "\t\tr" << i << " = x" << i << " ? " << rd << " : " << ru << ";\n";
+		}
+		if (param)
+		{
+			const int w = i / 32, b = i % 32;
+			os << "\t\tconst bool en = ((x" << i << " ? de" << w << " : ue" << w << ") >> " << b << ") & 1u;\n";
+			os << "\t\tr" << i << " = en ? r" << i << " : 0.0f;\n";
+		}
+		os << "\t}\n";
+		ni.literals = em.literals;
+		g.total_literals += em.literals;
+
+		if (is_obs[i] && opt.entropy == entropy_formula::log_sum)
+		{
+			// r*log2(r) of the selected rate; constant rates are folded here
+			// with the same float code the device uses.
+			os << "\tfloat l" << i << ";\n";
+			if (fast)
+			{
+				auto lr_of = [&](const expr* v, int pidx_v) -> std::string {
+					if (param)
+					{
+						if (pidx_v >= 0)
+							return "pv->rate_lr[" + std::to_string(pidx_v) + "]";
+						emitter e2(m, n, true, pidx);
+						return "mbg_rlog2r(" + e2.as_f(*v) + ")";
+					}
+					return float_literal(rate_log2_rate(m.evaluate(*v)));
+				};
+				os << "\tl" << i << " = (r" << i << " == 0.0f) ? 0.0f : (x" << i << " ? " << lr_of(D, ni.down_param)
+				   << " : " << lr_of(U, ni.up_param) << ");\n";
+			}
+			else
+				os << "\tl" << i << " = mbg_rlog2r(r" << i << ");\n";
+		}
+	}
+
+	// Total and prefix sums in node order (src/generator.cpp:124-144).
+	os << "\tfloat acc = 0.0f;\n";
+	for (int i = 0; i < N; i++)
+		os << "\tacc = acc + r" << i << "; const float c" << i << " = acc;\n";
+	os << "\tconst float total = acc;\n";
+	// Flip selection (src/jit_kernels/simulation.cu:9-21): number of prefix
+	// sums strictly below u2 * total.
+	os << "\tconst float rr = u2 * total;\n\tint flip = 0;\n";
+	for (int i = 0; i < N; i++)
+		os << "\tflip += (c" << i << " < rr) ? 1 : 0;\n";
+
+	// Transition entropy of the observed nodes.
+	os << "\tfloat H = 0.0f;\n";
+	if (!obs.empty())
+	{
+		if (opt.entropy == entropy_formula::log_sum)
+		{
+			os << "\tfloat S = 0.0f, T = 0.0f;\n\tint nnz = 0;\n";
+			for (int i : obs)
+				os << "\tS = S + r" << i << "; T = T + l" << i << "; nnz += (r" << i << " != 0.0f) ? 1 : 0;\n";
+			os << "\t{\n\t\tconst float h = mbg_log2f(S) - T / S;\n";
+			os << "\t\tH = (nnz > 1 && h > 0.0f) ? h : 0.0f;\n\t}\n";
+		}
+		else
+		{
+			os << "\tfloat S = 0.0f;\n";
+			for (int i : obs)
+				os << "\tS = S + r" << i << ";\n";
+			os << "\tif (S != 0.0f)\n\t{\n\t\tfloat p;\n";
+			for (int i : obs)
+				os << "\t\tp = r" << i << " / S; H = H - ((p == 0.0f) ? 0.0f : mbg_log2f(p) * p);\n";
+			os << "\t}\n";
+		}
+	}
+	os << "\tout->total = total;\n\tout->H = H;\n\tout->flip = flip;\n\tout->pad = 0;\n}\n\n";
+}
+
+void emit_obs_index(std::ostringstream& os, const model& m)
+{
+	const auto obs = m.observed_nodes();
+	os << "extern \"C\" __device__ uint64_t mbg_obs_index(const uint32_t* __restrict__ s)\n{\n\tuint64_t v = 0;\n";
+	// Runs of consecutive observed nodes inside one word are extracted with a
+	// single mask and shift (a parallel bit-extract in node order).
+	size_t j = 0;
+	while (j < obs.size())
+	{
+		int i0 = obs[j];
+		size_t k = j + 1;
+		while (k < obs.size() && obs[k] == obs[k - 1] + 1 && obs[k] / 32 == i0 / 32)
+			k++;
+		int len = (int)(k - j);
+		uint32_t mask = len == 32 ? 0xffffffffu : ((1u << len) - 1u);
+		os << "\tv |= (uint64_t)((s[" << i0 / 32 << "] >> " << i0 % 32 << ") & " << mask << "u) << " << j << ";\n";
+		j = k;
+	}
+	os << "\treturn v;\n}\n";
+}
+
+} // namespace
+
+generated generate(const model& m, const gen_options& opt)
+{
+	generated g;
+	const int N = m.n_nodes();
+	g.nodes.resize(N);
+
+	std::set<std::string> used;
+	for (auto& n : m.nodes)
+		for (const char* a : { "rate_up", "rate_down" })
+			if (const expr* e = n.attr(a))
+				collect_vars(m, n, *e, used);
+	std::map<std::string, int> pidx;
+	for (auto& v : used)
+	{
+		pidx[v] = (int)g.param_names.size();
+		g.param_names.push_back(v);
+	}
+
+	std::ostringstream os;
+	os << "// Generated by the MaBoSS MI355X generator: " << N << " nodes, " << m.n_words() << " state words, "
+	   << m.n_observed() << " observed nodes.\n";
+	os << "#include \"mbg_abi.h\"\n\n";
+	os << "#define MBG_N " << N << "\n#define MBG_W " << m.n_words() << "\n#define MBG_NOBS " << m.n_observed()
+	   << "\n\n";
+	emit_eval(os, m, opt, false, pidx, g);
+	// the parametric flavour does not recount literals
+	generated tmp;
+	tmp.nodes.resize(N);
+	emit_eval(os, m, opt, true, pidx, tmp);
+	for (int i = 0; i < N; i++)
+	{
+		g.nodes[i].up_param = tmp.nodes[i].up_param;
+		g.nodes[i].down_param = tmp.nodes[i].down_param;
+	}
+	emit_obs_index(os, m);
+	g.source = os.str();
+	return g;
+}
+
+} // namespace mbg

@@ -1,0 +1,252 @@
+// Kernel compiler: hipRTC compile of the generated source, link against the
+// precompiled kernel bitcode, load as a module.  Replaces
+// kernel_compiler::compile_simulation (src/kernel_compiler.cpp:61-164), which
+// used NVRTC (-dlto -rdc) + nvJitLink against embedded LTO fatbins.
+//
+// The link is whole-program: hiprtcLink merges the two LLVM bitcode modules
+// and optimises them together at -O3, so the generated rate code is inlined
+// into the kernel loop.  Linked code objects are cached on disk, keyed by a
+// hash of the generated source, the compile options and the kernel bitcode.
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include "engine.h"
+#include "timer.h"
+
+extern "C" {
+extern const char mbg_hdr_numeric[];
+extern const char mbg_hdr_abi[];
+}
+
+namespace mbg
+{
+
+// Kernel bitcode embedded at build time (see csrc/Makefile), one module per
+// (state words, parametric) pair.
+struct embedded_bc
+{
+	const unsigned char* begin;
+	const unsigned char* end;
+};
+embedded_bc kernel_bitcode(int words, bool param);
+
+void hip_check(hipError_t e, const char* what)
+{
+	if (e != hipSuccess)
+		throw error(MBG_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace
+{
+
+const char* const k_compile_flags[] = { "--offload-arch=gfx950",
+										"-fgpu-rdc",
+										"-O3",
+										"-std=c++17",
+										"-ffp-contract=off",
+										"-fno-gpu-flush-denormals-to-zero",
+										"-fhip-fp32-correctly-rounded-divide-sqrt",
+										"-Xclang",
+										"-target-feature",
+										"-Xclang",
+										"-packed-fp32-ops" };
+
+uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull)
+{
+	const unsigned char* p = (const unsigned char*)data;
+	for (size_t i = 0; i < n; i++)
+	{
+		h ^= p[i];
+		h *= 1099511628211ull;
+	}
+	return h;
+}
+
+std::string cache_dir()
+{
+	if (const char* d = std::getenv("MBG_CACHE_DIR"))
+		return d;
+	if (const char* h = std::getenv("HOME"))
+		return std::string(h) + "/.cache/maboss_gpu_amd";
+	return "/tmp/maboss_gpu_amd_cache";
+}
+
+void mkdirs(const std::string& path)
+{
+	std::string cur;
+	std::stringstream ss(path);
+	std::string part;
+	if (!path.empty() && path[0] == '/')
+		cur = "/";
+	while (std::getline(ss, part, '/'))
+	{
+		if (part.empty())
+			continue;
+		cur += part + "/";
+		::mkdir(cur.c_str(), 0755);
+	}
+}
+
+bool read_all(const std::string& path, std::vector<char>& out)
+{
+	std::ifstream f(path, std::ios::binary);
+	if (!f)
+		return false;
+	out.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+	return !out.empty();
+}
+
+void write_atomic(const std::string& path, const void* data, size_t n)
+{
+	std::string tmp = path + ".tmp" + std::to_string(::getpid());
+	{
+		std::ofstream f(tmp, std::ios::binary);
+		if (!f)
+			return;
+		f.write((const char*)data, (std::streamsize)n);
+		if (!f)
+			return;
+	}
+	std::rename(tmp.c_str(), path.c_str());
+}
+
+void rtc_check(hiprtcResult r, const char* what)
+{
+	if (r != HIPRTC_SUCCESS)
+		throw error(MBG_E_COMPILE, std::string(what) + ": " + hiprtcGetErrorString(r));
+}
+
+std::vector<char> compile_to_bitcode(const std::string& source)
+{
+	const char* headers[] = { mbg_hdr_numeric, mbg_hdr_abi };
+	const char* names[] = { "mbg_numeric.h", "mbg_abi.h" };
+	hiprtcProgram prog;
+	rtc_check(hiprtcCreateProgram(&prog, source.c_str(), "mbg_model.hip", 2, headers, names), "hiprtcCreateProgram");
+	hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(k_compile_flags) / sizeof(k_compile_flags[0])),
+										  k_compile_flags);
+	size_t log_size = 0;
+	hiprtcGetProgramLogSize(prog, &log_size);
+	std::string log(log_size, '\0');
+	if (log_size)
+		hiprtcGetProgramLog(prog, &log[0]);
+	if (r != HIPRTC_SUCCESS)
+	{
+		hiprtcDestroyProgram(&prog);
+		throw error(MBG_E_COMPILE, std::string("hipRTC compile failed: ") + hiprtcGetErrorString(r) + "\n" + log);
+	}
+	size_t bc_size = 0;
+	rtc_check(hiprtcGetBitcodeSize(prog, &bc_size), "hiprtcGetBitcodeSize");
+	std::vector<char> bc(bc_size);
+	rtc_check(hiprtcGetBitcode(prog, bc.data()), "hiprtcGetBitcode");
+	hiprtcDestroyProgram(&prog);
+	return bc;
+}
+
+std::vector<char> link_code_object(std::vector<char>& model_bc, const embedded_bc& kbc)
+{
+	hiprtcLinkState ls;
+	const char* isa_opts[] = { "-O3" };
+	hiprtcJIT_option lopts[] = { HIPRTC_JIT_IR_TO_ISA_OPT_EXT, HIPRTC_JIT_IR_TO_ISA_OPT_COUNT_EXT };
+	void* lvals[] = { (void*)isa_opts, (void*)(uintptr_t)1 };
+	rtc_check(hiprtcLinkCreate(2, lopts, lvals, &ls), "hiprtcLinkCreate");
+	hiprtcResult r = hiprtcLinkAddData(ls, HIPRTC_JIT_INPUT_LLVM_BITCODE, model_bc.data(), model_bc.size(),
+									   "mbg_model", 0, nullptr, nullptr);
+	if (r == HIPRTC_SUCCESS)
+		r = hiprtcLinkAddData(ls, HIPRTC_JIT_INPUT_LLVM_BITCODE, (void*)kbc.begin, (size_t)(kbc.end - kbc.begin),
+							  "mbg_kernels", 0, nullptr, nullptr);
+	void* bin = nullptr;
+	size_t bin_size = 0;
+	if (r == HIPRTC_SUCCESS)
+		r = hiprtcLinkComplete(ls, &bin, &bin_size);
+	std::vector<char> out;
+	if (r == HIPRTC_SUCCESS)
+		out.assign((const char*)bin, (const char*)bin + bin_size);
+	hiprtcLinkDestroy(ls);
+	if (r != HIPRTC_SUCCESS)
+		throw error(MBG_E_COMPILE, std::string("hipRTC link failed: ") + hiprtcGetErrorString(r));
+	return out;
+}
+
+} // namespace
+
+program::~program()
+{
+	if (mod_plain)
+		(void)hipModuleUnload(mod_plain);
+	if (mod_param)
+		(void)hipModuleUnload(mod_param);
+}
+
+hipFunction_t program::function(bool param)
+{
+	hipFunction_t& fn = param ? fn_param : fn_plain;
+	if (fn)
+		return fn;
+	hipModule_t& mod = param ? mod_param : mod_plain;
+
+	scope_timer tm("compiler> whole_compilation");
+	const double t0 = wall_seconds();
+	const embedded_bc kbc = kernel_bitcode(W, param);
+	if (!kbc.begin || kbc.end <= kbc.begin)
+		throw error(MBG_E_INTERNAL, "no precompiled kernel for " + std::to_string(W) + " state words");
+
+	uint64_t h = fnv1a(gen.source.data(), gen.source.size());
+	for (const char* f : k_compile_flags)
+		h = fnv1a(f, std::strlen(f), h);
+	h = fnv1a(kbc.begin, (size_t)(kbc.end - kbc.begin), h);
+	h = fnv1a(mbg_hdr_numeric, std::strlen(mbg_hdr_numeric), h);
+	h = fnv1a(mbg_hdr_abi, std::strlen(mbg_hdr_abi), h);
+	char name[64];
+	std::snprintf(name, sizeof(name), "%016llx_%d.co", (unsigned long long)h, param ? 1 : 0);
+	const std::string dir = cache_dir();
+	const std::string path = dir + "/" + name;
+
+	std::vector<char> co;
+	if (no_cache || !read_all(path, co))
+	{
+		std::vector<char> bc = compile_to_bitcode(gen.source);
+		co = link_code_object(bc, kbc);
+		if (!no_cache)
+		{
+			mkdirs(dir);
+			write_atomic(path, co.data(), co.size());
+		}
+	}
+	hip_check(hipModuleLoadData(&mod, co.data()), "hipModuleLoadData");
+	hip_check(hipModuleGetFunction(&fn, mod, "simulate"), "hipModuleGetFunction(simulate)");
+	compile_seconds += wall_seconds() - t0;
+	return fn;
+}
+
+std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const gen_options& opt, bool no_cache)
+{
+	auto p = std::make_unique<program>();
+	p->W = m->n_words();
+	if (p->W > max_lane_words)
+		throw error(MBG_E_LIMIT, "models with more than " + std::to_string(32 * max_lane_words)
+									 + " nodes are not supported by this build");
+	{
+		scope_timer tm("generator> generate");
+		p->gen = generate(*m, opt);
+	}
+	p->mdl = std::move(m);
+	p->opts = opt;
+	p->no_cache = no_cache;
+	hip_check(hipGetDevice(&p->device), "hipGetDevice");
+	hipDeviceProp_t prop;
+	hip_check(hipGetDeviceProperties(&prop, p->device), "hipGetDeviceProperties");
+	if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+		throw error(MBG_E_HIP, std::string("this build targets gfx950; current device is ") + prop.gcnArchName);
+	p->num_cus = prop.multiProcessorCount;
+	p->function(false);
+	return p;
+}
+
+} // namespace mbg

@@ -1,0 +1,392 @@
+// Simulation runner: sizes and allocates the statistics storage, launches the
+// fused persistent simulate kernel and gathers the results.
+//
+// Replaces simulation_runner::run_simulation (src/simulation_runner.cu:54-180)
+// and the per-batch stats fan-out (src/statistics/stats_composite.cpp:5-12).
+// There are no epochs: the reference's batches of at most 1e6 trajectories x
+// 100 records (:50-51), its partitions/refills (:129-158) and its per-epoch
+// memset of 400 MB (:161) are replaced by one launch in which lanes pull
+// trajectory ids from a queue until all are done.  A consequence worth
+// stating: the reference re-draws the dwell time (without flipping) of every
+// trajectory still running at an epoch boundary; here the flip always follows
+// its own dwell.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "../device/mbg_kernel_args.h"
+#include "engine.h"
+#include "timer.h"
+
+namespace mbg
+{
+
+void* device_buffer::ensure(size_t n)
+{
+	if (n <= bytes && ptr)
+		return ptr;
+	release();
+	size_t want = std::max<size_t>(n, 256);
+	hip_check(hipMalloc(&ptr, want), "hipMalloc");
+	bytes = want;
+	return ptr;
+}
+
+void device_buffer::release()
+{
+	if (ptr)
+		(void)hipFree(ptr);
+	ptr = nullptr;
+	bytes = 0;
+}
+
+namespace
+{
+
+uint64_t next_pow2(uint64_t v)
+{
+	uint64_t p = 1;
+	while (p < v)
+		p <<= 1;
+	return p;
+}
+
+int bits_for(uint64_t v)
+{
+	int b = 0;
+	while ((1ull << b) <= v && b < 64)
+		b++;
+	return b;
+}
+
+constexpr size_t k_lds_budget = 48 * 1024;
+constexpr uint64_t k_dense_limit = 8ull << 30;
+
+struct plan
+{
+	int win_mode, fin_mode, ent_lds;
+	int win_off = 0, fin_off = 0, ent_off = 0, lds_bytes = 0;
+	uint64_t win_cells = 0, fin_cells = 0, win_cap = 0, fin_cap = 0, fp_cap = 0;
+};
+
+plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int windows, int n_obs)
+{
+	plan p;
+	const uint64_t win_dense_bytes = n_obs <= 40 ? ((uint64_t)windows << n_obs) * 8 : ~0ull;
+	const uint64_t fin_dense_bytes = n_obs <= 40 ? (1ull << n_obs) * 8 : ~0ull;
+	size_t lds = 0;
+
+	p.ent_lds = (size_t)windows * 8 <= 16 * 1024;
+	if (p.ent_lds)
+	{
+		p.ent_off = 0;
+		lds += (size_t)windows * 8;
+	}
+
+	p.win_mode = sp.window_mode;
+	if (p.win_mode < 0)
+		p.win_mode = win_dense_bytes <= 32 * 1024 ? MBG_HIST_LDS
+					 : win_dense_bytes <= k_dense_limit ? MBG_HIST_DENSE
+														: MBG_HIST_HASH;
+	if (p.win_mode != MBG_HIST_HASH && win_dense_bytes > k_dense_limit)
+		throw error(MBG_E_LIMIT, "dense window histogram too large; use the hashed mode");
+	if (p.win_mode == MBG_HIST_LDS)
+	{
+		if (win_dense_bytes + lds > k_lds_budget)
+			throw error(MBG_E_LIMIT, "window histogram does not fit in LDS");
+		p.win_off = (int)(lds / 8);
+		lds += win_dense_bytes;
+	}
+
+	p.fin_mode = sp.final_mode;
+	if (p.fin_mode < 0)
+		p.fin_mode = fin_dense_bytes + lds <= k_lds_budget ? MBG_HIST_LDS
+					 : fin_dense_bytes <= (1ull << 30)	 ? MBG_HIST_DENSE
+														 : MBG_HIST_HASH;
+	if (p.fin_mode != MBG_HIST_HASH && fin_dense_bytes > k_dense_limit)
+		throw error(MBG_E_LIMIT, "dense final-state histogram too large; use the hashed mode");
+	if (p.fin_mode == MBG_HIST_LDS)
+	{
+		if (fin_dense_bytes + lds > k_lds_budget)
+			throw error(MBG_E_LIMIT, "final-state histogram does not fit in LDS");
+		p.fin_off = (int)(lds / 8);
+		lds += fin_dense_bytes;
+	}
+	p.lds_bytes = (int)lds;
+
+	if (p.win_mode == MBG_HIST_HASH)
+	{
+		if (n_obs + bits_for((uint64_t)windows) > 63)
+			throw error(MBG_E_LIMIT, "observed nodes plus window bits exceed the 63-bit hash key");
+		p.win_cap = sp.window_capacity ? next_pow2(sp.window_capacity)
+									   : next_pow2(std::min<uint64_t>(std::max<uint64_t>(n_traj * windows / 2, 1 << 16),
+																	   1ull << 28));
+	}
+	else
+		p.win_cells = (uint64_t)windows << n_obs;
+	if (p.fin_mode == MBG_HIST_HASH)
+	{
+		if (n_obs > 63)
+			throw error(MBG_E_LIMIT, "more than 63 observed nodes");
+		p.fin_cap = sp.final_capacity ? next_pow2(sp.final_capacity)
+									  : next_pow2(std::min<uint64_t>(std::max<uint64_t>(2 * n_traj, 1 << 12), 1ull << 27));
+	}
+	else
+		p.fin_cells = 1ull << n_obs;
+	p.fp_cap = sp.fixed_capacity ? next_pow2(sp.fixed_capacity)
+								 : next_pow2(std::min<uint64_t>(std::max<uint64_t>(2 * n_traj, 1 << 12), 1ull << 24));
+	(void)m;
+	return p;
+}
+
+} // namespace
+
+std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t stream)
+{
+	scope_timer tm("runner> simulate");
+	const double t0 = wall_seconds();
+	const model& m = *p.mdl;
+	auto res = std::make_unique<result>();
+	const int W = m.n_words();
+	const int n_obs = m.n_observed();
+	const int windows = m.windows();
+	const uint64_t n_traj = sp.n_traj < 0 ? (uint64_t)m.sample_count() : (uint64_t)sp.n_traj;
+	const uint64_t seed = sp.seed < 0 ? m.seed() : (uint64_t)sp.seed;
+	if (windows <= 0 || windows > (1 << 24))
+		throw error(MBG_E_LIMIT, "window count out of range");
+
+	res->windows = windows;
+	res->n_obs = n_obs;
+	res->n_words = W;
+	res->discrete = m.discrete_time();
+	res->tick = m.time_tick();
+	res->max_time = m.max_time();
+	res->n_traj = n_traj;
+	res->norm = sp.normalization > 0 ? sp.normalization : (double)n_traj;
+	for (auto& n : m.nodes)
+		res->names.push_back(n.name);
+	res->observed = m.observed_nodes();
+
+	hipFunction_t fn = p.function(false);
+	plan pl = make_plan(m, sp, n_traj, windows, n_obs);
+
+	for (int attempt = 0;; attempt++)
+	{
+		// ---- storage
+		auto* counters = (unsigned long long*)p.b_counters.ensure(MBG_NCOUNTERS * 8);
+		hip_check(hipMemsetAsync(counters, 0, MBG_NCOUNTERS * 8, stream), "hipMemsetAsync");
+		auto* ent = (unsigned long long*)p.b_ent.ensure((size_t)windows * 8);
+		hip_check(hipMemsetAsync(ent, 0, (size_t)windows * 8, stream), "hipMemsetAsync");
+		unsigned long long *win_vals = nullptr, *win_keys = nullptr, *fin_vals = nullptr, *fin_keys = nullptr;
+		if (pl.win_mode == MBG_HIST_HASH)
+		{
+			win_vals = (unsigned long long*)p.b_win_vals.ensure(pl.win_cap * 8);
+			win_keys = (unsigned long long*)p.b_win_keys.ensure(pl.win_cap * 8);
+			hip_check(hipMemsetAsync(win_vals, 0, pl.win_cap * 8, stream), "hipMemsetAsync");
+			hip_check(hipMemsetAsync(win_keys, 0xff, pl.win_cap * 8, stream), "hipMemsetAsync");
+		}
+		else
+		{
+			win_vals = (unsigned long long*)p.b_win_vals.ensure(pl.win_cells * 8);
+			hip_check(hipMemsetAsync(win_vals, 0, pl.win_cells * 8, stream), "hipMemsetAsync");
+		}
+		if (pl.fin_mode == MBG_HIST_HASH)
+		{
+			fin_vals = (unsigned long long*)p.b_fin_vals.ensure(pl.fin_cap * 8);
+			fin_keys = (unsigned long long*)p.b_fin_keys.ensure(pl.fin_cap * 8);
+			hip_check(hipMemsetAsync(fin_vals, 0, pl.fin_cap * 8, stream), "hipMemsetAsync");
+			hip_check(hipMemsetAsync(fin_keys, 0xff, pl.fin_cap * 8, stream), "hipMemsetAsync");
+		}
+		else
+		{
+			fin_vals = (unsigned long long*)p.b_fin_vals.ensure(pl.fin_cells * 8);
+			hip_check(hipMemsetAsync(fin_vals, 0, pl.fin_cells * 8, stream), "hipMemsetAsync");
+		}
+		const size_t fp_stride = 2 + (size_t)W;
+		auto* fp = (unsigned int*)p.b_fp.ensure(pl.fp_cap * fp_stride * 4);
+		hip_check(hipMemsetAsync(fp, 0, pl.fp_cap * fp_stride * 4, stream), "hipMemsetAsync");
+
+		std::vector<float> istate;
+		for (auto& n : m.nodes)
+			istate.push_back(n.istate);
+		auto* d_istate = (float*)p.b_istate.ensure(istate.size() * 4);
+		hip_check(hipMemcpyAsync(d_istate, istate.data(), istate.size() * 4, hipMemcpyHostToDevice, stream),
+				  "hipMemcpyAsync");
+
+		// ---- kernel arguments
+		mbg_sim_args a;
+		std::memset(&a, 0, sizeof(a));
+		a.n_traj = n_traj;
+		a.id_base = sp.first_id;
+		a.seed_lo = (uint32_t)seed;
+		a.seed_hi = (uint32_t)(seed >> 32);
+		a.variant = sp.variant;
+		a.max_steps = sp.max_steps ? sp.max_steps : (1u << 30);
+		a.n_nodes = m.n_nodes();
+		a.n_obs = n_obs;
+		a.windows = windows;
+		a.discrete = m.discrete_time();
+		a.max_time = m.max_time();
+		a.tick = m.time_tick();
+		a.qscale = a.discrete ? 0x1p32f : 0x1p32f / a.tick;
+		a.istate = d_istate;
+		a.win_mode = pl.win_mode;
+		a.fin_mode = pl.fin_mode;
+		a.ent_lds = pl.ent_lds;
+		a.lds_bytes = pl.lds_bytes;
+		a.win_lds_off = pl.win_off;
+		a.fin_lds_off = pl.fin_off;
+		a.ent_lds_off = pl.ent_off;
+		a.win_vals = win_vals;
+		a.win_keys = win_keys;
+		a.win_mask = pl.win_cap ? pl.win_cap - 1 : 0;
+		a.win_ent = ent;
+		a.fin_vals = fin_vals;
+		a.fin_keys = fin_keys;
+		a.fin_mask = pl.fin_cap ? pl.fin_cap - 1 : 0;
+		a.fp_table = fp;
+		a.fp_mask = pl.fp_cap - 1;
+		a.counters = counters;
+		a.pv = nullptr;
+
+		// ---- launch: a persistent grid filling every CU
+		const int block = sp.block_size > 0 ? sp.block_size : 256;
+		int per_cu = sp.blocks_per_cu;
+		if (per_cu <= 0)
+		{
+			hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, pl.lds_bytes),
+					  "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+			per_cu = std::max(per_cu, 1);
+		}
+		uint64_t lanes_needed = (n_traj + block - 1) / block;
+		uint64_t grid = std::min<uint64_t>((uint64_t)p.num_cus * per_cu, std::max<uint64_t>(lanes_needed, 1));
+
+		hipEvent_t e0, e1;
+		hip_check(hipEventCreate(&e0), "hipEventCreate");
+		hip_check(hipEventCreate(&e1), "hipEventCreate");
+		hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+		if (n_traj > 0)
+		{
+			void* args[] = { &a };
+			hip_check(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, block, 1, 1, pl.lds_bytes, stream, args, nullptr),
+					  "hipModuleLaunchKernel(simulate)");
+		}
+		hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+
+		// ---- gather
+		std::vector<unsigned long long> ctr(MBG_NCOUNTERS);
+		hip_check(hipMemcpyAsync(ctr.data(), counters, MBG_NCOUNTERS * 8, hipMemcpyDeviceToHost, stream),
+				  "hipMemcpyAsync");
+		hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+		float ms = 0;
+		hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+
+		const bool win_over = ctr[MBG_CTR_WIN_OVERFLOW] != 0, fin_over = ctr[MBG_CTR_FIN_OVERFLOW] != 0,
+				   fp_over = ctr[MBG_CTR_FP_OVERFLOW] != 0;
+		if (ctr[MBG_CTR_FP_TIMEOUT])
+			throw error(MBG_E_INTERNAL, "fixed-point table: a slot was never published");
+		if (win_over || fin_over || fp_over)
+		{
+			if (attempt >= 6)
+				throw error(MBG_E_OVERFLOW, "device hash table overflow after resizing");
+			if (win_over)
+			{
+				if (sp.window_capacity)
+					throw error(MBG_E_OVERFLOW, "window hash table overflow; raise window_capacity");
+				pl.win_cap *= 2;
+			}
+			if (fin_over)
+			{
+				if (sp.final_capacity)
+					throw error(MBG_E_OVERFLOW, "final-state hash table overflow; raise final_capacity");
+				pl.fin_cap *= 2;
+			}
+			if (fp_over)
+			{
+				if (sp.fixed_capacity)
+					throw error(MBG_E_OVERFLOW, "fixed-point table overflow; raise fixed_capacity");
+				pl.fp_cap *= 2;
+			}
+			continue;
+		}
+
+		res->kernel_ms = ms;
+		res->transitions = ctr[MBG_CTR_TRANSITIONS];
+		res->finished = ctr[MBG_CTR_FINISHED];
+		res->truncated = ctr[MBG_CTR_TRUNCATED];
+		res->window_mode = pl.win_mode;
+		res->final_mode = pl.fin_mode;
+		res->window_capacity = pl.win_cap;
+
+		res->ent.resize(windows);
+		hip_check(hipMemcpy(res->ent.data(), ent, (size_t)windows * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+
+		res->win.assign(windows, {});
+		if (pl.win_mode == MBG_HIST_HASH)
+		{
+			std::vector<unsigned long long> keys(pl.win_cap), vals(pl.win_cap);
+			hip_check(hipMemcpy(keys.data(), win_keys, pl.win_cap * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+			hip_check(hipMemcpy(vals.data(), win_vals, pl.win_cap * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+			const uint64_t omask = n_obs >= 64 ? ~0ull : ((1ull << n_obs) - 1);
+			for (uint64_t i = 0; i < pl.win_cap; i++)
+				if (keys[i] != MBG_EMPTY_KEY && vals[i])
+					res->win[keys[i] >> n_obs].emplace_back(keys[i] & omask, vals[i]);
+			for (auto& v : res->win)
+				std::sort(v.begin(), v.end());
+		}
+		else
+		{
+			std::vector<unsigned long long> vals(pl.win_cells);
+			hip_check(hipMemcpy(vals.data(), win_vals, pl.win_cells * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+			const uint64_t per = 1ull << n_obs;
+			for (int w = 0; w < windows; w++)
+				for (uint64_t s = 0; s < per; s++)
+					if (uint64_t v = vals[(uint64_t)w * per + s])
+						res->win[w].emplace_back(s, v);
+		}
+
+		if (pl.fin_mode == MBG_HIST_HASH)
+		{
+			std::vector<unsigned long long> keys(pl.fin_cap), vals(pl.fin_cap);
+			hip_check(hipMemcpy(keys.data(), fin_keys, pl.fin_cap * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+			hip_check(hipMemcpy(vals.data(), fin_vals, pl.fin_cap * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+			for (uint64_t i = 0; i < pl.fin_cap; i++)
+				if (keys[i] != MBG_EMPTY_KEY && vals[i])
+					res->fin.emplace_back(keys[i], vals[i]);
+			std::sort(res->fin.begin(), res->fin.end());
+		}
+		else
+		{
+			std::vector<unsigned long long> vals(pl.fin_cells);
+			hip_check(hipMemcpy(vals.data(), fin_vals, pl.fin_cells * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+			for (uint64_t s = 0; s < pl.fin_cells; s++)
+				if (vals[s])
+					res->fin.emplace_back(s, vals[s]);
+		}
+
+		{
+			std::vector<unsigned int> tab(pl.fp_cap * fp_stride);
+			hip_check(hipMemcpy(tab.data(), fp, tab.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+			for (uint64_t i = 0; i < pl.fp_cap; i++)
+			{
+				const unsigned int* e = &tab[i * fp_stride];
+				if (e[0] == 2u && e[1])
+					res->fixed.emplace_back(std::vector<uint32_t>(e + 2, e + 2 + W), e[1]);
+			}
+			// reference order: most significant word first (src/state.h:43-49)
+			std::sort(res->fixed.begin(), res->fixed.end(), [](const auto& x, const auto& y) {
+				for (int w = (int)x.first.size() - 1; w >= 0; w--)
+					if (x.first[w] != y.first[w])
+						return x.first[w] < y.first[w];
+				return false;
+			});
+		}
+		break;
+	}
+	res->total_ms = (wall_seconds() - t0) * 1e3;
+	return res;
+}
+
+} // namespace mbg

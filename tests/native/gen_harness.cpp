@@ -1,0 +1,30 @@
+// Test harness: runs a model's generated HIP source on the host.  The file is
+// compiled with -DGEN_FILE="path" and -I to the device headers; __device__ is
+// defined away so mbg_eval / mbg_obs_index become plain host functions.
+// stdin: lines "w0 .. w{W-1} u2bits" (hex); stdout: "total H flip obs" with
+// floats as hex bits -- the same format as the oracle's --eval-states.
+#define __device__
+#include <cstdio>
+#include <iostream>
+#include <vector>
+
+#include "mbg_abi.h"
+
+#include GEN_FILE
+
+int main()
+{
+	std::vector<uint32_t> s(MBG_W);
+	uint32_t u2b;
+	while (true)
+	{
+		for (int w = 0; w < MBG_W; w++)
+			if (!(std::cin >> std::hex >> s[w]))
+				return 0;
+		std::cin >> std::hex >> u2b;
+		mbg_eval_t e;
+		mbg_eval(s.data(), mbg_u2f(u2b), &e);
+		std::printf("%08x %08x %d %llx\n", mbg_f2u(e.total), mbg_f2u(e.H), e.flip,
+					(unsigned long long)mbg_obs_index(s.data()));
+	}
+}

@@ -1,0 +1,116 @@
+"""Generated per-model code, run on the host, against the oracle.
+
+The engine's generator emits HIP source; tests/native/gen_harness.cpp
+compiles it for the host with the same float rules (-ffp-contract=off) and
+evaluates total rate, transition entropy, flip node and observed index for
+many states.  The oracle evaluates the same states with its own parser and
+interpreter.  Both must agree bit for bit (both entropy formulas)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import maboss.gpu_amd as mg
+from maboss.gpu_amd import netgen
+from conftest import DEVICE_INC, NATIVE, ORACLE, REF_MODELS, model_paths, pipe
+
+HIPRTC_FLAGS = ["--offload-arch=gfx950", "-fgpu-rdc", "-O3", "-std=c++17", "-ffp-contract=off",
+                "-fno-gpu-flush-denormals-to-zero", "-fhip-fp32-correctly-rounded-divide-sqrt",
+                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _models(tmp):
+    out = [(n, *model_paths(n)) for n in REF_MODELS]
+    for w in ("plain_small", "plain_100", "batched_50", "grad_30"):
+        out.append((w, *netgen.write_workload(w, str(tmp))))
+    return out
+
+
+@pytest.fixture(scope="module")
+def models(tmp_path_factory):
+    return _models(tmp_path_factory.mktemp("wl"))
+
+
+def _states(n_nodes, count, seed):
+    rng = np.random.default_rng(seed)
+    W = (n_nodes + 31) // 32
+    if n_nodes <= 10:
+        st = np.arange(1 << n_nodes, dtype=np.uint64)[:, None].astype(np.uint32)
+    else:
+        st = rng.integers(0, 2**32, size=(count, W), dtype=np.uint64).astype(np.uint32)
+        if n_nodes % 32:
+            st[:, -1] &= np.uint32((1 << (n_nodes % 32)) - 1)
+    u2 = rng.uniform(0, 1, size=len(st)).astype(np.float32)
+    u2[:3] = [np.float32(1.0), np.float32(2**-24), np.float32(0.5)]
+    return st, u2
+
+
+@pytest.mark.parametrize("entropy", [0, 1])
+def test_generated_matches_oracle(models, native_bin, tmp_path, entropy):
+    for name, bnd, cfg in models:
+        m = mg.Model.load(bnd, cfg)
+        gen = tmp_path / f"{name}_{entropy}.hip"
+        gen.write_text(m.generate_code(entropy_formula=entropy))
+        exe = native_bin("gen_harness", extra=[f'-DGEN_FILE="{gen}"'], tag=str(gen))
+        st, u2 = _states(m.info.n_nodes, 3000, 5)
+        text = "".join(" ".join(f"{w:x}" for w in row) + f" {u.view(np.uint32):08x}\n" for row, u in zip(st, u2))
+        a = pipe(exe, text)
+        b = pipe(ORACLE, text, "--eval-states", "--entropy", "reference" if entropy else "log_sum", bnd, cfg)
+        assert len(a) == len(st)
+        assert a == b, name
+
+
+def test_generated_compiles_for_gfx950(models, tmp_path):
+    """The generated source compiles for gfx950 with the JIT's flags (hipcc
+    runs the same clang front-end as hipRTC)."""
+    for name, bnd, cfg in models[:5]:
+        src = tmp_path / f"{name}.hip"
+        src.write_text(mg.Model.load(bnd, cfg).generate_code())
+        p = subprocess.run(["/opt/rocm/bin/hipcc", "-x", "hip", "--cuda-device-only", "-emit-llvm", "-c",
+                            *HIPRTC_FLAGS, "-include", "hip/hip_runtime.h", f"-I{DEVICE_INC}", str(src), "-o", str(tmp_path / f"{name}.bc")],
+                           capture_output=True, text=True)
+        assert p.returncode == 0, p.stderr[-3000:]
+
+
+def test_entropy_special_cases(native_bin, tmp_path):
+    """Zero observed rate -> 0; one nonzero -> 0; k equal rates -> log2 k."""
+    bnd = "".join(f"node X{i} {{ logic = !X{i}; rate_up = @logic ? $r{i} : 0; rate_down = @logic ? 0 : $r{i}; }}\n"
+                  for i in range(6))
+    for rates, expect in [([0, 0, 0, 0, 0, 0], 0.0), ([0, 3, 0, 0, 0, 0], 0.0), ([2] * 4 + [0, 0], 2.0),
+                          ([1.5] * 3 + [0] * 3, np.log2(3)), ([7] * 6, np.log2(6))]:
+        cfg = "".join(f"$r{i} = {r};\n" for i, r in enumerate(rates))
+        m = mg.Model.from_strings(bnd, cfg)
+        for ent in (0, 1):
+            gen = tmp_path / f"e{ent}_{len(str(rates))}_{sum(rates)}.hip"
+            gen.write_text(m.generate_code(entropy_formula=ent))
+            exe = native_bin("gen_harness", extra=[f'-DGEN_FILE="{gen}"'], tag=str(gen))
+            out = pipe(exe, "0 3f000000\n")[0].split()
+            H = np.uint32(int(out[1], 16)).view(np.float32)
+            assert abs(H - expect) <= 2 * np.spacing(np.float32(max(expect, 1.0))), (rates, ent, H)
+            total = np.uint32(int(out[0], 16)).view(np.float32)
+            assert total == np.float32(sum(np.float32(r) for r in rates))
+
+
+def test_flip_selection_brute_force(native_bin, tmp_path):
+    """flip = first node whose prefix sum reaches u2*total; zero-rate nodes
+    are never chosen; u2 = 1 picks the last nonzero node."""
+    rates = [0.0, 1.25, 0.0, 3.0, 0.5, 0.0, 2.0, 0.0]
+    bnd = "".join(f"node X{i} {{ logic = !X{i}; rate_up = @logic ? $r{i} : 0; rate_down = @logic ? 0 : $r{i}; }}\n"
+                  for i in range(8))
+    cfg = "".join(f"$r{i} = {r};\n" for i, r in enumerate(rates))
+    m = mg.Model.from_strings(bnd, cfg)
+    gen = tmp_path / "flip.hip"
+    gen.write_text(m.generate_code())
+    exe = native_bin("gen_harness", extra=[f'-DGEN_FILE="{gen}"'], tag=str(gen))
+    us = np.concatenate([np.linspace(1e-6, 1, 997, dtype=np.float32), [np.float32(1.0)]])
+    out = pipe(exe, "".join(f"0 {u.view(np.uint32):08x}\n" for u in us))
+    r32 = np.array(rates, dtype=np.float32)
+    pref = np.cumsum(r32, dtype=np.float32)
+    for u, line in zip(us, out):
+        flip = int(line.split()[1 + 1])
+        target = np.float32(u * pref[-1])
+        expect = int(np.sum(pref < target))
+        assert flip == expect
+        assert r32[flip] > 0
+    assert int(out[-1].split()[2]) == 6
