@@ -119,8 +119,14 @@ typedef struct mbg_sim_params
 	int32_t block_size;      /* 0 = 256 */
 	int32_t blocks_per_cu;   /* 0 = occupancy limit */
 	double normalization;    /* <=0: n_trajectories (reference divides by sample_count) */
-	int32_t reserved[8];
+	int32_t flags;           /* MBG_SIM_* bits */
+	int32_t reserved[7];
 } mbg_sim_params;
+
+/* Keep the statistics in device memory and return only the counters and
+ * timings (no device-to-host copy of the histograms); used to time the
+ * device-side path. */
+#define MBG_SIM_DEVICE_ONLY 1
 
 /* Fill p with the defaults above. */
 void mbg_sim_params_default(mbg_sim_params* p);

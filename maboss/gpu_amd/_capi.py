@@ -73,7 +73,7 @@ class SimParams(ctypes.Structure):
         ("final_mode", ctypes.c_int32), ("window_capacity", ctypes.c_uint64),
         ("final_capacity", ctypes.c_uint64), ("fixed_capacity", ctypes.c_uint64),
         ("block_size", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
-        ("normalization", ctypes.c_double), ("reserved", ctypes.c_int32 * 8),
+        ("normalization", ctypes.c_double), ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
     ]
 
 

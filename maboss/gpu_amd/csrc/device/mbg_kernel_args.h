@@ -36,12 +36,12 @@ struct mbg_sim_args
 	int win_mode, fin_mode, ent_lds, lds_bytes; // storage modes, LDS size in bytes
 	int win_lds_off, fin_lds_off, ent_lds_off, pad1; // u64 offsets into dynamic LDS
 
-	unsigned long long* win_vals;   // dense cells or hash values
-	unsigned long long* win_keys;   // hash keys (MBG_HIST_HASH)
+	unsigned long long* win_vals;   // dense cells (MBG_HIST_DENSE)
+	unsigned long long* win_table;  // hash slots {key, value} (MBG_HIST_HASH), 16 B each
 	unsigned long long win_mask;    // hash capacity - 1
 	unsigned long long* win_ent;    // [windows] entropy sums
 	unsigned long long* fin_vals;
-	unsigned long long* fin_keys;
+	unsigned long long* fin_table;
 	unsigned long long fin_mask;
 	unsigned int* fp_table;         // slots of (2 + W) u32: tag, count, state words
 	unsigned long long fp_mask;

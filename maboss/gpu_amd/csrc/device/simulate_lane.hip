@@ -54,18 +54,22 @@ __device__ __forceinline__ uint64_t bcast64(uint64_t v, int lane)
 	return ((uint64_t)hi << 32) | lo;
 }
 
-__device__ __forceinline__ void hash_add(unsigned long long* keys, unsigned long long* vals, uint64_t mask,
-										 uint64_t key, uint64_t q, unsigned long long* overflow)
+// Open-addressing insert-or-add into a table of 16-byte slots {key, value}
+// (key and value share a cache line).  Linear probing, bounded by the
+// capacity; a full table is reported through the overflow counter.
+__device__ __forceinline__ void hash_add(unsigned long long* table, uint64_t mask, uint64_t key, uint64_t q,
+										 unsigned long long* overflow)
 {
 	uint64_t slot = mix64(key) & mask;
 	for (uint64_t p = 0; p <= mask; ++p)
 	{
-		unsigned long long k = __hip_atomic_load(keys + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		unsigned long long* e = table + 2 * slot;
+		unsigned long long k = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		if (k == MBG_EMPTY_KEY)
-			k = atomicCAS(keys + slot, MBG_EMPTY_KEY, (unsigned long long)key);
+			k = atomicCAS(e, MBG_EMPTY_KEY, (unsigned long long)key);
 		if (k == MBG_EMPTY_KEY || k == key)
 		{
-			atomicAdd(vals + slot, (unsigned long long)q);
+			atomicAdd(e + 1, (unsigned long long)q);
 			return;
 		}
 		slot = (slot + 1) & mask;
@@ -138,7 +142,7 @@ struct lane_hist
 		else if (a.win_mode == MBG_HIST_DENSE)
 			atomicAdd(a.win_vals + cell, (unsigned long long)q);
 		else
-			hash_add(a.win_keys, a.win_vals, a.win_mask, cell, q, a.counters + MBG_CTR_WIN_OVERFLOW);
+			hash_add(a.win_table, a.win_mask, cell, q, a.counters + MBG_CTR_WIN_OVERFLOW);
 	}
 
 	__device__ void ent(int w, uint64_t q) const
@@ -156,7 +160,7 @@ struct lane_hist
 		else if (a.fin_mode == MBG_HIST_DENSE)
 			atomicAdd(a.fin_vals + obs, 1ull);
 		else
-			hash_add(a.fin_keys, a.fin_vals, a.fin_mask, obs, 1ull, a.counters + MBG_CTR_FIN_OVERFLOW);
+			hash_add(a.fin_table, a.fin_mask, obs, 1ull, a.counters + MBG_CTR_FIN_OVERFLOW);
 	}
 };
 

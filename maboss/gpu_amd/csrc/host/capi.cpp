@@ -325,6 +325,7 @@ int mbg_simulate(mbg_program* p, const mbg_sim_params* params, void* stream, mbg
 			sp.block_size = params->block_size;
 			sp.blocks_per_cu = params->blocks_per_cu;
 			sp.normalization = params->normalization;
+			sp.device_only = (params->flags & MBG_SIM_DEVICE_ONLY) != 0;
 		}
 		if (sp.block_size < 0 || sp.block_size > 1024 || (sp.block_size % 64) != 0)
 			throw mbg::error(mbg::MBG_E_ARG, "block_size must be a multiple of 64 up to 1024");

@@ -68,6 +68,7 @@ struct sim_params
 	uint64_t window_capacity = 0, final_capacity = 0, fixed_capacity = 0;
 	int block_size = 0, blocks_per_cu = 0;
 	double normalization = 0;
+	bool device_only = false; // leave the histograms in HBM
 };
 
 struct result

@@ -43,6 +43,14 @@ void device_buffer::release()
 namespace
 {
 
+// Empty hash slots: key = all ones, value = 0 (a strided 2D memset writes the
+// 8-byte key half of every 16-byte slot).
+void hip_fill_slots(unsigned long long* table, uint64_t slots, hipStream_t stream)
+{
+	hip_check(hipMemsetAsync(table, 0, slots * 16, stream), "hipMemsetAsync");
+	hip_check(hipMemset2DAsync(table, 16, 0xff, 8, slots, stream), "hipMemset2DAsync");
+}
+
 uint64_t next_pow2(uint64_t v)
 {
 	uint64_t p = 1;
@@ -92,7 +100,7 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 		throw error(MBG_E_LIMIT, "dense window histogram too large; use the hashed mode");
 	if (p.win_mode == MBG_HIST_LDS)
 	{
-		if (win_dense_bytes + lds > k_lds_budget)
+		if (win_dense_bytes > k_lds_budget - lds)
 			throw error(MBG_E_LIMIT, "window histogram does not fit in LDS");
 		p.win_off = (int)(lds / 8);
 		lds += win_dense_bytes;
@@ -100,14 +108,14 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 
 	p.fin_mode = sp.final_mode;
 	if (p.fin_mode < 0)
-		p.fin_mode = fin_dense_bytes + lds <= k_lds_budget ? MBG_HIST_LDS
+		p.fin_mode = fin_dense_bytes <= k_lds_budget - lds ? MBG_HIST_LDS
 					 : fin_dense_bytes <= (1ull << 30)	 ? MBG_HIST_DENSE
 														 : MBG_HIST_HASH;
 	if (p.fin_mode != MBG_HIST_HASH && fin_dense_bytes > k_dense_limit)
 		throw error(MBG_E_LIMIT, "dense final-state histogram too large; use the hashed mode");
 	if (p.fin_mode == MBG_HIST_LDS)
 	{
-		if (fin_dense_bytes + lds > k_lds_budget)
+		if (fin_dense_bytes > k_lds_budget - lds)
 			throw error(MBG_E_LIMIT, "final-state histogram does not fit in LDS");
 		p.fin_off = (int)(lds / 8);
 		lds += fin_dense_bytes;
@@ -119,7 +127,7 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 		if (n_obs + bits_for((uint64_t)windows) > 63)
 			throw error(MBG_E_LIMIT, "observed nodes plus window bits exceed the 63-bit hash key");
 		p.win_cap = sp.window_capacity ? next_pow2(sp.window_capacity)
-									   : next_pow2(std::min<uint64_t>(std::max<uint64_t>(n_traj * windows / 2, 1 << 16),
+									   : next_pow2(std::min<uint64_t>(std::max<uint64_t>(n_traj * windows, 1 << 16),
 																	   1ull << 28));
 	}
 	else
@@ -180,10 +188,8 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		unsigned long long *win_vals = nullptr, *win_keys = nullptr, *fin_vals = nullptr, *fin_keys = nullptr;
 		if (pl.win_mode == MBG_HIST_HASH)
 		{
-			win_vals = (unsigned long long*)p.b_win_vals.ensure(pl.win_cap * 8);
-			win_keys = (unsigned long long*)p.b_win_keys.ensure(pl.win_cap * 8);
-			hip_check(hipMemsetAsync(win_vals, 0, pl.win_cap * 8, stream), "hipMemsetAsync");
-			hip_check(hipMemsetAsync(win_keys, 0xff, pl.win_cap * 8, stream), "hipMemsetAsync");
+			win_keys = (unsigned long long*)p.b_win_keys.ensure(pl.win_cap * 16);
+			hip_fill_slots(win_keys, pl.win_cap, stream);
 		}
 		else
 		{
@@ -192,10 +198,8 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		}
 		if (pl.fin_mode == MBG_HIST_HASH)
 		{
-			fin_vals = (unsigned long long*)p.b_fin_vals.ensure(pl.fin_cap * 8);
-			fin_keys = (unsigned long long*)p.b_fin_keys.ensure(pl.fin_cap * 8);
-			hip_check(hipMemsetAsync(fin_vals, 0, pl.fin_cap * 8, stream), "hipMemsetAsync");
-			hip_check(hipMemsetAsync(fin_keys, 0xff, pl.fin_cap * 8, stream), "hipMemsetAsync");
+			fin_keys = (unsigned long long*)p.b_fin_keys.ensure(pl.fin_cap * 16);
+			hip_fill_slots(fin_keys, pl.fin_cap, stream);
 		}
 		else
 		{
@@ -238,11 +242,11 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		a.fin_lds_off = pl.fin_off;
 		a.ent_lds_off = pl.ent_off;
 		a.win_vals = win_vals;
-		a.win_keys = win_keys;
+		a.win_table = win_keys;
 		a.win_mask = pl.win_cap ? pl.win_cap - 1 : 0;
 		a.win_ent = ent;
 		a.fin_vals = fin_vals;
-		a.fin_keys = fin_keys;
+		a.fin_table = fin_keys;
 		a.fin_mask = pl.fin_cap ? pl.fin_cap - 1 : 0;
 		a.fp_table = fp;
 		a.fp_mask = pl.fp_cap - 1;
@@ -320,19 +324,25 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		res->final_mode = pl.fin_mode;
 		res->window_capacity = pl.win_cap;
 
+		if (sp.device_only)
+		{
+			res->ent.assign(windows, 0);
+			res->win.assign(windows, {});
+			break;
+		}
+
 		res->ent.resize(windows);
 		hip_check(hipMemcpy(res->ent.data(), ent, (size_t)windows * 8, hipMemcpyDeviceToHost), "hipMemcpy");
 
 		res->win.assign(windows, {});
 		if (pl.win_mode == MBG_HIST_HASH)
 		{
-			std::vector<unsigned long long> keys(pl.win_cap), vals(pl.win_cap);
-			hip_check(hipMemcpy(keys.data(), win_keys, pl.win_cap * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-			hip_check(hipMemcpy(vals.data(), win_vals, pl.win_cap * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+			std::vector<unsigned long long> tab(pl.win_cap * 2);
+			hip_check(hipMemcpy(tab.data(), win_keys, pl.win_cap * 16, hipMemcpyDeviceToHost), "hipMemcpy");
 			const uint64_t omask = n_obs >= 64 ? ~0ull : ((1ull << n_obs) - 1);
 			for (uint64_t i = 0; i < pl.win_cap; i++)
-				if (keys[i] != MBG_EMPTY_KEY && vals[i])
-					res->win[keys[i] >> n_obs].emplace_back(keys[i] & omask, vals[i]);
+				if (tab[2 * i] != MBG_EMPTY_KEY && tab[2 * i + 1])
+					res->win[tab[2 * i] >> n_obs].emplace_back(tab[2 * i] & omask, tab[2 * i + 1]);
 			for (auto& v : res->win)
 				std::sort(v.begin(), v.end());
 		}
@@ -349,12 +359,11 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 
 		if (pl.fin_mode == MBG_HIST_HASH)
 		{
-			std::vector<unsigned long long> keys(pl.fin_cap), vals(pl.fin_cap);
-			hip_check(hipMemcpy(keys.data(), fin_keys, pl.fin_cap * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-			hip_check(hipMemcpy(vals.data(), fin_vals, pl.fin_cap * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+			std::vector<unsigned long long> tab(pl.fin_cap * 2);
+			hip_check(hipMemcpy(tab.data(), fin_keys, pl.fin_cap * 16, hipMemcpyDeviceToHost), "hipMemcpy");
 			for (uint64_t i = 0; i < pl.fin_cap; i++)
-				if (keys[i] != MBG_EMPTY_KEY && vals[i])
-					res->fin.emplace_back(keys[i], vals[i]);
+				if (tab[2 * i] != MBG_EMPTY_KEY && tab[2 * i + 1])
+					res->fin.emplace_back(tab[2 * i], tab[2 * i + 1]);
 			std::sort(res->fin.begin(), res->fin.end());
 		}
 		else

@@ -1,0 +1,201 @@
+"""GPU engine vs host oracle: bit-exact parity through the C-ABI.
+
+Every comparison is on the exact integer dump (fixed-point window sums,
+entropy sums, final-state counts, fixed points and counters), so "equal"
+means equal bits.  Sizes are chosen so the oracle finishes in seconds; the
+full-size runs check size-independent properties instead (shard additivity,
+determinism, conservation)."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import maboss.gpu_amd as mg
+from maboss.gpu_amd import netgen
+from conftest import REF_MODELS, model_paths, read_raw, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+TWO32 = 4294967296.0
+
+
+@pytest.fixture(scope="module")
+def wl(tmp_path_factory):
+    d = tmp_path_factory.mktemp("wl")
+    return {w: netgen.write_workload(w, str(d)) for w in netgen.WORKLOADS}
+
+
+_programs = {}
+
+
+def program(bnd, cfg, entropy=0):
+    key = (bnd, cfg, entropy)
+    if key not in _programs:
+        m = mg.Model.load(bnd, cfg)
+        _programs[key] = m.compile(entropy_formula=entropy)
+    return _programs[key]
+
+
+def gpu_raw(tmp_path, bnd, cfg, tag="g", entropy=0, **kw):
+    print(f"[gpu] {os.path.basename(bnd)} {kw}", flush=True)
+    r = program(bnd, cfg, entropy).simulate(**kw)
+    path = tmp_path / f"{tag}.raw"
+    r.write_raw(str(path))
+    return path, r
+
+
+def oracle_raw(tmp_path, bnd, cfg, tag="o", n=None, extra=()):
+    path = tmp_path / f"{tag}.raw"
+    args = ["--raw", path, *extra]
+    if n is not None:
+        args = ["--n", n, *args]
+    run_oracle(bnd, cfg, *args)
+    return path
+
+
+def assert_same(a, b):
+    ta, tb = open(a).read(), open(b).read()
+    if ta != tb:
+        ra, rb = read_raw(a), read_raw(b)
+        for k in ("T", "E", "F", "P"):
+            assert ra.get(k) == rb.get(k), k
+        diff = {k: (ra["W"].get(k), rb["W"].get(k)) for k in set(ra["W"]) | set(rb["W"])
+                if ra["W"].get(k) != rb["W"].get(k)}
+        raise AssertionError(f"{len(diff)} window cells differ, e.g. {list(diff.items())[:5]}")
+
+
+@pytest.mark.parametrize("name,n", [("cellcycle", 20000), ("metastasis", 4000), ("sizek", 3000),
+                                    ("Montagud2022_Prostate_Cancer", 2000)])
+def test_reference_models_bit_exact(tmp_path, name, n):
+    bnd, cfg = model_paths(name)
+    g, _ = gpu_raw(tmp_path, bnd, cfg, n_trajectories=n)
+    assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=n))
+
+
+@pytest.mark.parametrize("name,n", [("plain_small", 10000), ("plain_100", 3000), ("plain_100_discrete", 3000),
+                                    ("batched_50", 3000), ("grad_30", 5000)])
+def test_workloads_bit_exact(tmp_path, wl, name, n):
+    bnd, cfg = wl[name]
+    g, r = gpu_raw(tmp_path, bnd, cfg, n_trajectories=n)
+    assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=n))
+    if name.startswith("plain_100"):
+        assert r.info.window_mode == 1  # 50 observed nodes: hashed histogram
+
+
+def test_reference_entropy_formula(tmp_path):
+    bnd, cfg = model_paths("metastasis")
+    g, _ = gpu_raw(tmp_path, bnd, cfg, entropy=1, n_trajectories=3000)
+    assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=3000, extra=("--entropy", "reference")))
+
+
+@pytest.mark.parametrize("block,per_cu", [(64, 1), (128, 3), (512, 1), (1024, 1), (256, 0)])
+def test_launch_shape_invariance(tmp_path, wl, block, per_cu):
+    for bnd, cfg in (model_paths("cellcycle"), wl["plain_100"]):
+        a, _ = gpu_raw(tmp_path, bnd, cfg, "a", n_trajectories=5000, block_size=block, blocks_per_cu=per_cu)
+        b, _ = gpu_raw(tmp_path, bnd, cfg, "b", n_trajectories=5000)
+        assert_same(a, b)
+
+
+@pytest.mark.parametrize("win,fin", [("dense", "dense"), ("hash", "hash"), ("lds", "lds"), ("hash", "dense")])
+def test_storage_mode_invariance(tmp_path, win, fin):
+    bnd, cfg = model_paths("cellcycle")
+    a, r = gpu_raw(tmp_path, bnd, cfg, "a", n_trajectories=8000, window_mode=win, final_mode=fin)
+    b, _ = gpu_raw(tmp_path, bnd, cfg, "b", n_trajectories=8000)
+    assert r.info.window_mode == {"dense": 0, "hash": 1, "lds": 2}[win]
+    assert_same(a, b)
+
+
+def test_hash_growth_and_overflow_error(tmp_path, wl):
+    bnd, cfg = wl["plain_100"]
+    a, r = gpu_raw(tmp_path, bnd, cfg, "a", n_trajectories=2000, window_mode="hash", window_capacity=0)
+    with pytest.raises(mg.MbgError) as e:
+        program(bnd, cfg).simulate(n_trajectories=2000, window_mode="hash", window_capacity=1024)
+    assert e.value.code == 8
+    assert_same(a, oracle_raw(tmp_path, bnd, cfg, n=2000))
+
+
+def test_edge_cases(tmp_path):
+    # empty run
+    bnd, cfg = model_paths("cellcycle")
+    _, r = gpu_raw(tmp_path, bnd, cfg, n_trajectories=0)
+    assert r.info.transitions == 0 and r.final_states()[0].size == 0
+    # one trajectory, and an odd id offset
+    g, _ = gpu_raw(tmp_path, bnd, cfg, "one", n_trajectories=1, first_id=12345)
+    assert_same(g, oracle_raw(tmp_path, bnd, cfg, "o1", n=1, extra=("--first-id", 12345)))
+    # step cap: truncated trajectories are not final states
+    bnd2, cfg2 = model_paths("metastasis")
+    g, r = gpu_raw(tmp_path, bnd2, cfg2, "cap", n_trajectories=500, max_steps=7)
+    assert r.info.truncated > 0
+    assert_same(g, oracle_raw(tmp_path, bnd2, cfg2, "ocap", n=500, extra=("--max-steps", 7)))
+
+
+def test_fixed_points_discrete_and_continuous(tmp_path):
+    for disc in (0, 1):
+        b, c = tmp_path / f"d{disc}.bnd", tmp_path / f"d{disc}.cfg"
+        b.write_text("node A { rate_up = 0; rate_down = $l; }\nnode B { logic = A; rate_up = @logic ? 2 : 0; "
+                     "rate_down = @logic ? 0 : 2; }\nnode C { rate_up = 0; rate_down = 0; }\n")
+        c.write_text(f"$l = 0.5; A.istate = 1; C.is_internal = 1; max_time = 6; time_tick = 0.5; "
+                     f"discrete_time = {disc}; sample_count = 5000;")
+        g, r = gpu_raw(tmp_path, str(b), str(c), f"g{disc}")
+        assert_same(g, oracle_raw(tmp_path, str(b), str(c), f"o{disc}"))
+        assert r.fixed_points()[0].shape[0] >= 1
+
+
+def test_determinism_and_shards(tmp_path, wl):
+    bnd, cfg = wl["plain_100"]
+    n = 20000
+    a, ra = gpu_raw(tmp_path, bnd, cfg, "a", n_trajectories=n)
+    b, _ = gpu_raw(tmp_path, bnd, cfg, "b", n_trajectories=n)
+    assert_same(a, b)
+    s1, _ = gpu_raw(tmp_path, bnd, cfg, "s1", n_trajectories=n // 4)
+    s2, _ = gpu_raw(tmp_path, bnd, cfg, "s2", n_trajectories=n - n // 4, first_id=n // 4)
+    x, y1, y2 = read_raw(a), read_raw(s1), read_raw(s2)
+    merged = dict(y1["W"])
+    for k, v in y2["W"].items():
+        merged[k] = merged.get(k, 0) + v
+    assert merged == x["W"]
+    assert tuple(p + q for p, q in zip(y1["T"], y2["T"])) == x["T"]
+
+
+def test_cli_csv_matches_oracle_csv(tmp_path):
+    bnd, cfg = model_paths("cellcycle")
+    small = tmp_path / "cc.cfg"
+    small.write_text(open(cfg).read() + "\nsample_count = 20000;\n")
+    p = mg.run_cli("-o", str(tmp_path / "gpu"), bnd, str(small))
+    assert p.returncode == 0, p.stderr
+    run_oracle(bnd, str(small), "-o", tmp_path / "cpu")
+    for suffix in ("_probtraj.csv", "_fp.csv"):
+        assert open(tmp_path / ("gpu" + suffix)).read() == open(tmp_path / ("cpu" + suffix)).read(), suffix
+    p = mg.run_cli(bnd, str(small))
+    assert p.returncode == 0 and p.stdout.startswith("final points:\n")
+    run_oracle(bnd, str(small), "--print", tmp_path / "cpu.txt")
+    assert p.stdout == open(tmp_path / "cpu.txt").read()
+
+
+def test_full_size_plain_100_properties(tmp_path, wl):
+    """The headline workload at its full size (1e6 trajectories): shard
+    additivity, conservation of every window, all trajectories finished."""
+    bnd, cfg = wl["plain_100"]
+    n = 1000000
+    full, r = gpu_raw(tmp_path, bnd, cfg, "full", n_trajectories=n)
+    info = r.info
+    assert info.finished == n and info.truncated == 0
+    fin_states, fin_counts = r.final_states()
+    assert int(fin_counts.sum()) == n
+    half = n // 2
+    h1, _ = gpu_raw(tmp_path, bnd, cfg, "h1", n_trajectories=half)
+    h2, _ = gpu_raw(tmp_path, bnd, cfg, "h2", n_trajectories=n - half, first_id=half)
+    x, y1, y2 = read_raw(full), read_raw(h1), read_raw(h2)
+    merged = dict(y1["W"])
+    for k, v in y2["W"].items():
+        merged[k] = merged.get(k, 0) + v
+    assert merged == x["W"]
+    m = mg.Model.load(bnd, cfg)
+    tick = np.float32(m.info.time_tick)
+    for w in (0, 1, 50, 99):
+        tot = sum(v for (ww, _), v in x["W"].items() if ww == w)
+        lo = np.float32(w) * tick
+        hi = min(np.float32(w + 1) * tick, np.float32(m.info.max_time))
+        assert abs(tot / (n * TWO32) - float(hi - lo) / float(tick)) < 2e-6
