@@ -120,7 +120,8 @@ typedef struct mbg_sim_params
 	int32_t blocks_per_cu;   /* 0 = occupancy limit */
 	double normalization;    /* <=0: n_trajectories (reference divides by sample_count) */
 	int32_t flags;           /* MBG_SIM_* bits */
-	int32_t reserved[7];
+	int32_t kernel;          /* -1 auto (lane if N <= 256), 0 lane per trajectory, 1 wave per trajectory */
+	int32_t reserved[6];
 } mbg_sim_params;
 
 /* Keep the statistics in device memory and return only the counters and
@@ -159,6 +160,8 @@ typedef struct mbg_result_info
 	double total_ms;                 /* host wall time of mbg_simulate */
 	int32_t window_mode, final_mode;
 	uint64_t window_capacity;
+	int32_t kernel;                  /* 0 lane per trajectory, 1 wave per trajectory */
+	int32_t pad;
 } mbg_result_info;
 
 int mbg_result_get_info(const mbg_result* r, mbg_result_info* out);

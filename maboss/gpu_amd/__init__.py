@@ -125,7 +125,7 @@ class Program:
                  max_steps: int = 0, window_mode: str = "auto", final_mode: str = "auto",
                  window_capacity: int = 0, final_capacity: int = 0, fixed_capacity: int = 0,
                  block_size: int = 0, blocks_per_cu: int = 0, normalization: float = 0.0,
-                 device_only: bool = False, stream=None) -> "Result":
+                 device_only: bool = False, kernel: str = "auto", stream=None) -> "Result":
         p = SimParams()
         lib().mbg_sim_params_default(ctypes.byref(p))
         p.n_trajectories = n_trajectories
@@ -142,6 +142,7 @@ class Program:
         p.blocks_per_cu = blocks_per_cu
         p.normalization = normalization
         p.flags = 1 if device_only else 0
+        p.kernel = {"auto": -1, "lane": 0, "wave": 1}[kernel]
         h = ctypes.c_void_p()
         s = ctypes.c_void_p(stream) if stream else None
         check(lib().mbg_simulate(self._h, ctypes.byref(p), s, ctypes.byref(h)))

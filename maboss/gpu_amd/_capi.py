@@ -73,7 +73,8 @@ class SimParams(ctypes.Structure):
         ("final_mode", ctypes.c_int32), ("window_capacity", ctypes.c_uint64),
         ("final_capacity", ctypes.c_uint64), ("fixed_capacity", ctypes.c_uint64),
         ("block_size", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
-        ("normalization", ctypes.c_double), ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
+        ("normalization", ctypes.c_double), ("flags", ctypes.c_int32), ("kernel", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -88,7 +89,7 @@ class ResultInfo(ctypes.Structure):
         ("n_final_states", ctypes.c_uint64), ("n_window_cells", ctypes.c_uint64),
         ("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
         ("window_mode", ctypes.c_int32), ("final_mode", ctypes.c_int32),
-        ("window_capacity", ctypes.c_uint64),
+        ("window_capacity", ctypes.c_uint64), ("kernel", ctypes.c_int32), ("pad", ctypes.c_int32),
     ]
 
 

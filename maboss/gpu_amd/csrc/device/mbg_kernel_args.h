@@ -22,6 +22,33 @@
 
 #define MBG_EMPTY_KEY 0xffffffffffffffffull
 
+// Node of the wave-per-trajectory kernel: logic as a truth table over at
+// most MBG_WAVE_MAXPAR parent nodes, rates of the fast form
+// rate = (state == logic) ? 0 : (state ? down : up).
+#define MBG_WAVE_MAXPAR 6
+#define MBG_WAVE_NOOBS 0xffffu
+struct mbg_wnode
+{
+	unsigned short par[MBG_WAVE_MAXPAR]; // parent node indices (unused: 0)
+	unsigned char npar;
+	unsigned char pad;
+	unsigned short obs_pos; // bit of the observed index, or MBG_WAVE_NOOBS
+	unsigned long long tt;  // bit j = logic value for parent bits j
+	float up, down;
+};
+
+// Tables of the wave kernel, in device memory; the kernel stages them in LDS.
+struct mbg_wave_tables
+{
+	const mbg_wnode* nodes;        // [n_nodes]
+	const unsigned int* child_off; // [n_nodes + 1] CSR offsets into child
+	const unsigned short* child;   // nodes whose rate depends on a node
+	const unsigned short* obs;     // [n_obs] observed nodes in order
+	int n_child, npl;              // entries of child; nodes per lane
+	int tab_bytes;                 // LDS bytes of the staged tables (16-aligned)
+	int pad;
+};
+
 struct mbg_sim_args
 {
 	unsigned long long n_traj;  // trajectories in this launch
@@ -47,4 +74,7 @@ struct mbg_sim_args
 	unsigned long long fp_mask;
 	unsigned long long* counters;   // [MBG_NCOUNTERS]
 	const mbg_param_view* pv;       // per-variant tables (batched/gradient); null for plain
+	mbg_wave_tables wt;             // wave-per-trajectory kernel only
+	int wave_lds_off;               // byte offset of the per-wave rate arrays in LDS
+	int pad2;
 };

@@ -1,0 +1,488 @@
+// Wave-per-trajectory simulate kernel for large models (gfx950).
+//
+// One 64-lane wave owns one trajectory.  The N nodes are split into 64
+// contiguous slices of npl = ceil(N/64) nodes, one slice per lane, and the
+// state is distributed one 32-bit word per lane (word w lives in lane w, so
+// N <= 2048).  A node's logic is a truth table over at most 6 parents whose
+// bits are fetched from the owning lanes with ds_bpermute (__shfl); its rate
+// is (state == logic) ? 0 : (state ? down : up), the reference's fast form
+// (src/generator.cpp:93-107).
+//
+// Rates live in LDS (one [npl][64] array per wave).  After a flip only the
+// flipped node and its children are re-evaluated, lanes taking one of them
+// each.  Every step each lane re-sums its slice sequentially, a Kogge-Stone
+// scan over the 64 lanes gives the total and the prefix of every slice, and
+// a ballot picks the slice holding u2*total (src/jit_kernels/simulation.cu:
+// 9-21 selects by prefix; here the canonical order is slice-then-scan, and
+// zero-rate nodes are never chosen).  Time, entropy and the window, final
+// and fixed-point statistics follow src/jit_kernels/simulation.cu:106-163 and
+// src/jit_kernels/window_average_small.cu:50-136, as in the lane kernel.
+//
+// Random numbers: the same Philox stream as the lane kernel; each lane
+// computes one block of a 64-block batch, so one Philox evaluation per lane
+// serves 128 continuous (256 discrete) steps.
+#include <hip/hip_runtime.h>
+
+#include "mbg_kernel_args.h"
+#include "mbg_stats_device.h"
+
+using namespace mbg_dev;
+
+namespace
+{
+
+__device__ __forceinline__ float rdlane_f(float v, int l)
+{
+	return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+__device__ __forceinline__ uint32_t rdlane_u(uint32_t v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// Rate of node i in the wave's state (sw = this lane's state word).  Called by
+// all lanes together: every lane takes part in the parent-word shuffles.
+__device__ __forceinline__ float node_rate(const mbg_wnode* nodes, int i, bool valid, uint32_t sw)
+{
+	const mbg_wnode& nd = nodes[valid ? i : 0];
+	const int npar = valid ? nd.npar : 0;
+	uint32_t idx = 0;
+#pragma unroll
+	for (int j = 0; j < MBG_WAVE_MAXPAR; ++j)
+	{
+		const unsigned p = nd.par[j];
+		const uint32_t w = (uint32_t)__shfl((int)sw, (int)(p >> 5));
+		idx |= (j < npar ? ((w >> (p & 31u)) & 1u) : 0u) << j;
+	}
+	const uint32_t ws = (uint32_t)__shfl((int)sw, i >> 5);
+	const bool on = (ws >> (i & 31)) & 1u;
+	const bool logic = (nd.tt >> idx) & 1ull;
+	return (!valid || on == logic) ? 0.0f : (on ? nd.down : nd.up);
+}
+
+__device__ __forceinline__ float ks_scan(float v, int lane)
+{
+#pragma unroll
+	for (int off = 1; off < 64; off <<= 1)
+	{
+		const float y = __shfl_up(v, off);
+		if (lane >= off)
+			v = v + y;
+	}
+	return v;
+}
+
+template <int NPLMAX>
+__device__ void wave_main(const mbg_sim_args& a)
+{
+	extern __shared__ unsigned long long lds[];
+	char* ldsb = (char*)lds;
+	const int N = a.n_nodes;
+	const int npl = a.wt.npl;
+	const int W = (N + 31) >> 5;
+
+	// stage the node tables, clear the LDS histograms
+	{
+		const uint4* src = (const uint4*)a.wt.nodes;
+		uint4* dst = (uint4*)(ldsb + a.wave_lds_off);
+		for (int i = threadIdx.x; i < a.wt.tab_bytes / 16; i += blockDim.x)
+			dst[i] = src[i];
+		for (int i = threadIdx.x; i < a.lds_bytes / 8; i += blockDim.x)
+			lds[i] = 0ull;
+	}
+	__syncthreads();
+
+	const mbg_wnode* nodes = (const mbg_wnode*)(ldsb + a.wave_lds_off);
+	const unsigned* coff = (const unsigned*)(nodes + N);
+	const unsigned short* child = (const unsigned short*)(coff + N + 1);
+	const unsigned short* obsl = child + a.wt.n_child;
+	const int lane = __lane_id();
+	const int wave = threadIdx.x >> 6;
+	float* rates = (float*)(ldsb + a.wave_lds_off + a.wt.tab_bytes) + (size_t)wave * npl * 64;
+	auto raddr = [&](int node) { return (node % npl) * 64 + node / npl; };
+
+	const lane_hist hist { lds, a };
+	const uint32_t B0 = ((uint32_t)N + 3u) >> 2;
+
+	int pw = -1, pew = -1;
+	uint64_t pobs = 0, pq = 0, peq = 0;
+	uint64_t ntrans = 0, nfinished = 0, ntrunc = 0;
+
+	for (;;)
+	{
+		uint64_t tid = 0;
+		if (lane == 0)
+			tid = atomicAdd(a.counters + MBG_CTR_QUEUE, 1ull);
+		tid = bcast64(tid, 0);
+		if (tid >= a.n_traj)
+			break;
+		const uint64_t id = a.id_base + tid;
+		const uint32_t idlo = (uint32_t)id, idhi = (uint32_t)(id >> 32);
+
+		// initial state, word `lane` (src/jit_kernels/simulation.cu:34-71)
+		uint32_t sw = 0u;
+		if (lane < W)
+		{
+			for (int b = 0; b < 8; ++b)
+			{
+				const int i0 = lane * 32 + b * 4;
+				if (i0 >= N)
+					break;
+				const mbg_u32x4 r =
+					mbg_philox4x32_10(mbg_u32x4 { (uint32_t)(i0 >> 2), idlo, idhi, a.variant }, a.seed_lo, a.seed_hi);
+				const uint32_t rw[4] = { r.x, r.y, r.z, r.w };
+#pragma unroll
+				for (int j = 0; j < 4; ++j)
+					if (i0 + j < N && mbg_u01(rw[j]) <= a.istate[i0 + j])
+						sw |= 1u << (b * 4 + j);
+			}
+		}
+		// observed index: bit k = observed node k
+		uint64_t obs = 0;
+		{
+			const int o = lane < a.n_obs ? obsl[lane] : 0;
+			const uint32_t w = (uint32_t)__shfl((int)sw, o >> 5);
+			obs = __ballot(lane < a.n_obs && ((w >> (o & 31)) & 1u));
+		}
+		// all rates
+#pragma unroll
+		for (int k = 0; k < NPLMAX; ++k)
+		{
+			if (k < npl)
+			{
+				const int i = lane * npl + k;
+				const float rv = node_rate(nodes, i, i < N, sw);
+				rates[k * 64 + lane] = rv;
+			}
+		}
+
+		float t = 0.0f;
+		uint32_t k = 0;
+		mbg_u32x4 rb { 0, 0, 0, 0 };
+		bool fixed = false, ended_ok = false;
+		for (;;)
+		{
+			// this lane's slice and the scan over lanes
+			float r[NPLMAX];
+			float L = 0.0f;
+#pragma unroll
+			for (int s = 0; s < NPLMAX; ++s)
+			{
+				r[s] = s < npl ? rates[s * 64 + lane] : 0.0f;
+				L = L + r[s];
+			}
+			const float P = ks_scan(L, lane);
+			const float total = rdlane_f(P, 63);
+
+			// uniforms of step k: one Philox block per lane per batch
+			float u1 = 1.0f, u2;
+			if (a.discrete)
+			{
+				if ((k & 255u) == 0u)
+					rb = mbg_philox4x32_10(mbg_u32x4 { B0 + ((k >> 8) << 6) + (uint32_t)lane, idlo, idhi, a.variant },
+										   a.seed_lo, a.seed_hi);
+				u2 = mbg_u01(rdlane_u(pick(rb, k & 3u), (int)((k >> 2) & 63u)));
+			}
+			else
+			{
+				if ((k & 127u) == 0u)
+					rb = mbg_philox4x32_10(mbg_u32x4 { B0 + ((k >> 7) << 6) + (uint32_t)lane, idlo, idhi, a.variant },
+										   a.seed_lo, a.seed_hi);
+				const int src = (int)((k >> 1) & 63u);
+				u1 = mbg_u01(rdlane_u((k & 1u) ? rb.z : rb.x, src));
+				u2 = mbg_u01(rdlane_u((k & 1u) ? rb.w : rb.y, src));
+			}
+
+			const float tp = t;
+			float H = 0.0f;
+			if (total == 0.0f)
+			{
+				fixed = true;
+				t = a.max_time;
+			}
+			else
+			{
+				if (a.discrete)
+				{
+					t = t + a.tick;
+					t = (a.max_time - a.tick < t) ? a.max_time : t;
+				}
+				else
+					t = t + (-mbg_logf(u1)) / total;
+				t = fminf(t, a.max_time);
+				// transition entropy, log-sum form: observed node j adds to lane j % 64
+				float Sv = 0.0f, Tv = 0.0f;
+				int cnt = 0;
+				for (int j = lane; j < a.n_obs; j += 64)
+				{
+					const float v = rates[raddr(obsl[j])];
+					Sv = Sv + v;
+					Tv = Tv + mbg_rlog2r(v);
+					cnt += v != 0.0f;
+				}
+				const float S = rdlane_f(ks_scan(Sv, lane), 63);
+				const float T = rdlane_f(ks_scan(Tv, lane), 63);
+#pragma unroll
+				for (int off = 32; off > 0; off >>= 1)
+					cnt += __shfl_xor(cnt, off);
+				const float h = mbg_log2f(S) - T / S;
+				H = (cnt > 1 && h > 0.0f) ? h : 0.0f;
+			}
+			ntrans++;
+
+			// record (state during [tp, t), H) into the windows; lane 0 flushes
+			if (a.discrete)
+			{
+				const int w = (int)roundf(tp / a.tick);
+				if (w < a.windows)
+				{
+					if (w == pw && obs == pobs)
+						pq += 1ull;
+					else
+					{
+						if (pw >= 0 && lane == 0)
+							hist.win(pw, pobs, pq);
+						pw = w;
+						pobs = obs;
+						pq = 1ull;
+					}
+					const uint64_t qe = (uint64_t)rintf(H * a.qscale);
+					if (qe)
+					{
+						if (w == pew)
+							peq += qe;
+						else
+						{
+							if (pew >= 0 && lane == 0)
+								hist.ent(pew, peq);
+							pew = w;
+							peq = qe;
+						}
+					}
+				}
+			}
+			else
+			{
+				int w = (int)floorf(tp / a.tick);
+				float b = tp;
+				while (t > b)
+				{
+					const float wend = (float)(w + 1) * a.tick;
+					const float en = fminf(t, wend);
+					const float sl = en - b;
+					if (w < a.windows)
+					{
+						const uint64_t q = (uint64_t)rintf(sl * a.qscale);
+						if (q)
+						{
+							if (w == pw && obs == pobs)
+								pq += q;
+							else
+							{
+								if (pw >= 0 && lane == 0)
+									hist.win(pw, pobs, pq);
+								pw = w;
+								pobs = obs;
+								pq = q;
+							}
+						}
+						const uint64_t qe = (uint64_t)rintf((H * sl) * a.qscale);
+						if (qe)
+						{
+							if (w == pew)
+								peq += qe;
+							else
+							{
+								if (pew >= 0 && lane == 0)
+									hist.ent(pew, peq);
+								pew = w;
+								peq = qe;
+							}
+						}
+					}
+					w++;
+					b = en;
+				}
+			}
+
+			k++;
+			if (t >= a.max_time)
+			{
+				nfinished++;
+				ended_ok = true;
+				break;
+			}
+			if (k >= a.max_steps)
+			{
+				ntrunc++;
+				break;
+			}
+
+			// flip selection: first lane whose scan reaches u2*total with
+			// weight in its slice (else the last lane with weight), then the
+			// first slot of that slice reaching the remainder with a nonzero
+			// rate (else its last nonzero slot)
+			const float rr = u2 * total;
+			const unsigned long long c1 = __ballot(P >= rr && L > 0.0f);
+			const unsigned long long c2 = __ballot(L > 0.0f);
+			// c2 is nonzero whenever total > 0; the fallback only guards NaN rates
+			const int ls = c1 ? __ffsll(c1) - 1 : (c2 ? 63 - __clzll(c2) : 0);
+			const float prevP = __shfl_up(P, 1);
+			const float rem = rr - (lane == 0 ? 0.0f : prevP);
+			float qsum = 0.0f;
+			int ks = -1, last = -1;
+#pragma unroll
+			for (int s = 0; s < NPLMAX; ++s)
+			{
+				if (s < npl)
+				{
+					qsum = qsum + r[s];
+					if (r[s] > 0.0f)
+					{
+						last = s;
+						if (ks < 0 && qsum >= rem)
+							ks = s;
+					}
+				}
+			}
+			const int kstar = (int)rdlane_u((uint32_t)(ks >= 0 ? ks : last), ls);
+			const int f = min(max(ls * npl + kstar, 0), N - 1);
+
+			if (lane == (f >> 5))
+				sw ^= 1u << (f & 31);
+			const unsigned op = nodes[f].obs_pos;
+			if (op != MBG_WAVE_NOOBS)
+				obs ^= 1ull << op;
+
+			// re-evaluate the flipped node and its children
+			const int c0 = (int)coff[f];
+			const int items = 1 + (int)coff[f + 1] - c0;
+			for (int base = 0; base < items; base += 64)
+			{
+				const int j = base + lane;
+				const bool valid = j < items;
+				const int node = !valid ? 0 : (j == 0 ? f : (int)child[c0 + j - 1]);
+				const float rv = node_rate(nodes, node, valid, sw);
+				if (valid)
+					rates[raddr(node)] = rv;
+			}
+			__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+		}
+
+		// trajectory end: final state and fixed points
+		if (ended_ok)
+		{
+			if (lane == 0)
+				hist.fin(obs);
+			if (fixed)
+			{
+				constexpr int stride_pad = 2;
+				const int stride = stride_pad + W;
+				uint64_t h = 0x9e3779b97f4a7c15ull;
+				for (int w = 0; w < W; ++w)
+					h = mix64(h ^ rdlane_u(sw, w));
+				uint64_t slot = h & a.fp_mask;
+				for (uint64_t p = 0; p <= a.fp_mask; ++p)
+				{
+					unsigned int* e = a.fp_table + slot * stride;
+					uint32_t tag = 0, claimed = 0;
+					if (lane == 0)
+					{
+						tag = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+						if (tag == 0u)
+						{
+							tag = atomicCAS(e, 0u, 1u);
+							claimed = tag == 0u;
+						}
+					}
+					claimed = rdlane_u(claimed, 0);
+					if (claimed)
+					{
+						if (lane < W)
+							e[2 + lane] = sw;
+						__threadfence();
+						if (lane == 0)
+						{
+							__hip_atomic_store(e, 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+							atomicAdd(e + 1, 1u);
+						}
+						break;
+					}
+					if (lane == 0)
+						for (int spin = 0; tag == 1u && spin < (1 << 22); ++spin)
+						{
+							__builtin_amdgcn_s_sleep(2);
+							tag = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+						}
+					tag = rdlane_u(tag, 0);
+					if (tag != 2u)
+					{
+						if (lane == 0)
+							atomicAdd(a.counters + MBG_CTR_FP_TIMEOUT, 1ull);
+						break;
+					}
+					__threadfence();
+					const bool same =
+						lane >= W || __hip_atomic_load(e + 2 + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == sw;
+					if (__ballot(!same) == 0ull)
+					{
+						if (lane == 0)
+							atomicAdd(e + 1, 1u);
+						break;
+					}
+					slot = (slot + 1) & a.fp_mask;
+					if (p == a.fp_mask && lane == 0)
+						atomicAdd(a.counters + MBG_CTR_FP_OVERFLOW, 1ull);
+				}
+			}
+		}
+	}
+
+	if (lane == 0)
+	{
+		if (pw >= 0)
+			hist.win(pw, pobs, pq);
+		if (pew >= 0)
+			hist.ent(pew, peq);
+		atomicAdd(a.counters + MBG_CTR_TRANSITIONS, (unsigned long long)ntrans);
+		atomicAdd(a.counters + MBG_CTR_FINISHED, (unsigned long long)nfinished);
+		if (ntrunc)
+			atomicAdd(a.counters + MBG_CTR_TRUNCATED, (unsigned long long)ntrunc);
+	}
+
+	__syncthreads();
+	if (a.lds_bytes)
+	{
+		const uint64_t nwin = a.win_mode == MBG_HIST_LDS ? ((uint64_t)a.windows << a.n_obs) : 0;
+		for (uint64_t i = threadIdx.x; i < nwin; i += blockDim.x)
+		{
+			const unsigned long long v = lds[a.win_lds_off + i];
+			if (v)
+				atomicAdd(a.win_vals + i, v);
+		}
+		if (a.ent_lds)
+			for (int i = threadIdx.x; i < a.windows; i += blockDim.x)
+			{
+				const unsigned long long v = lds[a.ent_lds_off + i];
+				if (v)
+					atomicAdd(a.win_ent + i, v);
+			}
+		const uint64_t nfin = a.fin_mode == MBG_HIST_LDS ? (1ull << a.n_obs) : 0;
+		for (uint64_t i = threadIdx.x; i < nfin; i += blockDim.x)
+		{
+			const unsigned long long v = lds[a.fin_lds_off + i];
+			if (v)
+				atomicAdd(a.fin_vals + i, v);
+		}
+	}
+}
+
+} // namespace
+
+#define MBG_WAVE_KERNEL(n)                                                                                             \
+	extern "C" __global__ void __launch_bounds__(512) simulate_wave_n##n(mbg_sim_args a) { wave_main<n>(a); }
+
+MBG_WAVE_KERNEL(1)
+MBG_WAVE_KERNEL(2)
+MBG_WAVE_KERNEL(4)
+MBG_WAVE_KERNEL(8)
+MBG_WAVE_KERNEL(16)
+MBG_WAVE_KERNEL(32)

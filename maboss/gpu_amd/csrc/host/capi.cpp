@@ -71,43 +71,6 @@ mbg::gen_options to_gen(const mbg_compile_opts* o)
 	return g;
 }
 
-float interp(const mbg::model& m, const mbg::node_t& cur, const mbg::expr& e, const uint32_t* s, int depth)
-{
-	using mbg::op;
-	if (depth > 64)
-		throw mbg::error(mbg::MBG_E_MODEL, "alias nesting too deep");
-	auto f = [&](const mbg::expr& x) { return interp(m, cur, x, s, depth); };
-	switch (e.kind)
-	{
-		case op::node: {
-			int i = m.node_index(e.name);
-			return (float)((s[i / 32] >> (i % 32)) & 1u);
-		}
-		case op::alias: return interp(m, cur, *cur.attr(e.name.substr(1)), s, depth + 1);
-		case op::lit: return e.value;
-		case op::var: return m.variables.at(e.name);
-		case op::paren:
-		case op::pos: return f(*e.a);
-		case op::neg: return -f(*e.a);
-		case op::lnot: return !f(*e.a);
-		case op::add: return f(*e.a) + f(*e.b);
-		case op::sub: return f(*e.a) - f(*e.b);
-		case op::mul: return f(*e.a) * f(*e.b);
-		case op::div: return f(*e.a) / f(*e.b);
-		case op::land: return f(*e.a) && f(*e.b);
-		case op::lor: return f(*e.a) || f(*e.b);
-		case op::lxor: return (f(*e.a) != 0) != (f(*e.b) != 0);
-		case op::eq: return f(*e.a) == f(*e.b);
-		case op::ne: return f(*e.a) != f(*e.b);
-		case op::lt: return f(*e.a) < f(*e.b);
-		case op::le: return f(*e.a) <= f(*e.b);
-		case op::gt: return f(*e.a) > f(*e.b);
-		case op::ge: return f(*e.a) >= f(*e.b);
-		case op::ternary: return f(*e.a) ? f(*e.b) : f(*e.c);
-	}
-	return 0.f;
-}
-
 } // namespace
 
 extern "C" {
@@ -229,7 +192,7 @@ int mbg_model_eval_rate(const mbg_model* mm, int32_t node, const uint32_t* s, fl
 		const mbg::model& m = *mm->m;
 		const mbg::node_t& n = m.nodes[node];
 		const bool on = (s[node / 32] >> (node % 32)) & 1u;
-		*rate = interp(m, n, *n.attr(on ? "rate_down" : "rate_up"), s, 0);
+		*rate = m.eval_in_state(n, *n.attr(on ? "rate_down" : "rate_up"), s);
 	});
 }
 
@@ -302,6 +265,7 @@ void mbg_sim_params_default(mbg_sim_params* p)
 	p->seed = -1;
 	p->window_mode = -1;
 	p->final_mode = -1;
+	p->kernel = -1;
 }
 
 int mbg_simulate(mbg_program* p, const mbg_sim_params* params, void* stream, mbg_result** out)
@@ -326,6 +290,7 @@ int mbg_simulate(mbg_program* p, const mbg_sim_params* params, void* stream, mbg
 			sp.blocks_per_cu = params->blocks_per_cu;
 			sp.normalization = params->normalization;
 			sp.device_only = (params->flags & MBG_SIM_DEVICE_ONLY) != 0;
+			sp.kernel = params->kernel;
 		}
 		if (sp.block_size < 0 || sp.block_size > 1024 || (sp.block_size % 64) != 0)
 			throw mbg::error(mbg::MBG_E_ARG, "block_size must be a multiple of 64 up to 1024");
@@ -366,6 +331,7 @@ int mbg_result_get_info(const mbg_result* rr, mbg_result_info* out)
 		out->window_mode = r.window_mode;
 		out->final_mode = r.final_mode;
 		out->window_capacity = r.window_capacity;
+		out->kernel = r.kernel;
 	});
 }
 

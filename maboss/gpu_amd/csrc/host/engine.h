@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "../device/mbg_kernel_args.h"
 #include "generator.h"
 #include "model.h"
 
@@ -16,6 +17,20 @@ namespace mbg
 
 // Throws mbg::error(MBG_E_HIP) with the HIP error string.
 void hip_check(hipError_t e, const char* what);
+
+// Tables of the wave-per-trajectory kernel (built from the model on the host).
+struct wave_tables
+{
+	int npl = 1;
+	std::vector<mbg_wnode> nodes;
+	std::vector<unsigned> child_off;
+	std::vector<unsigned short> child;
+	std::vector<unsigned short> obs;
+	std::vector<char> image; // nodes, child_off, child, obs; padded to 16 bytes
+};
+
+// Throws MBG_E_LIMIT with the reason when the model does not fit the wave kernel.
+wave_tables build_wave_tables(const model& m);
 
 // Device allocation that only grows; reused by successive runs of a program.
 struct device_buffer
@@ -33,7 +48,7 @@ struct device_buffer
 struct program
 {
 	// run workspace (kept between runs so repeated launches do not reallocate)
-	device_buffer b_counters, b_win_vals, b_win_keys, b_ent, b_fin_vals, b_fin_keys, b_fp, b_istate, b_pv;
+	device_buffer b_counters, b_win_vals, b_win_keys, b_ent, b_fin_vals, b_fin_keys, b_fp, b_istate, b_pv, b_wave;
 
 	std::shared_ptr<const model> mdl;
 	generated gen;
@@ -48,8 +63,15 @@ struct program
 	int device = 0;
 	int num_cus = 0;
 
+	// wave-per-trajectory kernel (precompiled, table driven)
+	hipModule_t mod_wave = nullptr;
+	std::unique_ptr<wave_tables> wtab;
+	std::string wave_unsupported; // reason, when the model does not fit the wave kernel
+
 	~program();
 	hipFunction_t function(bool param);
+	// Kernel for the wave path at the model's nodes-per-lane (loads on first use).
+	hipFunction_t wave_function();
 };
 
 std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const gen_options& opt, bool no_cache);
@@ -68,6 +90,7 @@ struct sim_params
 	uint64_t window_capacity = 0, final_capacity = 0, fixed_capacity = 0;
 	int block_size = 0, blocks_per_cu = 0;
 	double normalization = 0;
+	int kernel = -1; // -1 auto, 0 lane per trajectory, 1 wave per trajectory
 	bool device_only = false; // leave the histograms in HBM
 };
 
@@ -90,6 +113,7 @@ struct result
 	uint64_t transitions = 0, finished = 0, truncated = 0;
 	double kernel_ms = 0, total_ms = 0;
 	int window_mode = 0, final_mode = 0;
+	int kernel = 0; // 0 lane, 1 wave
 	uint64_t window_capacity = 0;
 
 	double prob(uint64_t raw) const;     // window cell -> probability

@@ -23,6 +23,7 @@
 extern "C" {
 extern const char mbg_hdr_numeric[];
 extern const char mbg_hdr_abi[];
+extern const unsigned char mbg_co_wave[];
 }
 
 namespace mbg
@@ -182,6 +183,23 @@ program::~program()
 		(void)hipModuleUnload(mod_plain);
 	if (mod_param)
 		(void)hipModuleUnload(mod_param);
+	if (mod_wave)
+		(void)hipModuleUnload(mod_wave);
+}
+
+hipFunction_t program::wave_function()
+{
+	if (!wtab)
+		throw error(MBG_E_LIMIT, "the wave kernel cannot run this model: " + wave_unsupported);
+	if (!mod_wave)
+		hip_check(hipModuleLoadData(&mod_wave, mbg_co_wave), "hipModuleLoadData(simulate_wave)");
+	int n = 1;
+	while (n < wtab->npl)
+		n <<= 1;
+	hipFunction_t fn = nullptr;
+	const std::string name = "simulate_wave_n" + std::to_string(n);
+	hip_check(hipModuleGetFunction(&fn, mod_wave, name.c_str()), "hipModuleGetFunction(simulate_wave)");
+	return fn;
 }
 
 hipFunction_t program::function(bool param)
@@ -229,9 +247,17 @@ std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const g
 {
 	auto p = std::make_unique<program>();
 	p->W = m->n_words();
-	if (p->W > max_lane_words)
+	try
+	{
+		p->wtab = std::make_unique<wave_tables>(build_wave_tables(*m));
+	}
+	catch (const error& e)
+	{
+		p->wave_unsupported = e.what();
+	}
+	if (p->W > max_lane_words && !p->wtab)
 		throw error(MBG_E_LIMIT, "models with more than " + std::to_string(32 * max_lane_words)
-									 + " nodes are not supported by this build");
+									 + " nodes need the wave kernel, which cannot run this one: " + p->wave_unsupported);
 	{
 		scope_timer tm("generator> generate");
 		p->gen = generate(*m, opt);
@@ -245,7 +271,10 @@ std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const g
 	if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
 		throw error(MBG_E_HIP, std::string("this build targets gfx950; current device is ") + prop.gcnArchName);
 	p->num_cus = prop.multiProcessorCount;
-	p->function(false);
+	if (p->W <= max_lane_words)
+		p->function(false);
+	else
+		p->wave_function();
 	return p;
 }
 

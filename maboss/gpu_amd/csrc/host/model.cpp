@@ -717,6 +717,57 @@ float model::evaluate(const expr& e) const
 	throw error(MBG_E_INTERNAL, "bad expression");
 }
 
+namespace
+{
+float interp(const model& m, const node_t& cur, const expr& e, const uint32_t* s, int depth)
+{
+	if (depth > 64)
+		throw error(MBG_E_MODEL, "alias nesting too deep");
+	auto f = [&](const expr& x) { return interp(m, cur, x, s, depth); };
+	switch (e.kind)
+	{
+		case op::node: {
+			int i = m.node_index(e.name);
+			if (i < 0)
+				throw error(MBG_E_MODEL, "unknown node name: " + e.name);
+			return (float)((s[i / 32] >> (i % 32)) & 1u);
+		}
+		case op::alias: {
+			const expr* t = cur.attr(e.name.substr(1));
+			if (!t)
+				throw error(MBG_E_MODEL, "Node " + cur.name + " does not have attribute " + e.name.substr(1));
+			return interp(m, cur, *t, s, depth + 1);
+		}
+		case op::lit: return e.value;
+		case op::var: return m.variables.at(e.name);
+		case op::paren:
+		case op::pos: return f(*e.a);
+		case op::neg: return -f(*e.a);
+		case op::lnot: return !f(*e.a);
+		case op::add: return f(*e.a) + f(*e.b);
+		case op::sub: return f(*e.a) - f(*e.b);
+		case op::mul: return f(*e.a) * f(*e.b);
+		case op::div: return f(*e.a) / f(*e.b);
+		case op::land: return f(*e.a) && f(*e.b);
+		case op::lor: return f(*e.a) || f(*e.b);
+		case op::lxor: return (f(*e.a) != 0) != (f(*e.b) != 0);
+		case op::eq: return f(*e.a) == f(*e.b);
+		case op::ne: return f(*e.a) != f(*e.b);
+		case op::lt: return f(*e.a) < f(*e.b);
+		case op::le: return f(*e.a) <= f(*e.b);
+		case op::gt: return f(*e.a) > f(*e.b);
+		case op::ge: return f(*e.a) >= f(*e.b);
+		case op::ternary: return f(*e.a) ? f(*e.b) : f(*e.c);
+	}
+	return 0.f;
+}
+} // namespace
+
+float model::eval_in_state(const node_t& cur, const expr& e, const uint32_t* state) const
+{
+	return interp(*this, cur, e, state, 0);
+}
+
 void parse_bnd(model& m, const std::string& text, const std::string& fname)
 {
 	parser p(m, text, fname);

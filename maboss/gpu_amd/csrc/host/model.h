@@ -123,6 +123,10 @@ struct model
 	// Constant evaluation with the reference's float semantics
 	// (src/parser/expressions.cpp:10-23, :50-81, :156-159).
 	float evaluate(const expr& e) const;
+
+	// Value of e in a state (node references read the state's bits, @aliases
+	// resolve against node cur), with the same float semantics.
+	float eval_in_state(const node_t& cur, const expr& e, const uint32_t* state) const;
 };
 
 // Parse a .bnd text then a .cfg text into a model (src/parser/driver.cpp:16-31).

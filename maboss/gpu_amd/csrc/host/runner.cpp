@@ -175,7 +175,13 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		res->names.push_back(n.name);
 	res->observed = m.observed_nodes();
 
-	hipFunction_t fn = p.function(false);
+	const bool use_wave = sp.kernel == 1 || (sp.kernel < 0 && W > max_lane_words);
+	if (!use_wave && W > max_lane_words)
+		throw error(MBG_E_LIMIT, "the lane kernel serves up to " + std::to_string(32 * max_lane_words) + " nodes");
+	if (use_wave && p.opts.entropy == entropy_formula::reference)
+		throw error(MBG_E_ARG, "the wave kernel computes the log-sum entropy only");
+	hipFunction_t fn = use_wave ? p.wave_function() : p.function(false);
+	res->kernel = use_wave ? 1 : 0;
 	plan pl = make_plan(m, sp, n_traj, windows, n_obs);
 
 	for (int attempt = 0;; attempt++)
@@ -255,15 +261,44 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 
 		// ---- launch: a persistent grid filling every CU
 		const int block = sp.block_size > 0 ? sp.block_size : 256;
+		{
+			int max_threads = 0;
+			hip_check(hipFuncGetAttribute(&max_threads, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, fn),
+					  "hipFuncGetAttribute");
+			if (block > max_threads || block % 64 != 0)
+				throw error(MBG_E_ARG, "block_size " + std::to_string(block) + " is not a multiple of 64 up to "
+										   + std::to_string(max_threads) + " for this kernel");
+		}
+		int lds_total = pl.lds_bytes;
+		uint64_t work_per_block = (uint64_t)block; // trajectories in flight per block
+		if (use_wave)
+		{
+			const wave_tables& wt = *p.wtab;
+			auto* img = (char*)p.b_wave.ensure(wt.image.size());
+			hip_check(hipMemcpyAsync(img, wt.image.data(), wt.image.size(), hipMemcpyHostToDevice, stream),
+					  "hipMemcpyAsync");
+			a.wt.nodes = (const mbg_wnode*)img;
+			a.wt.child_off = (const unsigned*)(img + wt.nodes.size() * sizeof(mbg_wnode));
+			a.wt.child = (const unsigned short*)(a.wt.child_off + wt.child_off.size());
+			a.wt.obs = a.wt.child + wt.child.size();
+			a.wt.n_child = (int)wt.child.size();
+			a.wt.npl = wt.npl;
+			a.wt.tab_bytes = (int)wt.image.size();
+			a.wave_lds_off = (pl.lds_bytes + 15) & ~15;
+			lds_total = a.wave_lds_off + a.wt.tab_bytes + (block / 64) * wt.npl * 64 * 4;
+			if (lds_total > 160 * 1024)
+				throw error(MBG_E_LIMIT, "wave kernel tables do not fit in LDS; use a smaller block size");
+			work_per_block = (uint64_t)(block / 64);
+		}
 		int per_cu = sp.blocks_per_cu;
 		if (per_cu <= 0)
 		{
-			hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, pl.lds_bytes),
+			hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds_total),
 					  "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
 			per_cu = std::max(per_cu, 1);
 		}
-		uint64_t lanes_needed = (n_traj + block - 1) / block;
-		uint64_t grid = std::min<uint64_t>((uint64_t)p.num_cus * per_cu, std::max<uint64_t>(lanes_needed, 1));
+		uint64_t blocks_needed = (n_traj + work_per_block - 1) / work_per_block;
+		uint64_t grid = std::min<uint64_t>((uint64_t)p.num_cus * per_cu, std::max<uint64_t>(blocks_needed, 1));
 
 		hipEvent_t e0, e1;
 		hip_check(hipEventCreate(&e0), "hipEventCreate");
@@ -272,7 +307,7 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		if (n_traj > 0)
 		{
 			void* args[] = { &a };
-			hip_check(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, block, 1, 1, pl.lds_bytes, stream, args, nullptr),
+			hip_check(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, block, 1, 1, lds_total, stream, args, nullptr),
 					  "hipModuleLaunchKernel(simulate)");
 		}
 		hip_check(hipEventRecord(e1, stream), "hipEventRecord");

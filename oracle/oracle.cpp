@@ -33,6 +33,7 @@
 //   --threads T      OpenMP threads (default: all)
 //   --compiled       compile the formulas with g++ instead of interpreting
 //   --entropy E      log_sum (default) or reference
+//   --order O        seq (default: lane kernel order) or wave (wave kernel order)
 //   --max-steps K    step cap per trajectory (default 2^30)
 //   --raw PATH       exact integer dump (same format as the engine's)
 //   -o PREFIX        write PREFIX_probtraj.csv and PREFIX_fp.csv
@@ -780,7 +781,94 @@ struct Sim
 	std::vector<ExP> up, down;
 	rates_fn compiled = nullptr;
 	bool reference_entropy = false;
+	// Wave order: the summation order of the wave-per-trajectory kernel.
+	// Node i belongs to lane i / npl (64 lanes, npl = ceil(N / 64) nodes
+	// each); lane sums are sequential over the lane's slice and combined by
+	// an inclusive Kogge-Stone scan (offsets 1, 2, 4, ..., 32).
+	bool wave = false;
+	int npl = 1;
 	uint32_t seed_lo = 0, seed_hi = 0, variant = 0, max_steps = 1u << 30;
+
+	static void ks_scan(float* v)
+	{
+		for (int off = 1; off < 64; off <<= 1)
+		{
+			float nv[64];
+			for (int l = 0; l < 64; l++)
+				nv[l] = l >= off ? v[l] + v[l - off] : v[l];
+			std::memcpy(v, nv, sizeof(nv));
+		}
+	}
+
+	// Total rate; in wave order also the lane sums L and scan P.
+	float total_of(const float* r, float* L, float* P) const
+	{
+		if (!wave)
+		{
+			float total = 0.0f;
+			for (int i = 0; i < N; i++)
+				total = total + r[i];
+			return total;
+		}
+		for (int l = 0; l < 64; l++)
+		{
+			float acc = 0.0f;
+			for (int k = 0; k < npl; k++)
+			{
+				const int i = l * npl + k;
+				acc = acc + (i < N ? r[i] : 0.0f);
+			}
+			L[l] = P[l] = acc;
+		}
+		ks_scan(P);
+		return P[63];
+	}
+
+	// Flip selection for rr = u2 * total.
+	int select_flip(const float* r, float total, float rr, const float* L, const float* P) const
+	{
+		if (!wave)
+		{
+			// number of prefix sums strictly below rr (simulation.cu:9-21)
+			float acc = 0.0f;
+			int flip = 0;
+			for (int i = 0; i < N; i++)
+			{
+				acc = acc + r[i];
+				flip += (acc < rr) ? 1 : 0;
+			}
+			return flip;
+		}
+		(void)total;
+		// first lane whose scan reaches rr and whose slice has weight; else
+		// the last lane with weight
+		int ls = -1;
+		for (int l = 0; l < 64 && ls < 0; l++)
+			if (P[l] >= rr && L[l] > 0.0f)
+				ls = l;
+		if (ls < 0)
+			for (int l = 63; l >= 0 && ls < 0; l--)
+				if (L[l] > 0.0f)
+					ls = l;
+		const float rem = rr - (ls == 0 ? 0.0f : P[ls - 1]);
+		// first slot whose running sum reaches rem with a nonzero rate; else
+		// the last slot with a nonzero rate
+		float q = 0.0f;
+		int ks = -1, last = -1;
+		for (int k = 0; k < npl; k++)
+		{
+			const int i = ls * npl + k;
+			const float v = i < N ? r[i] : 0.0f;
+			q = q + v;
+			if (v > 0.0f)
+			{
+				last = k;
+				if (ks < 0 && q >= rem)
+					ks = k;
+			}
+		}
+		return ls * npl + (ks >= 0 ? ks : last);
+	}
 
 	void rates(const uint32_t* s, float* r) const
 	{
@@ -826,6 +914,30 @@ struct Sim
 		// H = log2(S) - T/S, T = sum r log2 r; 0 if at most one rate is nonzero
 		float S = 0.0f, T = 0.0f;
 		int nnz = 0;
+		if (wave)
+		{
+			// observed node k contributes to lane k % 64 (sequentially), then
+			// Kogge-Stone scans of the lane values
+			float Sl[64], Tl[64];
+			for (int l = 0; l < 64; l++)
+			{
+				Sl[l] = 0.0f;
+				Tl[l] = 0.0f;
+				for (int k = l; k < nobs; k += 64)
+				{
+					const float v = r[obs[k]];
+					Sl[l] = Sl[l] + v;
+					Tl[l] = Tl[l] + (v == 0.0f ? 0.0f : v * log2_poly(v));
+					nnz += v != 0.0f;
+				}
+			}
+			ks_scan(Sl);
+			ks_scan(Tl);
+			S = Sl[63];
+			T = Tl[63];
+			const float h = log2_poly(S) - T / S;
+			return (nnz > 1 && h > 0.0f) ? h : 0.0f;
+		}
 		for (int i : obs)
 		{
 			S = S + r[i];
@@ -916,9 +1028,8 @@ void trajectory(const Sim& S, uint64_t id, Stats& st, std::vector<float>& r, std
 			u2 = u01(x.v[2 * (k & 1) + 1]);
 		}
 		S.rates(s.data(), r.data());
-		float total = 0.0f;
-		for (int i = 0; i < S.N; i++)
-			total = total + r[i];
+		float L[64], P[64];
+		const float total = S.total_of(r.data(), L, P);
 		const float tp = t;
 		bool fixed = false;
 		float H = 0.0f;
@@ -987,15 +1098,7 @@ void trajectory(const Sim& S, uint64_t id, Stats& st, std::vector<float>& r, std
 			st.truncated++;
 			return;
 		}
-		// flip: number of prefix sums strictly below u2 * total (simulation.cu:9-21)
-		const float rr = u2 * total;
-		float acc = 0.0f;
-		int flip = 0;
-		for (int i = 0; i < S.N; i++)
-		{
-			acc = acc + r[i];
-			flip += (acc < rr) ? 1 : 0;
-		}
+		const int flip = S.select_flip(r.data(), total, u2 * total, L, P);
 		s[flip >> 5] ^= 1u << (flip & 31);
 	}
 }
@@ -1077,7 +1180,7 @@ int main(int argc, char** argv)
 	long long n_override = -1, first = 0, seed_override = -1;
 	int threads = 0;
 	bool compiled = false, timing = false, eval_states = false;
-	std::string raw, prefix, print_path, entropy = "log_sum";
+	std::string raw, prefix, print_path, entropy = "log_sum", order = "seq";
 	unsigned variant = 0, max_steps = 1u << 30;
 	for (int i = 1; i < argc; i++)
 	{
@@ -1099,6 +1202,8 @@ int main(int argc, char** argv)
 			threads = std::stoi(next());
 		else if (a == "--compiled")
 			compiled = true;
+		else if (a == "--order")
+			order = next();
 		else if (a == "--entropy")
 			entropy = next();
 		else if (a == "--max-steps")
@@ -1152,6 +1257,10 @@ int main(int argc, char** argv)
 	S.tick = m.consts["time_tick"];
 	S.windows = (int)std::ceil(S.max_time / S.tick);
 	S.reference_entropy = entropy == "reference";
+	S.wave = order == "wave";
+	S.npl = (S.N + 63) / 64;
+	if (S.wave && S.reference_entropy)
+		die("the wave order uses the log-sum entropy");
 	S.max_steps = max_steps;
 	S.variant = variant;
 	uint64_t seed = seed_override >= 0 ? (uint64_t)seed_override : (uint64_t)m.consts_exact["seed_pseudorandom"];
@@ -1173,17 +1282,9 @@ int main(int argc, char** argv)
 					return 0;
 			std::cin >> std::hex >> u2b;
 			S.rates(st.data(), r.data());
-			float total = 0.0f;
-			for (int i = 0; i < S.N; i++)
-				total = total + r[i];
-			const float rr = bitsf(u2b) * total;
-			float acc = 0.0f;
-			int flip = 0;
-			for (int i = 0; i < S.N; i++)
-			{
-				acc = acc + r[i];
-				flip += (acc < rr) ? 1 : 0;
-			}
+			float L[64], P[64];
+			const float total = S.total_of(r.data(), L, P);
+			const int flip = total == 0.0f ? 0 : S.select_flip(r.data(), total, bitsf(u2b) * total, L, P);
 			std::printf("%08x %08x %d %llx\n", fbits(total), fbits(S.entropy(r.data())), flip,
 						(unsigned long long)S.obs_index(st.data()));
 		}

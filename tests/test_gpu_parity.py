@@ -84,18 +84,54 @@ def test_workloads_bit_exact(tmp_path, wl, name, n):
         assert r.info.window_mode == 1  # 50 observed nodes: hashed histogram
 
 
+@pytest.mark.parametrize("name,n", [("plain_small", 4000), ("plain_100", 2000), ("plain_100_discrete", 2000),
+                                    ("grad_30", 3000), ("scale_1000", 300)])
+def test_wave_kernel_bit_exact(tmp_path, wl, name, n):
+    """Wave-per-trajectory kernel against the oracle's wave summation order."""
+    bnd, cfg = wl[name]
+    g, r = gpu_raw(tmp_path, bnd, cfg, kernel="wave", n_trajectories=n)
+    assert r.info.kernel == 1
+    assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=n, extra=("--order", "wave")))
+
+
+def test_wave_kernel_reference_model(tmp_path):
+    bnd, cfg = model_paths("cellcycle")
+    g, _ = gpu_raw(tmp_path, bnd, cfg, kernel="wave", n_trajectories=20000)
+    assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=20000, extra=("--order", "wave")))
+    # nodes with more than 6 inputs do not fit the wave kernel's truth tables
+    bnd, cfg = model_paths("metastasis")
+    with pytest.raises(mg.MbgError) as e:
+        program(bnd, cfg).simulate(n_trajectories=10, kernel="wave")
+    assert e.value.code == 5
+
+
+def test_scale_1000_auto_uses_wave(tmp_path, wl):
+    bnd, cfg = wl["scale_1000"]
+    _, r = gpu_raw(tmp_path, bnd, cfg, n_trajectories=64)
+    assert r.info.kernel == 1 and r.info.finished == 64
+
+
 def test_reference_entropy_formula(tmp_path):
     bnd, cfg = model_paths("metastasis")
     g, _ = gpu_raw(tmp_path, bnd, cfg, entropy=1, n_trajectories=3000)
     assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=3000, extra=("--entropy", "reference")))
 
 
-@pytest.mark.parametrize("block,per_cu", [(64, 1), (128, 3), (512, 1), (1024, 1), (256, 0)])
+@pytest.mark.parametrize("block,per_cu", [(64, 1), (128, 3), (192, 2), (256, 1), (256, 0)])
 def test_launch_shape_invariance(tmp_path, wl, block, per_cu):
     for bnd, cfg in (model_paths("cellcycle"), wl["plain_100"]):
         a, _ = gpu_raw(tmp_path, bnd, cfg, "a", n_trajectories=5000, block_size=block, blocks_per_cu=per_cu)
         b, _ = gpu_raw(tmp_path, bnd, cfg, "b", n_trajectories=5000)
         assert_same(a, b)
+
+
+def test_block_size_validated():
+    """Block sizes beyond the kernel's launch bound are refused on the host."""
+    bnd, cfg = model_paths("cellcycle")
+    for block in (512, 96):
+        with pytest.raises(mg.MbgError) as e:
+            program(bnd, cfg).simulate(n_trajectories=100, block_size=block)
+        assert e.value.code == 1
 
 
 @pytest.mark.parametrize("win,fin", [("dense", "dense"), ("hash", "hash"), ("lds", "lds"), ("hash", "dense")])
