@@ -136,6 +136,47 @@ void mbg_sim_params_default(mbg_sim_params* p);
  * before returning (the stream is synchronised). */
 int mbg_simulate(mbg_program* p, const mbg_sim_params* params, void* stream, mbg_result** out);
 
+/* ---- batched variants.  No counterpart in the reference: its rates are
+ *      baked into the generated code (src/generator.cpp:104-106), one model
+ *      runs per process (src/main.cpp:154-164) and mutants are a TODO
+ *      (README.md:33).  Variant v draws from the stream tagged v, so its
+ *      result equals a plain run with variant = v and the same rates. ---- */
+typedef struct mbg_variant
+{
+	int64_t n_trajectories;
+	float max_time;           /* <= 0: the model's max_time (tick and windows follow) */
+	int32_t pad;
+	const float* rates;       /* [n_params] in mbg_model_param_name order, host memory; NULL: the model's */
+	const int8_t* mutations;  /* [n_nodes]: +1 forced on (never switches off, starts on), -1 forced
+	                             off, 0 free; host memory; NULL: none */
+} mbg_variant;
+
+typedef struct mbg_batch_params
+{
+	int64_t seed;             /* <0: seed_pseudorandom of the model */
+	uint32_t max_steps;       /* 0: default cap */
+	int32_t block_size;       /* 0: autotuned (64, 128 or 256) */
+	int32_t blocks_per_cu;    /* 0: autotuned / occupancy */
+	int32_t traj_per_lane;    /* a work chunk holds 64*traj_per_lane ids of one variant; 0: autotuned */
+	int32_t use_graph;        /* 1: capture the launch in a hipGraph and replay it */
+	int32_t flags;            /* MBG_SIM_DEVICE_ONLY */
+	int32_t reserved[8];
+} mbg_batch_params;
+
+typedef struct mbg_batch_info
+{
+	double kernel_ms;         /* device time of the batched launch (graph or kernel) */
+	double total_ms;          /* host wall time of mbg_simulate_batched */
+	uint64_t transitions;     /* all variants */
+	int32_t block_size, blocks_per_cu, traj_per_lane, used_graph;
+} mbg_batch_info;
+
+void mbg_batch_params_default(mbg_batch_params* p);
+/* Simulate n_variants variants in one launch.  results[v] receives variant
+ * v's statistics (free each with mbg_result_free); info may be NULL. */
+int mbg_simulate_batched(mbg_program* p, const mbg_batch_params* params, int32_t n_variants,
+						 const mbg_variant* variants, void* stream, mbg_result** results, mbg_batch_info* info);
+
 /* ---- results (replaces stats::finalize / visualize / write_csv,
  *      src/statistics/stats.h:15-29) ---- */
 void mbg_result_free(mbg_result* r);

@@ -16,8 +16,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._capi import (CLI_PATH, CompileOpts, MbgError, ModelInfo, ResultInfo,
-                    SimParams, check, lib)
+from ._capi import (CLI_PATH, BatchInfo, BatchParams, CompileOpts, MbgError, ModelInfo,
+                    ResultInfo, SimParams, Variant, check, lib)
 
 __all__ = ["Model", "Program", "Result", "simulate", "MbgError", "run_cli"]
 
@@ -147,6 +147,53 @@ class Program:
         s = ctypes.c_void_p(stream) if stream else None
         check(lib().mbg_simulate(self._h, ctypes.byref(p), s, ctypes.byref(h)))
         return Result(h, self.model)
+
+    def simulate_batched(self, variants, seed: int = -1, max_steps: int = 0, block_size: int = 0,
+                         blocks_per_cu: int = 0, traj_per_lane: int = 0, use_graph: bool = True,
+                         device_only: bool = False, stream=None):
+        """Run several variants of the model in one launch.
+
+        ``variants`` is a list of dicts with ``n_trajectories`` and optional
+        ``max_time``, ``rates`` (one per ``Model.param_names``) and
+        ``mutations`` (one int per node: +1 on, -1 off, 0 free).  Returns
+        ``(results, info)``: one :class:`Result` per variant and the launch's
+        :class:`BatchInfo`.
+        """
+        n_nodes, n_params = self.model.info.n_nodes, self.model.info.n_params
+        arr = (Variant * len(variants))()
+        keep = []
+        for v, spec in zip(arr, variants):
+            v.n_trajectories = int(spec["n_trajectories"])
+            v.max_time = float(spec.get("max_time", 0.0))
+            if spec.get("rates") is not None:
+                r = np.ascontiguousarray(spec["rates"], dtype=np.float32)
+                if r.shape != (n_params,):
+                    raise ValueError(f"rates must have {n_params} entries")
+                keep.append(r)
+                v.rates = r.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            if spec.get("mutations") is not None:
+                mu = np.ascontiguousarray(spec["mutations"], dtype=np.int8)
+                if mu.shape != (n_nodes,):
+                    raise ValueError(f"mutations must have {n_nodes} entries")
+                keep.append(mu)
+                v.mutations = mu.ctypes.data_as(ctypes.POINTER(ctypes.c_int8))
+        bp = BatchParams()
+        lib().mbg_batch_params_default(ctypes.byref(bp))
+        bp.seed = seed
+        bp.max_steps = max_steps
+        bp.block_size = block_size
+        bp.blocks_per_cu = blocks_per_cu
+        bp.traj_per_lane = traj_per_lane
+        bp.use_graph = int(use_graph)
+        bp.flags = 1 if device_only else 0
+        outs = (ctypes.c_void_p * len(variants))()
+        info = BatchInfo()
+        s = ctypes.c_void_p(stream) if stream else None
+        check(lib().mbg_simulate_batched(self._h, ctypes.byref(bp), len(variants), arr, s,
+                                         ctypes.cast(outs, ctypes.POINTER(ctypes.c_void_p)),
+                                         ctypes.byref(info)))
+        del keep
+        return [Result(ctypes.c_void_p(h), self.model) for h in outs], info
 
 
 @dataclass

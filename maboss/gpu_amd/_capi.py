@@ -31,6 +31,7 @@ EXPORTED = [
     "mbg_generate_code", "mbg_free_string", "mbg_program_compile", "mbg_program_free",
     "mbg_program_compile_seconds",
     "mbg_sim_params_default", "mbg_simulate",
+    "mbg_batch_params_default", "mbg_simulate_batched",
     "mbg_result_free", "mbg_result_get_info", "mbg_result_window_size", "mbg_result_window",
     "mbg_result_window_probs", "mbg_result_entropies", "mbg_result_final_size",
     "mbg_result_final", "mbg_result_fixed_size", "mbg_result_fixed",
@@ -78,6 +79,32 @@ class SimParams(ctypes.Structure):
     ]
 
 
+class Variant(ctypes.Structure):
+    _fields_ = [
+        ("n_trajectories", ctypes.c_int64), ("max_time", ctypes.c_float),
+        ("pad", ctypes.c_int32), ("rates", ctypes.POINTER(ctypes.c_float)),
+        ("mutations", ctypes.POINTER(ctypes.c_int8)),
+    ]
+
+
+class BatchParams(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_int64), ("max_steps", ctypes.c_uint32),
+        ("block_size", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
+        ("traj_per_lane", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+        ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+        ("transitions", ctypes.c_uint64), ("block_size", ctypes.c_int32),
+        ("blocks_per_cu", ctypes.c_int32), ("traj_per_lane", ctypes.c_int32),
+        ("used_graph", ctypes.c_int32),
+    ]
+
+
 class ResultInfo(ctypes.Structure):
     _fields_ = [
         ("windows", ctypes.c_int32), ("n_observed", ctypes.c_int32),
@@ -122,6 +149,9 @@ def _declare(lib):
         "mbg_program_compile_seconds": ([P], ctypes.c_double),
         "mbg_sim_params_default": ([ctypes.POINTER(SimParams)], None),
         "mbg_simulate": ([P, ctypes.POINTER(SimParams), P, pp], ctypes.c_int),
+        "mbg_batch_params_default": ([ctypes.POINTER(BatchParams)], None),
+        "mbg_simulate_batched": ([P, ctypes.POINTER(BatchParams), i32, ctypes.POINTER(Variant), P, pp,
+                                  ctypes.POINTER(BatchInfo)], ctypes.c_int),
         "mbg_result_free": ([P], None),
         "mbg_result_get_info": ([P, ctypes.POINTER(ResultInfo)], ctypes.c_int),
         "mbg_result_window_size": ([P, i32], i64),

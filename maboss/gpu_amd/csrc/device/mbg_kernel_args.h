@@ -18,7 +18,8 @@
 #define MBG_CTR_FP_TIMEOUT 5     // fixed-point slot never published (bounded wait ended)
 #define MBG_CTR_QUEUE 6          // trajectory work queue
 #define MBG_CTR_FINISHED 7       // trajectories ended at max_time or a fixed point
-#define MBG_NCOUNTERS 8
+#define MBG_CTR_TAIL_OVERFLOW 8  // tail table full
+#define MBG_NCOUNTERS 10
 
 #define MBG_EMPTY_KEY 0xffffffffffffffffull
 
@@ -72,9 +73,42 @@ struct mbg_sim_args
 	unsigned long long fin_mask;
 	unsigned int* fp_table;         // slots of (2 + W) u32: tag, count, state words
 	unsigned long long fp_mask;
+	// Tail records: the final record of a trajectory adds the same quantum
+	// to every window after its first one, so those windows are counted per
+	// (first full window, observed state) here and expanded after the run.
+	unsigned long long* tail_table; // hash slots {key, count}; null: per-window adds
+	unsigned long long tail_mask;
 	unsigned long long* counters;   // [MBG_NCOUNTERS]
 	const mbg_param_view* pv;       // per-variant tables (batched/gradient); null for plain
 	mbg_wave_tables wt;             // wave-per-trajectory kernel only
 	int wave_lds_off;               // byte offset of the per-wave rate arrays in LDS
 	int pad2;
+
+	// batched form only
+	const struct mbg_batch_variant* bv;  // [n_variants]
+	const struct mbg_batch_chunk* chunks; // work list, each chunk inside one variant
+	unsigned int n_chunks, pad3;
+	unsigned long long* vctr;            // per-variant {transitions, finished, truncated}
+};
+
+// One variant of the batched form: its own rates, mutant masks, initial
+// probabilities, max_time / windows and slices of the statistics storage.
+struct mbg_batch_variant
+{
+	unsigned long long n_traj;
+	unsigned long long win_off; // first window cell of [windows << n_obs]
+	unsigned long long fin_off; // first final-state cell of [2^n_obs]
+	unsigned int ent_off;       // first entropy sum of [windows]
+	int windows;
+	float max_time;
+	unsigned int ctr_off; // first of its 3 counters in vctr
+	const float* istate;  // [n_nodes]
+	mbg_param_view pv;
+};
+
+// A run of trajectory ids [begin, end) of one variant, taken by one wave.
+struct mbg_batch_chunk
+{
+	unsigned int variant, pad;
+	unsigned long long begin, end;
 };

@@ -1,0 +1,182 @@
+// Per-trajectory pieces of the lane-per-trajectory kernels (plain and
+// batched): initial state, the uniforms of a step, the time advance and the
+// folding of one record into the window statistics.  Semantics in
+// src/jit_kernels/simulation.cu:34-71, :106-163 and
+// src/jit_kernels/window_average_small.cu:50-136; stream layout in
+// mbg_numeric.h.
+#pragma once
+
+#include "mbg_stats_device.h"
+
+namespace mbg_dev
+{
+
+// bit i = (u_i <= istate_i), u_i = draw i of trajectory id's stream
+template <int W>
+__device__ __forceinline__ void init_state(uint32_t (&s)[W], uint64_t id, uint32_t variant, uint32_t seed_lo,
+										   uint32_t seed_hi, int n_nodes, const float* istate)
+{
+#pragma unroll
+	for (int w = 0; w < W; ++w)
+	{
+		uint32_t word = 0u;
+		for (int b = 0; b < 32; b += 4)
+		{
+			const int i = w * 32 + b;
+			if (i >= n_nodes)
+				break;
+			const mbg_u32x4 r = mbg_philox4x32_10(
+				mbg_u32x4 { (uint32_t)(i >> 2), (uint32_t)id, (uint32_t)(id >> 32), variant }, seed_lo, seed_hi);
+			const uint32_t rw[4] = { r.x, r.y, r.z, r.w };
+#pragma unroll
+			for (int j = 0; j < 4; ++j)
+				if (i + j < n_nodes && mbg_u01(rw[j]) <= istate[i + j])
+					word |= 1u << (b + j);
+		}
+		s[w] = word;
+	}
+}
+
+// u1 (dwell) and u2 (flip) of step k; discrete time draws u2 only
+__device__ __forceinline__ void step_uniforms(bool discrete, uint32_t B0, uint32_t k, uint64_t id, uint32_t variant,
+											  uint32_t seed_lo, uint32_t seed_hi, float& u1, float& u2)
+{
+	if (discrete)
+	{
+		const mbg_u32x4 r = mbg_philox4x32_10(mbg_u32x4 { B0 + (k >> 2), (uint32_t)id, (uint32_t)(id >> 32), variant },
+											  seed_lo, seed_hi);
+		u1 = 1.0f;
+		u2 = mbg_u01(pick(r, k & 3u));
+	}
+	else
+	{
+		const mbg_u32x4 r = mbg_philox4x32_10(mbg_u32x4 { B0 + (k >> 1), (uint32_t)id, (uint32_t)(id >> 32), variant },
+											  seed_lo, seed_hi);
+		u1 = mbg_u01((k & 1u) ? r.z : r.x);
+		u2 = mbg_u01((k & 1u) ? r.w : r.y);
+	}
+}
+
+// New time after a step with this total rate (simulation.cu:114-131)
+__device__ __forceinline__ float advance_time(bool discrete, float t, float total, float u1, float tick, float max_time,
+											  bool& fixed)
+{
+	if (total == 0.0f)
+	{
+		fixed = true;
+		return max_time;
+	}
+	if (discrete)
+	{
+		t = t + tick;
+		t = (max_time - tick < t) ? max_time : t;
+	}
+	else
+		t = t + (-mbg_logf(u1)) / total;
+	return fminf(t, max_time);
+}
+
+// Register-combined contributions of one lane to the window cells and
+// entropy sums; flushed through the callbacks when the cell changes.
+struct pending
+{
+	int pw = -1, pew = -1;
+	uint64_t pobs = 0, pq = 0, peq = 0;
+
+	template <typename FW>
+	__device__ __forceinline__ void win(int w, uint64_t obs, uint64_t q, FW&& flush)
+	{
+		if (w == pw && obs == pobs)
+			pq += q;
+		else
+		{
+			if (pw >= 0)
+				flush(pw, pobs, pq);
+			pw = w;
+			pobs = obs;
+			pq = q;
+		}
+	}
+
+	template <typename FE>
+	__device__ __forceinline__ void ent(int w, uint64_t q, FE&& flush)
+	{
+		if (w == pew)
+			peq += q;
+		else
+		{
+			if (pew >= 0)
+				flush(pew, peq);
+			pew = w;
+			peq = q;
+		}
+	}
+
+	template <typename FW, typename FE>
+	__device__ __forceinline__ void flush_all(FW&& fw, FE&& fe)
+	{
+		if (pw >= 0)
+			fw(pw, pobs, pq);
+		if (pew >= 0)
+			fe(pew, peq);
+		pw = pew = -1;
+	}
+};
+
+// Fold the record (state obs during [tp, t), entropy H) into the windows:
+// continuous time splits the dwell over windows (window_average_small.cu:72-91),
+// discrete time counts one step at window round(tp / tick) (:123-129).
+// With final_tail (the trajectory's last record, t == max_time), the windows
+// after the first one get no per-window addition: ft(w, obs) records them
+// once, to be expanded with the per-window quantum after the run.
+template <typename FW, typename FE, typename FT>
+__device__ __forceinline__ void record(bool discrete, float tp, float t, uint64_t obs, float H, float tick, float qscale,
+									   int windows, pending& p, FW&& fw, FE&& fe, FT&& ft, bool final_tail)
+{
+	if (discrete)
+	{
+		const int w = (int)roundf(tp / tick);
+		if (w < windows)
+		{
+			p.win(w, obs, 1ull, fw);
+			const uint64_t qe = (uint64_t)rintf(H * qscale);
+			if (qe)
+				p.ent(w, qe, fe);
+		}
+		return;
+	}
+	int w = (int)floorf(tp / tick);
+	const int w_first = w;
+	bool tail_done = false;
+	float b = tp;
+	while (t > b)
+	{
+		const float wend = (float)(w + 1) * tick;
+		const float en = fminf(t, wend);
+		const float sl = en - b;
+		if (w < windows)
+		{
+			if (final_tail && w > w_first)
+			{
+				if (!tail_done)
+				{
+					ft(w, obs);
+					tail_done = true;
+				}
+			}
+			else
+			{
+				const uint64_t q = (uint64_t)rintf(sl * qscale);
+				if (q)
+					p.win(w, obs, q, fw);
+			}
+			const uint64_t qe = (uint64_t)rintf((H * sl) * qscale);
+			if (qe)
+				p.ent(w, qe, fe);
+		}
+		w++;
+		b = en;
+	}
+}
+
+} // namespace mbg_dev

@@ -38,9 +38,7 @@ __device__ __forceinline__ void hash_add(unsigned long long* table, uint64_t mas
 	for (uint64_t p = 0; p <= mask; ++p)
 	{
 		unsigned long long* e = table + 2 * slot;
-		unsigned long long k = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		if (k == MBG_EMPTY_KEY)
-			k = atomicCAS(e, MBG_EMPTY_KEY, (unsigned long long)key);
+		const unsigned long long k = atomicCAS(e, MBG_EMPTY_KEY, (unsigned long long)key);
 		if (k == MBG_EMPTY_KEY || k == key)
 		{
 			atomicAdd(e + 1, (unsigned long long)q);
@@ -73,6 +71,12 @@ struct lane_hist
 			atomicAdd(lds + a.ent_lds_off + w, (unsigned long long)q);
 		else
 			atomicAdd(a.win_ent + w, (unsigned long long)q);
+	}
+
+	// One more trajectory whose final record covers windows w0.. in state obs.
+	__device__ void tail(int w0, uint64_t obs) const
+	{
+		hash_add(a.tail_table, a.tail_mask, ((uint64_t)w0 << a.n_obs) | obs, 1ull, a.counters + MBG_CTR_TAIL_OVERFLOW);
 	}
 
 	__device__ void fin(uint64_t obs) const

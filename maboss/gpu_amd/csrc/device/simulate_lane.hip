@@ -1,14 +1,12 @@
-// Fused lane-per-trajectory simulate kernel for gfx950.
+// Fused lane-per-trajectory simulate kernels for gfx950.
 //
 // One lane owns one trajectory at a time, with its W state words, time and
-// step counter in VGPRs.  The kernel is persistent: lanes pull trajectory ids
-// from a wave-aggregated atomic work queue until the queue is drained, so
-// uneven trajectory lengths do not leave lanes idle.  Each SSA step
-// (src/jit_kernels/simulation.cu:106-148) feeds its dwell slice straight into
-// the window histogram (src/jit_kernels/window_average_small.cu:50-136),
-// without writing a trajectory record to HBM; the end of a trajectory feeds
-// the final-state histogram (src/jit_kernels/final_states.cu:9-21) and the
-// fixed-point table (src/statistics/fixed_states.cu:84-125).
+// step counter in VGPRs.  Each SSA step (src/jit_kernels/simulation.cu:
+// 106-148) feeds its dwell slice straight into the window histogram
+// (src/jit_kernels/window_average_small.cu:50-136), without writing a
+// trajectory record to HBM; the end of a trajectory feeds the final-state
+// histogram (src/jit_kernels/final_states.cu:9-21) and the fixed-point table
+// (src/statistics/fixed_states.cu:84-125).
 //
 // All histogram cells are unsigned 64-bit fixed point (dwell * 2^32 / tick,
 // rounded to nearest), so the result does not depend on the order in which
@@ -16,11 +14,20 @@
 // the host oracle.  Consecutive contributions of one lane to the same
 // (window, observed state) cell are summed in registers before one atomic.
 //
-// Built once per state-word count MBG_KW into LLVM bitcode; hipRTC links it
-// with the generated mbg_eval / mbg_eval_p / mbg_obs_index of a model.
+// `simulate` (plain form) is persistent: lanes pull trajectory ids from a
+// wave-aggregated atomic work queue until it is drained, so uneven lengths do
+// not leave lanes idle.  `simulate_batched` (parametric build) runs many
+// model variants in one launch: a wave takes a chunk of ids of one variant,
+// so the variant's rate table is wave-uniform, and its lanes refill from the
+// chunk until it is exhausted.
+//
+// Built once per state-word count MBG_KW (and MBG_PARAM = 0 / 1) into LLVM
+// bitcode; hipRTC links it with the generated mbg_eval / mbg_eval_p /
+// mbg_obs_index of a model.
 #include <hip/hip_runtime.h>
 
 #include "mbg_kernel_args.h"
+#include "mbg_lane_core.h"
 #include "mbg_stats_device.h"
 
 #ifndef MBG_KW
@@ -38,20 +45,23 @@ namespace
 
 constexpr int W = MBG_KW;
 
-// Fixed-point table insert.  Called by one lane of a wave at a time, so the
-// only wait (for a slot another wave is publishing) is on a wave that has
-// already claimed the slot and only has stores left; the wait is bounded.
-__device__ void fp_insert(const mbg_sim_args& a, const uint32_t* s)
+// Fixed-point table insert.  Slots hold {tag, count, key words}; the key is
+// the state (plus the variant in the batched form).  Called by one lane of a
+// wave at a time, so the only wait (for a slot another wave is publishing)
+// is on a wave that has already claimed the slot and only has stores left;
+// the wait is bounded.
+template <int KW>
+__device__ void fp_insert(unsigned int* table, uint64_t mask, const uint32_t (&key)[KW], unsigned long long* counters)
 {
-	constexpr int stride = 2 + W;
+	constexpr int stride = 2 + KW;
 	uint64_t h = 0x9e3779b97f4a7c15ull;
 #pragma unroll
-	for (int w = 0; w < W; ++w)
-		h = mix64(h ^ s[w]);
-	uint64_t slot = h & a.fp_mask;
-	for (uint64_t p = 0; p <= a.fp_mask; ++p)
+	for (int w = 0; w < KW; ++w)
+		h = mix64(h ^ key[w]);
+	uint64_t slot = h & mask;
+	for (uint64_t p = 0; p <= mask; ++p)
 	{
-		unsigned int* e = a.fp_table + slot * stride;
+		unsigned int* e = table + slot * stride;
 		unsigned int tag = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 		if (tag == 0u)
 		{
@@ -59,8 +69,8 @@ __device__ void fp_insert(const mbg_sim_args& a, const uint32_t* s)
 			if (tag == 0u)
 			{
 #pragma unroll
-				for (int w = 0; w < W; ++w)
-					e[2 + w] = s[w];
+				for (int w = 0; w < KW; ++w)
+					e[2 + w] = key[w];
 				__hip_atomic_store(e, 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 				atomicAdd(e + 1, 1u);
 				return;
@@ -73,24 +83,45 @@ __device__ void fp_insert(const mbg_sim_args& a, const uint32_t* s)
 		}
 		if (tag != 2u)
 		{
-			atomicAdd(a.counters + MBG_CTR_FP_TIMEOUT, 1ull);
+			atomicAdd(counters + MBG_CTR_FP_TIMEOUT, 1ull);
 			return;
 		}
 		bool same = true;
 #pragma unroll
-		for (int w = 0; w < W; ++w)
-			same &= __hip_atomic_load(e + 2 + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == s[w];
+		for (int w = 0; w < KW; ++w)
+			same &= __hip_atomic_load(e + 2 + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == key[w];
 		if (same)
 		{
 			atomicAdd(e + 1, 1u);
 			return;
 		}
-		slot = (slot + 1) & a.fp_mask;
+		slot = (slot + 1) & mask;
 	}
-	atomicAdd(a.counters + MBG_CTR_FP_OVERFLOW, 1ull);
+	atomicAdd(counters + MBG_CTR_FP_OVERFLOW, 1ull);
+}
+
+__device__ __forceinline__ void wave_add_counters(unsigned long long* dst, uint64_t ntrans, uint64_t nfin, uint64_t ntrunc)
+{
+	for (int off = 32; off > 0; off >>= 1)
+	{
+		ntrans += __shfl_xor(ntrans, off);
+		nfin += __shfl_xor(nfin, off);
+		ntrunc += __shfl_xor(ntrunc, off);
+	}
+	if (__lane_id() == 0)
+	{
+		if (ntrans)
+			atomicAdd(dst + 0, (unsigned long long)ntrans);
+		if (nfin)
+			atomicAdd(dst + 1, (unsigned long long)nfin);
+		if (ntrunc)
+			atomicAdd(dst + 2, (unsigned long long)ntrunc);
+	}
 }
 
 } // namespace
+
+#if !MBG_PARAM
 
 extern "C" __global__ void __launch_bounds__(256) simulate(mbg_sim_args a)
 {
@@ -102,6 +133,10 @@ extern "C" __global__ void __launch_bounds__(256) simulate(mbg_sim_args a)
 	const lane_hist hist { lds, a };
 	const int lane = __lane_id();
 	const uint32_t B0 = ((uint32_t)a.n_nodes + 3u) >> 2;
+	auto fw = [&](int w, uint64_t o, uint64_t q) { hist.win(w, o, q); };
+	auto fe = [&](int w, uint64_t q) { hist.ent(w, q); };
+	auto ft = [&](int w, uint64_t o) { hist.tail(w, o); };
+	const bool use_tails = a.tail_table != nullptr;
 
 	uint32_t s[W];
 #pragma unroll
@@ -111,15 +146,8 @@ extern "C" __global__ void __launch_bounds__(256) simulate(mbg_sim_args a)
 	uint32_t k = 0;
 	uint64_t id = 0;
 	bool active = false, exhausted = false;
-
-	// register-combined window / entropy contributions
-	int pw = -1, pew = -1;
-	uint64_t pobs = 0, pq = 0, peq = 0;
+	pending pend;
 	uint64_t ntrans = 0, nfinished = 0, ntrunc = 0;
-
-#if MBG_PARAM
-	const mbg_param_view* pv = a.pv;
-#endif
 
 	for (;;)
 	{
@@ -138,28 +166,7 @@ extern "C" __global__ void __launch_bounds__(256) simulate(mbg_sim_args a)
 				if (mine < a.n_traj)
 				{
 					id = a.id_base + mine;
-					// Initial state (src/jit_kernels/simulation.cu:34-71):
-					// bit i = (u_i <= istate_i), u_i = draw i of the stream.
-#pragma unroll
-					for (int w = 0; w < W; ++w)
-					{
-						uint32_t word = 0u;
-						for (int b = 0; b < 32; b += 4)
-						{
-							const int i = w * 32 + b;
-							if (i >= a.n_nodes)
-								break;
-							const mbg_u32x4 r = mbg_philox4x32_10(
-								mbg_u32x4 { (uint32_t)(i >> 2), (uint32_t)id, (uint32_t)(id >> 32), a.variant },
-								a.seed_lo, a.seed_hi);
-							const uint32_t rw[4] = { r.x, r.y, r.z, r.w };
-#pragma unroll
-							for (int j = 0; j < 4; ++j)
-								if (i + j < a.n_nodes && mbg_u01(rw[j]) <= a.istate[i + j])
-									word |= 1u << (b + j);
-						}
-						s[w] = word;
-					}
+					init_state<W>(s, id, a.variant, a.seed_lo, a.seed_hi, a.n_nodes, a.istate);
 					t = 0.0f;
 					k = 0;
 					active = true;
@@ -174,125 +181,16 @@ extern "C" __global__ void __launch_bounds__(256) simulate(mbg_sim_args a)
 		bool ended = false, fixed = false;
 		if (active)
 		{
-			// Uniforms of step k (see mbg_numeric.h for the stream layout).
-			float u1 = 1.0f, u2;
-			if (a.discrete)
-			{
-				const mbg_u32x4 r = mbg_philox4x32_10(
-					mbg_u32x4 { B0 + (k >> 2), (uint32_t)id, (uint32_t)(id >> 32), a.variant }, a.seed_lo, a.seed_hi);
-				u2 = mbg_u01(pick(r, k & 3u));
-			}
-			else
-			{
-				const mbg_u32x4 r = mbg_philox4x32_10(
-					mbg_u32x4 { B0 + (k >> 1), (uint32_t)id, (uint32_t)(id >> 32), a.variant }, a.seed_lo, a.seed_hi);
-				u1 = mbg_u01((k & 1u) ? r.z : r.x);
-				u2 = mbg_u01((k & 1u) ? r.w : r.y);
-			}
-
+			float u1, u2;
+			step_uniforms(a.discrete, B0, k, id, a.variant, a.seed_lo, a.seed_hi, u1, u2);
 			mbg_eval_t e;
-#if MBG_PARAM
-			mbg_eval_p(s, u2, pv, &e);
-#else
 			mbg_eval(s, u2, &e);
-#endif
 			const float tp = t;
-			float H = 0.0f;
-			if (e.total == 0.0f)
-			{
-				fixed = true;
-				t = a.max_time;
-			}
-			else
-			{
-				if (a.discrete)
-				{
-					t = t + a.tick;
-					t = (a.max_time - a.tick < t) ? a.max_time : t;
-				}
-				else
-					t = t + (-mbg_logf(u1)) / e.total;
-				t = fminf(t, a.max_time);
-				H = e.H;
-			}
+			t = advance_time(a.discrete, t, e.total, u1, a.tick, a.max_time, fixed);
+			const float H = fixed ? 0.0f : e.H;
 			ntrans++;
-
-			// The record (state during [tp, t), H) goes into the windows.
-			const uint64_t obs = mbg_obs_index(s);
-			if (a.discrete)
-			{
-				const int w = (int)roundf(tp / a.tick);
-				if (w < a.windows)
-				{
-					if (w == pw && obs == pobs)
-						pq += 1ull;
-					else
-					{
-						if (pw >= 0)
-							hist.win(pw, pobs, pq);
-						pw = w;
-						pobs = obs;
-						pq = 1ull;
-					}
-					const uint64_t qe = (uint64_t)rintf(H * a.qscale);
-					if (qe)
-					{
-						if (w == pew)
-							peq += qe;
-						else
-						{
-							if (pew >= 0)
-								hist.ent(pew, peq);
-							pew = w;
-							peq = qe;
-						}
-					}
-				}
-			}
-			else
-			{
-				int w = (int)floorf(tp / a.tick);
-				float b = tp;
-				while (t > b)
-				{
-					const float wend = (float)(w + 1) * a.tick;
-					const float en = fminf(t, wend);
-					const float sl = en - b;
-					if (w < a.windows)
-					{
-						const uint64_t q = (uint64_t)rintf(sl * a.qscale);
-						if (q)
-						{
-							if (w == pw && obs == pobs)
-								pq += q;
-							else
-							{
-								if (pw >= 0)
-									hist.win(pw, pobs, pq);
-								pw = w;
-								pobs = obs;
-								pq = q;
-							}
-						}
-						const uint64_t qe = (uint64_t)rintf((H * sl) * a.qscale);
-						if (qe)
-						{
-							if (w == pew)
-								peq += qe;
-							else
-							{
-								if (pew >= 0)
-									hist.ent(pew, peq);
-								pew = w;
-								peq = qe;
-							}
-						}
-					}
-					w++;
-					b = en;
-				}
-			}
-
+			record(a.discrete, tp, t, mbg_obs_index(s), H, a.tick, a.qscale, a.windows, pend, fw, fe, ft,
+				   use_tails && t >= a.max_time);
 			k++;
 			if (t >= a.max_time)
 			{
@@ -313,8 +211,7 @@ extern "C" __global__ void __launch_bounds__(256) simulate(mbg_sim_args a)
 		}
 
 		// Trajectory end: final state and fixed points (uniform region).
-		const uint64_t endm = __ballot(ended);
-		if (endm)
+		if (__ballot(ended))
 		{
 			if (ended && t >= a.max_time)
 				hist.fin(mbg_obs_index(s));
@@ -323,7 +220,7 @@ extern "C" __global__ void __launch_bounds__(256) simulate(mbg_sim_args a)
 			{
 				const int l = __ffsll((unsigned long long)fm) - 1;
 				if (lane == l)
-					fp_insert(a, s);
+					fp_insert<W>(a.fp_table, a.fp_mask, s, a.counters);
 				fm &= fm - 1ull;
 			}
 			if (ended)
@@ -331,25 +228,10 @@ extern "C" __global__ void __launch_bounds__(256) simulate(mbg_sim_args a)
 		}
 	}
 
-	if (pw >= 0)
-		hist.win(pw, pobs, pq);
-	if (pew >= 0)
-		hist.ent(pew, peq);
-
-	// Wave totals of the counters, one atomic per wave.
-	for (int off = 32; off > 0; off >>= 1)
-	{
-		ntrans += __shfl_xor(ntrans, off);
-		nfinished += __shfl_xor(nfinished, off);
-		ntrunc += __shfl_xor(ntrunc, off);
-	}
-	if (lane == 0)
-	{
-		atomicAdd(a.counters + MBG_CTR_TRANSITIONS, (unsigned long long)ntrans);
-		atomicAdd(a.counters + MBG_CTR_FINISHED, (unsigned long long)nfinished);
-		if (ntrunc)
-			atomicAdd(a.counters + MBG_CTR_TRUNCATED, (unsigned long long)ntrunc);
-	}
+	pend.flush_all(fw, fe);
+	wave_add_counters(a.counters + MBG_CTR_TRANSITIONS, ntrans, nfinished, 0);
+	if (ntrunc)
+		atomicAdd(a.counters + MBG_CTR_TRUNCATED, (unsigned long long)ntrunc);
 
 	// Flush the block's LDS histograms.
 	__syncthreads();
@@ -377,4 +259,142 @@ extern "C" __global__ void __launch_bounds__(256) simulate(mbg_sim_args a)
 				atomicAdd(a.fin_vals + i, v);
 		}
 	}
+}
+
+#else // MBG_PARAM
+
+// Batched variants.  Statistics of variant v live at its offsets in the
+// dense window / entropy / final arrays; fixed points share one table keyed
+// by (state, variant); the stream of variant v is the plain run's stream
+// with variant tag v, so variant v reproduces a plain run bit for bit.
+extern "C" __global__ void __launch_bounds__(256) simulate_batched(mbg_sim_args a)
+{
+	const int lane = __lane_id();
+	const uint32_t B0 = ((uint32_t)a.n_nodes + 3u) >> 2;
+	uint32_t s[W];
+#pragma unroll
+	for (int w = 0; w < W; ++w)
+		s[w] = 0u;
+
+	for (;;)
+	{
+		uint32_t c = 0;
+		if (lane == 0)
+			c = atomicAdd((unsigned int*)(a.counters + MBG_CTR_QUEUE), 1u);
+		c = __builtin_amdgcn_readfirstlane(c);
+		if (c >= a.n_chunks)
+			break;
+		const mbg_batch_chunk ch = a.chunks[c];
+		const uint32_t v = __builtin_amdgcn_readfirstlane(ch.variant);
+		const mbg_batch_variant& vd = a.bv[v];
+		const float max_time = vd.max_time;
+		const int windows = vd.windows;
+		const unsigned long long win_off = vd.win_off, fin_off = vd.fin_off;
+		const unsigned ent_off = vd.ent_off;
+		const float* istate = vd.istate;
+		const mbg_param_view pv = vd.pv;
+
+		auto fw = [&](int w, uint64_t o, uint64_t q) {
+			atomicAdd(a.win_vals + win_off + (((uint64_t)w << a.n_obs) | o), (unsigned long long)q);
+		};
+		auto fe = [&](int w, uint64_t q) { atomicAdd(a.win_ent + ent_off + w, (unsigned long long)q); };
+
+		uint64_t next = ch.begin;
+		const uint64_t end = ch.end;
+		float t = 0.0f;
+		uint32_t k = 0;
+		uint64_t id = 0;
+		bool active = false;
+		pending pend;
+		uint64_t ntrans = 0, nfinished = 0, ntrunc = 0;
+
+		for (;;)
+		{
+			const uint64_t need = __ballot(!active);
+			if (need && next < end)
+			{
+				const uint64_t mine = next + __popcll(need & ((1ull << lane) - 1ull));
+				if (!active && mine < end)
+				{
+					id = mine;
+					init_state<W>(s, id, v, a.seed_lo, a.seed_hi, a.n_nodes, istate);
+					t = 0.0f;
+					k = 0;
+					active = true;
+				}
+				next = min(end, next + (uint64_t)__popcll(need));
+			}
+			if (__ballot(active) == 0)
+				break;
+
+			bool ended = false, fixed = false;
+			if (active)
+			{
+				float u1, u2;
+				step_uniforms(a.discrete, B0, k, id, v, a.seed_lo, a.seed_hi, u1, u2);
+				mbg_eval_t e;
+				mbg_eval_p(s, u2, &pv, &e);
+				const float tp = t;
+				t = advance_time(a.discrete, t, e.total, u1, a.tick, max_time, fixed);
+				const float H = fixed ? 0.0f : e.H;
+				ntrans++;
+				record(a.discrete, tp, t, mbg_obs_index(s), H, a.tick, a.qscale, windows, pend, fw, fe,
+					   [](int, uint64_t) {}, false);
+				k++;
+				if (t >= max_time)
+				{
+					ended = true;
+					nfinished++;
+				}
+				else if (k >= a.max_steps)
+				{
+					ended = true;
+					ntrunc++;
+				}
+				else
+				{
+#pragma unroll
+					for (int w = 0; w < W; ++w)
+						s[w] ^= ((e.flip >> 5) == w) ? (1u << (e.flip & 31)) : 0u;
+				}
+			}
+
+			if (__ballot(ended))
+			{
+				if (ended && t >= max_time)
+					atomicAdd(a.fin_vals + fin_off + mbg_obs_index(s), 1ull);
+				uint64_t fm = __ballot(ended && fixed);
+				while (fm)
+				{
+					const int l = __ffsll((unsigned long long)fm) - 1;
+					if (lane == l)
+					{
+						uint32_t key[W + 1];
+#pragma unroll
+						for (int w = 0; w < W; ++w)
+							key[w] = s[w];
+						key[W] = v;
+						fp_insert<W + 1>(a.fp_table, a.fp_mask, key, a.counters);
+					}
+					fm &= fm - 1ull;
+				}
+				if (ended)
+					active = false;
+			}
+		}
+		pend.flush_all(fw, fe);
+		wave_add_counters(a.vctr + vd.ctr_off, ntrans, nfinished, ntrunc);
+	}
+}
+
+#endif
+
+// Fills n 64-bit words with a value; used instead of memset nodes inside
+// captured graphs.
+extern "C" __global__ void __launch_bounds__(256) mbg_fill_u64(unsigned long long* p, unsigned long long n,
+															   unsigned long long value)
+{
+	for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+		 i += (unsigned long long)gridDim.x * blockDim.x)
+		p[i] = value;
 }

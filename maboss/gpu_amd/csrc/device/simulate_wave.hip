@@ -25,6 +25,7 @@
 
 #include "mbg_kernel_args.h"
 #include "mbg_stats_device.h"
+#include "mbg_lane_core.h"
 
 using namespace mbg_dev;
 
@@ -102,8 +103,20 @@ __device__ void wave_main(const mbg_sim_args& a)
 	const lane_hist hist { lds, a };
 	const uint32_t B0 = ((uint32_t)N + 3u) >> 2;
 
-	int pw = -1, pew = -1;
-	uint64_t pobs = 0, pq = 0, peq = 0;
+	pending pend;
+	auto fw = [&](int w, uint64_t o, uint64_t q) {
+		if (lane == 0)
+			hist.win(w, o, q);
+	};
+	auto fe = [&](int w, uint64_t q) {
+		if (lane == 0)
+			hist.ent(w, q);
+	};
+	auto ft = [&](int w, uint64_t o) {
+		if (lane == 0)
+			hist.tail(w, o);
+	};
+	const bool use_tails = a.tail_table != nullptr;
 	uint64_t ntrans = 0, nfinished = 0, ntrunc = 0;
 
 	for (;;)
@@ -228,80 +241,10 @@ __device__ void wave_main(const mbg_sim_args& a)
 			}
 			ntrans++;
 
-			// record (state during [tp, t), H) into the windows; lane 0 flushes
-			if (a.discrete)
-			{
-				const int w = (int)roundf(tp / a.tick);
-				if (w < a.windows)
-				{
-					if (w == pw && obs == pobs)
-						pq += 1ull;
-					else
-					{
-						if (pw >= 0 && lane == 0)
-							hist.win(pw, pobs, pq);
-						pw = w;
-						pobs = obs;
-						pq = 1ull;
-					}
-					const uint64_t qe = (uint64_t)rintf(H * a.qscale);
-					if (qe)
-					{
-						if (w == pew)
-							peq += qe;
-						else
-						{
-							if (pew >= 0 && lane == 0)
-								hist.ent(pew, peq);
-							pew = w;
-							peq = qe;
-						}
-					}
-				}
-			}
-			else
-			{
-				int w = (int)floorf(tp / a.tick);
-				float b = tp;
-				while (t > b)
-				{
-					const float wend = (float)(w + 1) * a.tick;
-					const float en = fminf(t, wend);
-					const float sl = en - b;
-					if (w < a.windows)
-					{
-						const uint64_t q = (uint64_t)rintf(sl * a.qscale);
-						if (q)
-						{
-							if (w == pw && obs == pobs)
-								pq += q;
-							else
-							{
-								if (pw >= 0 && lane == 0)
-									hist.win(pw, pobs, pq);
-								pw = w;
-								pobs = obs;
-								pq = q;
-							}
-						}
-						const uint64_t qe = (uint64_t)rintf((H * sl) * a.qscale);
-						if (qe)
-						{
-							if (w == pew)
-								peq += qe;
-							else
-							{
-								if (pew >= 0 && lane == 0)
-									hist.ent(pew, peq);
-								pew = w;
-								peq = qe;
-							}
-						}
-					}
-					w++;
-					b = en;
-				}
-			}
+			// record (state during [tp, t), H) into the windows; the values
+			// are wave-uniform and lane 0 performs the flushes
+			record(a.discrete, tp, t, obs, H, a.tick, a.qscale, a.windows, pend, fw, fe, ft,
+				   use_tails && t >= a.max_time);
 
 			k++;
 			if (t >= a.max_time)
@@ -436,12 +379,9 @@ __device__ void wave_main(const mbg_sim_args& a)
 		}
 	}
 
+	pend.flush_all(fw, fe);
 	if (lane == 0)
 	{
-		if (pw >= 0)
-			hist.win(pw, pobs, pq);
-		if (pew >= 0)
-			hist.ent(pew, peq);
 		atomicAdd(a.counters + MBG_CTR_TRANSITIONS, (unsigned long long)ntrans);
 		atomicAdd(a.counters + MBG_CTR_FINISHED, (unsigned long long)nfinished);
 		if (ntrunc)

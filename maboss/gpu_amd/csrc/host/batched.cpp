@@ -1,0 +1,431 @@
+// Batched form: many variants of one model -- different rate vectors, mutant
+// nodes forced on or off, trajectory counts and horizons -- simulated in one
+// launch of the parametric lane kernel (simulate_batched).
+//
+// The reference has no counterpart: rates are baked into its generated code
+// (src/generator.cpp:104-106, src/parser/expressions.cpp:211-214), one model
+// runs per process (src/main.cpp:154-164) and mutants are a TODO
+// (README.md:33).  Variant v draws from the stream tagged v, so its result is
+// bit-identical to a plain run with variant = v and the same rates.
+//
+// The launch (statistics cleared by fill kernels, then the batched kernel) is
+// captured in a hipGraph and replayed while the batch layout is unchanged.
+// The launch shape (block size, blocks per CU, trajectories per lane of a
+// work chunk) is chosen by timing the candidates on the batch itself, once
+// per batch layout.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "../device/mbg_kernel_args.h"
+#include "../device/mbg_numeric.h"
+#include "engine.h"
+#include "timer.h"
+
+namespace mbg
+{
+
+namespace
+{
+
+uint64_t next_pow2(uint64_t v)
+{
+	uint64_t p = 1;
+	while (p < v)
+		p <<= 1;
+	return p;
+}
+
+uint64_t mix(uint64_t h, uint64_t v)
+{
+	h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+	return h;
+}
+
+struct batch_layout
+{
+	int nv = 0;
+	std::vector<int> windows;
+	std::vector<float> max_time;
+	std::vector<uint64_t> n, win_off, fin_off;
+	std::vector<unsigned> ent_off;
+	uint64_t win_cells = 0, ent_cells = 0, fin_cells = 0, fp_cap = 0;
+	size_t fp_stride = 0;
+};
+
+void fill(program& p, hipStream_t stream, void* ptr, uint64_t words, uint64_t value)
+{
+	if (!words)
+		return;
+	void* args[] = { &ptr, &words, &value };
+	const unsigned grid = (unsigned)std::min<uint64_t>(2048, (words + 255) / 256);
+	hip_check(hipModuleLaunchKernel(p.fn_fill, grid, 1, 1, 256, 1, 1, 0, stream, args, nullptr),
+			  "hipModuleLaunchKernel(mbg_fill_u64)");
+}
+
+} // namespace
+
+std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_params& bp,
+													  const std::vector<variant_spec>& vs, hipStream_t stream,
+													  batch_timing* timing)
+{
+	scope_timer tm("batched> simulate");
+	const double t0 = wall_seconds();
+	const model& m = *p.mdl;
+	const int W = m.n_words(), N = m.n_nodes(), n_obs = m.n_observed();
+	if (W > max_lane_words)
+		throw error(MBG_E_LIMIT, "the batched form serves models of up to 256 nodes");
+	if (n_obs > 20)
+		throw error(MBG_E_LIMIT, "the batched form keeps dense per-variant windows: at most 20 observed nodes");
+	std::vector<std::unique_ptr<result>> out;
+	if (vs.empty())
+		return out;
+	hipFunction_t fn = p.function(true);
+	const int np = (int)p.gen.param_names.size();
+	const float tick = m.time_tick();
+	const uint64_t seed = bp.seed < 0 ? m.seed() : (uint64_t)bp.seed;
+
+	// ---- layout of the per-variant statistics
+	batch_layout L;
+	L.nv = (int)vs.size();
+	uint64_t total_traj = 0;
+	for (int v = 0; v < L.nv; v++)
+	{
+		const float mt = vs[v].max_time > 0 ? vs[v].max_time : m.max_time();
+		const int win = (int)std::ceil(mt / tick);
+		if (vs[v].n_traj < 0 || win <= 0 || win > (1 << 20))
+			throw error(MBG_E_ARG, "variant " + std::to_string(v) + ": bad trajectory count or horizon");
+		if (!vs[v].rates.empty() && (int)vs[v].rates.size() != np)
+			throw error(MBG_E_ARG, "variant " + std::to_string(v) + ": expected " + std::to_string(np) + " rates");
+		if (!vs[v].mut.empty() && (int)vs[v].mut.size() != N)
+			throw error(MBG_E_ARG, "variant " + std::to_string(v) + ": expected " + std::to_string(N) + " mutations");
+		L.max_time.push_back(mt);
+		L.windows.push_back(win);
+		L.n.push_back((uint64_t)vs[v].n_traj);
+		L.win_off.push_back(L.win_cells);
+		L.ent_off.push_back((unsigned)L.ent_cells);
+		L.fin_off.push_back(L.fin_cells);
+		L.win_cells += (uint64_t)win << n_obs;
+		L.ent_cells += (uint64_t)win;
+		L.fin_cells += 1ull << n_obs;
+		total_traj += (uint64_t)vs[v].n_traj;
+	}
+	L.fp_stride = 3 + (size_t)W;
+	L.fp_cap = next_pow2(std::min<uint64_t>(std::max<uint64_t>(2 * total_traj, 4096), 1ull << 24));
+
+	// ---- per-variant tables: descriptors, then {rate, rate_lr, up_en, dn_en, istate} blocks
+	std::vector<float> base_rates(np);
+	for (int k = 0; k < np; k++)
+		base_rates[k] = m.variables.at(p.gen.param_names[k]);
+	const size_t block_bytes = (((size_t)2 * np + N + 2 * W) * 4 + 15) & ~(size_t)15;
+	const size_t desc_bytes = ((size_t)L.nv * sizeof(mbg_batch_variant) + 15) & ~(size_t)15;
+	std::vector<char> tab(desc_bytes + (size_t)L.nv * block_bytes, 0);
+	char* dbase = (char*)p.b_bvar.ensure(tab.size());
+	for (int v = 0; v < L.nv; v++)
+	{
+		const size_t off = desc_bytes + (size_t)v * block_bytes;
+		float* rate = (float*)(tab.data() + off);
+		float* rate_lr = rate + np;
+		uint32_t* up_en = (uint32_t*)(rate_lr + np);
+		uint32_t* dn_en = up_en + W;
+		float* istate = (float*)(dn_en + W);
+		const std::vector<float>& r = vs[v].rates.empty() ? base_rates : vs[v].rates;
+		for (int k = 0; k < np; k++)
+		{
+			rate[k] = r[k];
+			rate_lr[k] = rate_log2_rate(r[k]);
+		}
+		for (int w = 0; w < W; w++)
+			up_en[w] = dn_en[w] = 0xffffffffu;
+		for (int i = 0; i < N; i++)
+		{
+			istate[i] = m.nodes[i].istate;
+			const int8_t mu = vs[v].mut.empty() ? 0 : vs[v].mut[i];
+			if (mu > 0) // forced on: never switches off, starts on
+			{
+				dn_en[i / 32] &= ~(1u << (i % 32));
+				istate[i] = 1.0f;
+			}
+			else if (mu < 0) // forced off
+			{
+				up_en[i / 32] &= ~(1u << (i % 32));
+				istate[i] = 0.0f;
+			}
+		}
+		mbg_batch_variant d;
+		std::memset(&d, 0, sizeof(d));
+		d.n_traj = L.n[v];
+		d.win_off = L.win_off[v];
+		d.fin_off = L.fin_off[v];
+		d.ent_off = L.ent_off[v];
+		d.windows = L.windows[v];
+		d.max_time = L.max_time[v];
+		d.ctr_off = 3u * (unsigned)v;
+		d.istate = (const float*)(dbase + ((char*)istate - tab.data()));
+		d.pv.rate = (const float*)(dbase + ((char*)rate - tab.data()));
+		d.pv.rate_lr = (const float*)(dbase + ((char*)rate_lr - tab.data()));
+		d.pv.up_en = (const uint32_t*)(dbase + ((char*)up_en - tab.data()));
+		d.pv.dn_en = (const uint32_t*)(dbase + ((char*)dn_en - tab.data()));
+		std::memcpy(tab.data() + (size_t)v * sizeof(mbg_batch_variant), &d, sizeof(d));
+	}
+	hip_check(hipMemcpyAsync(dbase, tab.data(), tab.size(), hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
+
+	// ---- storage
+	auto* win = (unsigned long long*)p.b_bwin.ensure(std::max<uint64_t>(L.win_cells, 1) * 8);
+	auto* ent = (unsigned long long*)p.b_bent.ensure(std::max<uint64_t>(L.ent_cells, 1) * 8);
+	auto* fin = (unsigned long long*)p.b_bfin.ensure(std::max<uint64_t>(L.fin_cells, 1) * 8);
+	const uint64_t fp_words = (L.fp_cap * L.fp_stride * 4 + 7) / 8;
+	auto* fp = (unsigned int*)p.b_bfp.ensure(fp_words * 8);
+	auto* vctr = (unsigned long long*)p.b_bctr.ensure((size_t)L.nv * 3 * 8);
+	auto* counters = (unsigned long long*)p.b_counters.ensure(MBG_NCOUNTERS * 8);
+
+	// ---- launch shape
+	uint64_t tune_key = mix(mix(mix(0, (uint64_t)L.nv), total_traj), (uint64_t)p.num_cus);
+	for (int v = 0; v < L.nv; v++)
+		tune_key = mix(tune_key, ((uint64_t)L.windows[v] << 32) ^ L.n[v]);
+
+	std::vector<mbg_batch_chunk> chunks;
+	auto make_chunks = [&](int tpl) {
+		chunks.clear();
+		const uint64_t cs = 64ull * (uint64_t)tpl;
+		std::vector<int> order(L.nv);
+		std::iota(order.begin(), order.end(), 0);
+		// longest horizons first, so the tail of the launch is short work
+		std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return L.max_time[x] > L.max_time[y]; });
+		for (int v : order)
+			for (uint64_t b = 0; b < L.n[v]; b += cs)
+				chunks.push_back(mbg_batch_chunk { (unsigned)v, 0u, b, std::min(L.n[v], b + cs) });
+	};
+
+	mbg_sim_args a;
+	std::memset(&a, 0, sizeof(a));
+	a.seed_lo = (uint32_t)seed;
+	a.seed_hi = (uint32_t)(seed >> 32);
+	a.max_steps = bp.max_steps ? bp.max_steps : (1u << 30);
+	a.n_nodes = N;
+	a.n_obs = n_obs;
+	a.discrete = m.discrete_time();
+	a.tick = tick;
+	a.qscale = a.discrete ? 0x1p32f : 0x1p32f / tick;
+	a.win_vals = win;
+	a.win_ent = ent;
+	a.fin_vals = fin;
+	a.fp_table = fp;
+	a.fp_mask = L.fp_cap - 1;
+	a.counters = counters;
+	a.bv = (const mbg_batch_variant*)dbase;
+	a.vctr = vctr;
+
+	auto enqueue = [&](int block, int per_cu, int tpl) {
+		make_chunks(tpl);
+		auto* dch = (mbg_batch_chunk*)p.b_bchunk.ensure(std::max<size_t>(chunks.size(), 1) * sizeof(mbg_batch_chunk));
+		hip_check(hipMemcpyAsync(dch, chunks.data(), chunks.size() * sizeof(mbg_batch_chunk), hipMemcpyHostToDevice,
+								 stream),
+				  "hipMemcpyAsync");
+		a.chunks = dch;
+		a.n_chunks = (unsigned)chunks.size();
+		const uint64_t waves = (uint64_t)p.num_cus * per_cu * (block / 64);
+		const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)p.num_cus * per_cu,
+																	   (chunks.size() + block / 64 - 1) / (block / 64)));
+		(void)waves;
+		return grid;
+	};
+	auto launch = [&](int block, uint64_t grid) {
+		fill(p, stream, counters, MBG_NCOUNTERS, 0);
+		fill(p, stream, win, L.win_cells, 0);
+		fill(p, stream, ent, L.ent_cells, 0);
+		fill(p, stream, fin, L.fin_cells, 0);
+		fill(p, stream, fp, fp_words, 0);
+		fill(p, stream, vctr, (uint64_t)L.nv * 3, 0);
+		if (a.n_chunks)
+		{
+			void* args[] = { &a };
+			hip_check(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, block, 1, 1, 0, stream, args, nullptr),
+					  "hipModuleLaunchKernel(simulate_batched)");
+		}
+	};
+	auto occupancy = [&](int block) {
+		int per = 0;
+		hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, block, 0),
+				  "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+		return std::max(per, 1);
+	};
+	auto timed = [&](auto&& body) {
+		hipEvent_t e0, e1;
+		hip_check(hipEventCreate(&e0), "hipEventCreate");
+		hip_check(hipEventCreate(&e1), "hipEventCreate");
+		hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+		body();
+		hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+		hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+		float ms = 0;
+		hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+		return (double)ms;
+	};
+
+	int block = bp.block, per_cu = bp.per_cu, tpl = bp.tpl;
+	if (!block || !tpl)
+	{
+		if (p.batch_tuned.key != tune_key)
+		{
+			// autotuner: time every candidate on this batch, keep the fastest
+			double best = 1e300;
+			for (int b : { 64, 128, 256 })
+				for (int t : { 1, 2, 4 })
+				{
+					const int pc = occupancy(b);
+					const uint64_t g = enqueue(b, pc, t);
+					const double ms = timed([&] { launch(b, g); });
+					if (ms < best)
+					{
+						best = ms;
+						p.batch_tuned = program::tuned { tune_key, b, pc, t };
+					}
+				}
+		}
+		block = block ? block : p.batch_tuned.block;
+		tpl = tpl ? tpl : p.batch_tuned.tpl;
+		per_cu = per_cu ? per_cu : p.batch_tuned.per_cu;
+	}
+	if (block <= 0 || block > 256 || block % 64)
+		throw error(MBG_E_ARG, "batched block size must be 64, 128, 192 or 256");
+	if (!per_cu)
+		per_cu = occupancy(block);
+	const uint64_t grid = enqueue(block, per_cu, tpl);
+
+	double ms = 0;
+	bool used_graph = false;
+	if (bp.graph)
+	{
+		uint64_t gkey = mix(mix(mix(tune_key, (uint64_t)block), (uint64_t)per_cu), (uint64_t)tpl);
+		gkey = mix(mix(mix(gkey, (uint64_t)(uintptr_t)win), (uint64_t)(uintptr_t)fp), (uint64_t)(uintptr_t)dbase);
+		gkey = mix(mix(gkey, (uint64_t)(uintptr_t)a.chunks), seed);
+		gkey = mix(gkey, a.max_steps);
+		if (!p.batch_graph || p.batch_graph_key != gkey)
+		{
+			if (p.batch_graph)
+			{
+				(void)hipGraphExecDestroy(p.batch_graph);
+				p.batch_graph = nullptr;
+			}
+			hipStream_t cs;
+			hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+			hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+			hipGraph_t g;
+			hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+			hipStream_t saved = stream;
+			stream = cs;
+			launch(block, grid);
+			stream = saved;
+			hip_check(hipStreamEndCapture(cs, &g), "hipStreamEndCapture");
+			hip_check(hipGraphInstantiate(&p.batch_graph, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+			(void)hipGraphDestroy(g);
+			(void)hipStreamDestroy(cs);
+			p.batch_graph_key = gkey;
+		}
+		ms = timed([&] { hip_check(hipGraphLaunch(p.batch_graph, stream), "hipGraphLaunch"); });
+		used_graph = true;
+	}
+	else
+		ms = timed([&] { launch(block, grid); });
+
+	std::vector<unsigned long long> ctr(MBG_NCOUNTERS), vc((size_t)L.nv * 3);
+	hip_check(hipMemcpy(ctr.data(), counters, MBG_NCOUNTERS * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+	hip_check(hipMemcpy(vc.data(), vctr, vc.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+	if (ctr[MBG_CTR_FP_TIMEOUT])
+		throw error(MBG_E_INTERNAL, "fixed-point table: a slot was never published");
+	if (ctr[MBG_CTR_FP_OVERFLOW])
+		throw error(MBG_E_OVERFLOW, "fixed-point table overflow in the batched run");
+
+	uint64_t all_trans = 0;
+	for (int v = 0; v < L.nv; v++)
+		all_trans += vc[3 * v];
+	if (timing)
+	{
+		timing->kernel_ms = ms;
+		timing->transitions = all_trans;
+		timing->block = block;
+		timing->per_cu = per_cu;
+		timing->tpl = tpl;
+		timing->graph = used_graph;
+	}
+
+	// ---- gather per variant
+	std::vector<unsigned long long> hwin, hent, hfin;
+	std::vector<unsigned int> hfp;
+	if (!bp.device_only)
+	{
+		hwin.resize(L.win_cells);
+		hent.resize(L.ent_cells);
+		hfin.resize(L.fin_cells);
+		hfp.resize(L.fp_cap * L.fp_stride);
+		hip_check(hipMemcpy(hwin.data(), win, L.win_cells * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+		hip_check(hipMemcpy(hent.data(), ent, L.ent_cells * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+		hip_check(hipMemcpy(hfin.data(), fin, L.fin_cells * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+		hip_check(hipMemcpy(hfp.data(), fp, hfp.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+	}
+	std::vector<std::vector<std::pair<std::vector<uint32_t>, uint64_t>>> fixed(L.nv);
+	for (uint64_t i = 0; i < (bp.device_only ? 0 : L.fp_cap); i++)
+	{
+		const unsigned int* e = &hfp[i * L.fp_stride];
+		if (e[0] == 2u && e[1] && e[2 + W] < (unsigned)L.nv)
+			fixed[e[2 + W]].emplace_back(std::vector<uint32_t>(e + 2, e + 2 + W), e[1]);
+	}
+	const uint64_t per = 1ull << n_obs;
+	for (int v = 0; v < L.nv; v++)
+	{
+		auto r = std::make_unique<result>();
+		r->windows = L.windows[v];
+		r->n_obs = n_obs;
+		r->n_words = W;
+		r->discrete = m.discrete_time();
+		r->tick = tick;
+		r->max_time = L.max_time[v];
+		r->n_traj = L.n[v];
+		r->norm = (double)L.n[v];
+		for (auto& nd : m.nodes)
+			r->names.push_back(nd.name);
+		r->observed = m.observed_nodes();
+		r->transitions = vc[3 * v];
+		r->finished = vc[3 * v + 1];
+		r->truncated = vc[3 * v + 2];
+		r->kernel_ms = ms;
+		r->window_mode = MBG_HIST_DENSE;
+		r->final_mode = MBG_HIST_DENSE;
+		r->ent.assign(L.windows[v], 0);
+		r->win.assign(L.windows[v], {});
+		if (!bp.device_only)
+		{
+			for (int w = 0; w < L.windows[v]; w++)
+			{
+				r->ent[w] = hent[L.ent_off[v] + w];
+				const unsigned long long* c = &hwin[L.win_off[v] + (uint64_t)w * per];
+				for (uint64_t s = 0; s < per; s++)
+					if (c[s])
+						r->win[w].emplace_back(s, c[s]);
+			}
+			for (uint64_t s = 0; s < per; s++)
+				if (hfin[L.fin_off[v] + s])
+					r->fin.emplace_back(s, hfin[L.fin_off[v] + s]);
+			r->fixed = std::move(fixed[v]);
+			std::sort(r->fixed.begin(), r->fixed.end(), [](const auto& x, const auto& y) {
+				for (int w = (int)x.first.size() - 1; w >= 0; w--)
+					if (x.first[w] != y.first[w])
+						return x.first[w] < y.first[w];
+				return false;
+			});
+		}
+		out.push_back(std::move(r));
+	}
+	const double total = (wall_seconds() - t0) * 1e3;
+	for (auto& r : out)
+		r->total_ms = total;
+	if (timing)
+		timing->total_ms = total;
+	return out;
+}
+
+} // namespace mbg

@@ -300,6 +300,65 @@ int mbg_simulate(mbg_program* p, const mbg_sim_params* params, void* stream, mbg
 	});
 }
 
+void mbg_batch_params_default(mbg_batch_params* p)
+{
+	if (!p)
+		return;
+	std::memset(p, 0, sizeof(*p));
+	p->seed = -1;
+	p->use_graph = 1;
+}
+
+int mbg_simulate_batched(mbg_program* p, const mbg_batch_params* params, int32_t n_variants,
+						 const mbg_variant* variants, void* stream, mbg_result** results, mbg_batch_info* info)
+{
+	return guarded([&] {
+		if (!p || n_variants < 0 || (n_variants && (!variants || !results)))
+			throw mbg::error(mbg::MBG_E_ARG, "bad argument");
+		mbg::batch_params bp;
+		if (params)
+		{
+			bp.seed = params->seed;
+			bp.max_steps = params->max_steps;
+			bp.block = params->block_size;
+			bp.per_cu = params->blocks_per_cu;
+			bp.tpl = params->traj_per_lane;
+			bp.graph = params->use_graph != 0;
+			bp.device_only = (params->flags & MBG_SIM_DEVICE_ONLY) != 0;
+		}
+		const int np = (int)p->p->gen.param_names.size(), N = p->p->mdl->n_nodes();
+		std::vector<mbg::variant_spec> vs(n_variants);
+		for (int v = 0; v < n_variants; v++)
+		{
+			vs[v].n_traj = variants[v].n_trajectories;
+			vs[v].max_time = variants[v].max_time;
+			if (variants[v].rates)
+				vs[v].rates.assign(variants[v].rates, variants[v].rates + np);
+			if (variants[v].mutations)
+				vs[v].mut.assign(variants[v].mutations, variants[v].mutations + N);
+		}
+		mbg::batch_timing bt;
+		auto rs = mbg::simulate_batched(*p->p, bp, vs, (hipStream_t)stream, &bt);
+		for (int v = 0; v < n_variants; v++)
+		{
+			auto rr = std::make_unique<mbg_result>();
+			rr->r = std::move(rs[v]);
+			results[v] = rr.release();
+		}
+		if (info)
+		{
+			std::memset(info, 0, sizeof(*info));
+			info->kernel_ms = bt.kernel_ms;
+			info->total_ms = bt.total_ms;
+			info->transitions = bt.transitions;
+			info->block_size = bt.block;
+			info->blocks_per_cu = bt.per_cu;
+			info->traj_per_lane = bt.tpl;
+			info->used_graph = bt.graph ? 1 : 0;
+		}
+	});
+}
+
 void mbg_result_free(mbg_result* r) { delete r; }
 
 int mbg_result_get_info(const mbg_result* rr, mbg_result_info* out)

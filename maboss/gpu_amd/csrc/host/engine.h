@@ -63,6 +63,25 @@ struct program
 	int device = 0;
 	int num_cus = 0;
 
+	hipFunction_t fn_fill = nullptr; // mbg_fill_u64 of the parametric module
+
+	// utility kernels (slot fill, tail expansion), precompiled
+	hipModule_t mod_util = nullptr;
+	hipFunction_t fn_fill_slots = nullptr, fn_expand_tails = nullptr;
+	void load_util();
+	device_buffer b_tail;
+
+	// batched form: launch shape chosen by the autotuner, keyed by the batch
+	struct tuned
+	{
+		uint64_t key = 0;
+		int block = 0, per_cu = 0, tpl = 0;
+	} batch_tuned;
+	// captured batched launch, replayed while the batch layout is unchanged
+	hipGraphExec_t batch_graph = nullptr;
+	uint64_t batch_graph_key = 0;
+	device_buffer b_bwin, b_bent, b_bfin, b_bfp, b_bctr, b_bvar, b_bchunk, b_btab;
+
 	// wave-per-trajectory kernel (precompiled, table driven)
 	hipModule_t mod_wave = nullptr;
 	std::unique_ptr<wave_tables> wtab;
@@ -128,5 +147,35 @@ struct result
 };
 
 std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t stream);
+
+// ---- batched form
+struct variant_spec
+{
+	int64_t n_traj = 0;
+	float max_time = 0;         // <= 0: the model's
+	std::vector<float> rates;   // [n_params] or empty: the model's values
+	std::vector<int8_t> mut;    // [n_nodes]: +1 forced on, -1 forced off, 0 free; or empty
+};
+
+struct batch_params
+{
+	int64_t seed = -1;
+	uint32_t max_steps = 0;
+	int block = 0, per_cu = 0, tpl = 0; // 0: autotuned
+	bool graph = true;                  // capture fills + launch in a hipGraph, replay it
+	bool device_only = false;
+};
+
+struct batch_timing
+{
+	double kernel_ms = 0, total_ms = 0;
+	uint64_t transitions = 0;
+	int block = 0, per_cu = 0, tpl = 0;
+	bool graph = false;
+};
+
+std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_params& bp,
+													  const std::vector<variant_spec>& vs, hipStream_t stream,
+													  batch_timing* timing);
 
 } // namespace mbg

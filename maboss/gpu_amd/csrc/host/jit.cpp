@@ -24,6 +24,7 @@ extern "C" {
 extern const char mbg_hdr_numeric[];
 extern const char mbg_hdr_abi[];
 extern const unsigned char mbg_co_wave[];
+extern const unsigned char mbg_co_util[];
 }
 
 namespace mbg
@@ -185,6 +186,20 @@ program::~program()
 		(void)hipModuleUnload(mod_param);
 	if (mod_wave)
 		(void)hipModuleUnload(mod_wave);
+	if (batch_graph)
+		(void)hipGraphExecDestroy(batch_graph);
+	if (mod_util)
+		(void)hipModuleUnload(mod_util);
+}
+
+void program::load_util()
+{
+	if (mod_util)
+		return;
+	hip_check(hipModuleLoadData(&mod_util, mbg_co_util), "hipModuleLoadData(mbg_util)");
+	hip_check(hipModuleGetFunction(&fn_fill_slots, mod_util, "mbg_fill_slots"), "hipModuleGetFunction(mbg_fill_slots)");
+	hip_check(hipModuleGetFunction(&fn_expand_tails, mod_util, "mbg_expand_tails"),
+			  "hipModuleGetFunction(mbg_expand_tails)");
 }
 
 hipFunction_t program::wave_function()
@@ -238,7 +253,13 @@ hipFunction_t program::function(bool param)
 		}
 	}
 	hip_check(hipModuleLoadData(&mod, co.data()), "hipModuleLoadData");
-	hip_check(hipModuleGetFunction(&fn, mod, "simulate"), "hipModuleGetFunction(simulate)");
+	if (param)
+	{
+		hip_check(hipModuleGetFunction(&fn, mod, "simulate_batched"), "hipModuleGetFunction(simulate_batched)");
+		hip_check(hipModuleGetFunction(&fn_fill, mod, "mbg_fill_u64"), "hipModuleGetFunction(mbg_fill_u64)");
+	}
+	else
+		hip_check(hipModuleGetFunction(&fn, mod, "simulate"), "hipModuleGetFunction(simulate)");
 	compile_seconds += wall_seconds() - t0;
 	return fn;
 }

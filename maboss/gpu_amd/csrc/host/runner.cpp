@@ -43,12 +43,14 @@ void device_buffer::release()
 namespace
 {
 
-// Empty hash slots: key = all ones, value = 0 (a strided 2D memset writes the
-// 8-byte key half of every 16-byte slot).
-void hip_fill_slots(unsigned long long* table, uint64_t slots, hipStream_t stream)
+// Empty hash slots {key = all ones, value = 0}, written by a fill kernel.
+void hip_fill_slots(program& p, unsigned long long* table, uint64_t slots, hipStream_t stream)
 {
-	hip_check(hipMemsetAsync(table, 0, slots * 16, stream), "hipMemsetAsync");
-	hip_check(hipMemset2DAsync(table, 16, 0xff, 8, slots, stream), "hipMemset2DAsync");
+	p.load_util();
+	void* args[] = { &table, &slots };
+	const unsigned grid = (unsigned)std::min<uint64_t>(4096, (slots + 255) / 256);
+	hip_check(hipModuleLaunchKernel(p.fn_fill_slots, grid, 1, 1, 256, 1, 1, 0, stream, args, nullptr),
+			  "hipModuleLaunchKernel(mbg_fill_slots)");
 }
 
 uint64_t next_pow2(uint64_t v)
@@ -183,6 +185,10 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 	hipFunction_t fn = use_wave ? p.wave_function() : p.function(false);
 	res->kernel = use_wave ? 1 : 0;
 	plan pl = make_plan(m, sp, n_traj, windows, n_obs);
+	// tail records for window storage in HBM (continuous time)
+	const bool tails = !m.discrete_time() && pl.win_mode != MBG_HIST_LDS;
+	uint64_t tail_cap = tails ? next_pow2(std::min<uint64_t>(std::max<uint64_t>(n_traj / 4, 1 << 12), 1 << 20)) : 0;
+	p.load_util();
 
 	for (int attempt = 0;; attempt++)
 	{
@@ -195,7 +201,7 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		if (pl.win_mode == MBG_HIST_HASH)
 		{
 			win_keys = (unsigned long long*)p.b_win_keys.ensure(pl.win_cap * 16);
-			hip_fill_slots(win_keys, pl.win_cap, stream);
+			hip_fill_slots(p, win_keys, pl.win_cap, stream);
 		}
 		else
 		{
@@ -205,7 +211,13 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		if (pl.fin_mode == MBG_HIST_HASH)
 		{
 			fin_keys = (unsigned long long*)p.b_fin_keys.ensure(pl.fin_cap * 16);
-			hip_fill_slots(fin_keys, pl.fin_cap, stream);
+			hip_fill_slots(p, fin_keys, pl.fin_cap, stream);
+		}
+		unsigned long long* tail_tab = nullptr;
+		if (tails)
+		{
+			tail_tab = (unsigned long long*)p.b_tail.ensure(tail_cap * 16);
+			hip_fill_slots(p, tail_tab, tail_cap, stream);
 		}
 		else
 		{
@@ -258,6 +270,8 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		a.fp_mask = pl.fp_cap - 1;
 		a.counters = counters;
 		a.pv = nullptr;
+		a.tail_table = tail_tab;
+		a.tail_mask = tail_cap ? tail_cap - 1 : 0;
 
 		// ---- launch: a persistent grid filling every CU
 		const int block = sp.block_size > 0 ? sp.block_size : 256;
@@ -309,6 +323,12 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 			void* args[] = { &a };
 			hip_check(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, block, 1, 1, lds_total, stream, args, nullptr),
 					  "hipModuleLaunchKernel(simulate)");
+			if (tails)
+			{
+				const unsigned eg = (unsigned)std::min<uint64_t>(1024, (tail_cap + 255) / 256);
+				hip_check(hipModuleLaunchKernel(p.fn_expand_tails, eg, 1, 1, 256, 1, 1, 0, stream, args, nullptr),
+						  "hipModuleLaunchKernel(mbg_expand_tails)");
+			}
 		}
 		hip_check(hipEventRecord(e1, stream), "hipEventRecord");
 
@@ -323,13 +343,15 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		(void)hipEventDestroy(e1);
 
 		const bool win_over = ctr[MBG_CTR_WIN_OVERFLOW] != 0, fin_over = ctr[MBG_CTR_FIN_OVERFLOW] != 0,
-				   fp_over = ctr[MBG_CTR_FP_OVERFLOW] != 0;
+				   fp_over = ctr[MBG_CTR_FP_OVERFLOW] != 0, tail_over = ctr[MBG_CTR_TAIL_OVERFLOW] != 0;
 		if (ctr[MBG_CTR_FP_TIMEOUT])
 			throw error(MBG_E_INTERNAL, "fixed-point table: a slot was never published");
-		if (win_over || fin_over || fp_over)
+		if (win_over || fin_over || fp_over || tail_over)
 		{
 			if (attempt >= 6)
 				throw error(MBG_E_OVERFLOW, "device hash table overflow after resizing");
+			if (tail_over)
+				tail_cap *= 4;
 			if (win_over)
 			{
 				if (sp.window_capacity)

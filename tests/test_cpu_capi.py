@@ -61,3 +61,31 @@ def test_cli_usage():
     assert p.returncode == 1 and "Usage: MaBoSS.GPU [-o prefix] bnd_file cfg_file" in p.stdout
     p = mg.run_cli("/nonexistent.bnd", "/nonexistent.cfg")
     assert p.returncode == 1 and "cannot open" in p.stderr
+
+
+STRUCTS = {
+    "mbg_model_info": _capi.ModelInfo, "mbg_compile_opts": _capi.CompileOpts,
+    "mbg_sim_params": _capi.SimParams, "mbg_result_info": _capi.ResultInfo,
+    "mbg_variant": _capi.Variant, "mbg_batch_params": _capi.BatchParams,
+    "mbg_batch_info": _capi.BatchInfo,
+}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors of the C structs have the header's size and field offsets."""
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "maboss_gpu.h"', "int main(void){"]
+    for cname, py in STRUCTS.items():
+        lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            lines.append(f'printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    for line in out.splitlines():
+        cname, field, val = line.split()
+        py = STRUCTS[cname]
+        want = ctypes.sizeof(py) if field == "sizeof" else getattr(py, field).offset
+        assert int(val) == want, (cname, field)
