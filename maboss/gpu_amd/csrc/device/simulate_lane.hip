@@ -100,7 +100,11 @@ __device__ void fp_insert(unsigned int* table, uint64_t mask, const uint32_t (&k
 	atomicAdd(counters + MBG_CTR_FP_OVERFLOW, 1ull);
 }
 
-__device__ __forceinline__ void wave_add_counters(unsigned long long* dst, uint64_t ntrans, uint64_t nfin, uint64_t ntrunc)
+// Wave-reduced counter update: transitions, finished and truncated
+// trajectories go to their three destinations.
+__device__ __forceinline__ void wave_add_counters(unsigned long long* d_trans, unsigned long long* d_fin,
+												  unsigned long long* d_trunc, uint64_t ntrans, uint64_t nfin,
+												  uint64_t ntrunc)
 {
 	for (int off = 32; off > 0; off >>= 1)
 	{
@@ -111,11 +115,11 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long* dst, uint6
 	if (__lane_id() == 0)
 	{
 		if (ntrans)
-			atomicAdd(dst + 0, (unsigned long long)ntrans);
+			atomicAdd(d_trans, (unsigned long long)ntrans);
 		if (nfin)
-			atomicAdd(dst + 1, (unsigned long long)nfin);
+			atomicAdd(d_fin, (unsigned long long)nfin);
 		if (ntrunc)
-			atomicAdd(dst + 2, (unsigned long long)ntrunc);
+			atomicAdd(d_trunc, (unsigned long long)ntrunc);
 	}
 }
 
@@ -229,9 +233,8 @@ extern "C" __global__ void __launch_bounds__(256) simulate(mbg_sim_args a)
 	}
 
 	pend.flush_all(fw, fe);
-	wave_add_counters(a.counters + MBG_CTR_TRANSITIONS, ntrans, nfinished, 0);
-	if (ntrunc)
-		atomicAdd(a.counters + MBG_CTR_TRUNCATED, (unsigned long long)ntrunc);
+	wave_add_counters(a.counters + MBG_CTR_TRANSITIONS, a.counters + MBG_CTR_FINISHED, a.counters + MBG_CTR_TRUNCATED,
+					  ntrans, nfinished, ntrunc);
 
 	// Flush the block's LDS histograms.
 	__syncthreads();
@@ -383,7 +386,8 @@ extern "C" __global__ void __launch_bounds__(256) simulate_batched(mbg_sim_args 
 			}
 		}
 		pend.flush_all(fw, fe);
-		wave_add_counters(a.vctr + vd.ctr_off, ntrans, nfinished, ntrunc);
+		wave_add_counters(a.vctr + vd.ctr_off, a.vctr + vd.ctr_off + 1, a.vctr + vd.ctr_off + 2, ntrans, nfinished,
+						  ntrunc);
 	}
 }
 

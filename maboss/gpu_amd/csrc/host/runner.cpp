@@ -213,16 +213,16 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 			fin_keys = (unsigned long long*)p.b_fin_keys.ensure(pl.fin_cap * 16);
 			hip_fill_slots(p, fin_keys, pl.fin_cap, stream);
 		}
+		else
+		{
+			fin_vals = (unsigned long long*)p.b_fin_vals.ensure(pl.fin_cells * 8);
+			hip_check(hipMemsetAsync(fin_vals, 0, pl.fin_cells * 8, stream), "hipMemsetAsync");
+		}
 		unsigned long long* tail_tab = nullptr;
 		if (tails)
 		{
 			tail_tab = (unsigned long long*)p.b_tail.ensure(tail_cap * 16);
 			hip_fill_slots(p, tail_tab, tail_cap, stream);
-		}
-		else
-		{
-			fin_vals = (unsigned long long*)p.b_fin_vals.ensure(pl.fin_cells * 8);
-			hip_check(hipMemsetAsync(fin_vals, 0, pl.fin_cells * 8, stream), "hipMemsetAsync");
 		}
 		const size_t fp_stride = 2 + (size_t)W;
 		auto* fp = (unsigned int*)p.b_fp.ensure(pl.fp_cap * fp_stride * 4);
@@ -349,7 +349,9 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		if (win_over || fin_over || fp_over || tail_over)
 		{
 			if (attempt >= 6)
-				throw error(MBG_E_OVERFLOW, "device hash table overflow after resizing");
+				throw error(MBG_E_OVERFLOW, std::string("device hash table overflow after resizing (")
+												+ (win_over ? "window " : "") + (fin_over ? "final " : "")
+												+ (fp_over ? "fixed-point " : "") + (tail_over ? "tail" : "") + ")");
 			if (tail_over)
 				tail_cap *= 4;
 			if (win_over)
