@@ -87,7 +87,9 @@ typedef struct mbg_compile_opts
 {
 	int32_t entropy_formula; /* 0 = log-sum form (default), 1 = reference form */
 	int32_t no_cache;        /* 1: bypass the code-object cache */
-	int32_t reserved[6];
+	int32_t with_gradient;   /* mbg_generate_code only: append the gradient kernel's
+	                            model code and include (MBG_ERR_LIMIT if the model has none) */
+	int32_t reserved[5];
 } mbg_compile_opts;
 
 /* Generated HIP source of the model (caller frees with mbg_free_string). */
@@ -176,6 +178,45 @@ void mbg_batch_params_default(mbg_batch_params* p);
  * v's statistics (free each with mbg_result_free); info may be NULL. */
 int mbg_simulate_batched(mbg_program* p, const mbg_batch_params* params, int32_t n_variants,
 						 const mbg_variant* variants, void* stream, mbg_result** results, mbg_batch_info* info);
+
+/* ---- gradient form.  No counterpart in the reference (its rates are
+ *      compile-time constants, src/generator.cpp:104-106, and nothing is
+ *      differentiated).  Likelihood-ratio (score function) estimate of
+ *      dL/dtheta for the model's rate parameters (mbg_model_param_name
+ *      order), given dL/dP for the window probabilities P[w][x] of the
+ *      forward run with the same trajectory ids, seed, stream tag and rates.
+ *      Continuous time; every rate must be a bare $parameter in the fast
+ *      form (`@logic ? $u : 0`, `@logic ? 0 : $d`) or parameter-free. ---- */
+typedef struct mbg_grad_params
+{
+	int64_t n_trajectories;   /* <0: sample_count of the model */
+	uint64_t first_id;        /* global id of the first trajectory */
+	int64_t seed;             /* <0: seed_pseudorandom of the model */
+	uint32_t variant;         /* stream tag of the forward run (batched variant index) */
+	uint32_t max_steps;       /* 0: default cap */
+	int32_t block_size;       /* 0: from the LDS budget (64..256) */
+	int32_t blocks_per_cu;    /* 0: occupancy limit */
+	int32_t reserved[6];
+} mbg_grad_params;
+
+typedef struct mbg_grad_info
+{
+	double kernel_ms;         /* device time of the gradient kernel */
+	uint64_t transitions;     /* SSA steps replayed */
+	int32_t block_size, grid;
+} mbg_grad_info;
+
+void mbg_grad_params_default(mbg_grad_params* p);
+/* MBG_OK when the model has a gradient form, else the reason (mbg_last_error). */
+int mbg_grad_check(mbg_program* p);
+/* grad_params[j] = sum_{w,x} grad_probs[w << n_obs | x] * dP[w][x]/dtheta_j.
+ * rates: device [n_params] floats (NULL: the model's values); grad_probs:
+ * device [windows << n_observed] doubles; grad_params: device [n_params]
+ * doubles, overwritten.  Stream ordered; when info is non-NULL the stream is
+ * synchronised and info filled.  Calls on one program share its workspace:
+ * issue them on one stream. */
+int mbg_backward(mbg_program* p, const mbg_grad_params* params, const float* rates, const double* grad_probs,
+				 double* grad_params, void* stream, mbg_grad_info* info);
 
 /* ---- results (replaces stats::finalize / visualize / write_csv,
  *      src/statistics/stats.h:15-29) ---- */

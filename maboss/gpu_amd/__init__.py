@@ -88,8 +88,8 @@ class Model:
         check(lib().mbg_model_eval_rate(self._h, node, arr, ctypes.byref(out)))
         return out.value
 
-    def generate_code(self, entropy_formula: int = 0) -> str:
-        opts = CompileOpts(entropy_formula=entropy_formula)
+    def generate_code(self, entropy_formula: int = 0, with_gradient: bool = False) -> str:
+        opts = CompileOpts(entropy_formula=entropy_formula, with_gradient=int(with_gradient))
         p = ctypes.c_void_p()
         check(lib().mbg_generate_code(self._h, ctypes.byref(opts), ctypes.byref(p)))
         try:

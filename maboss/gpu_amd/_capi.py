@@ -32,6 +32,7 @@ EXPORTED = [
     "mbg_program_compile_seconds",
     "mbg_sim_params_default", "mbg_simulate",
     "mbg_batch_params_default", "mbg_simulate_batched",
+    "mbg_grad_params_default", "mbg_grad_check", "mbg_backward",
     "mbg_result_free", "mbg_result_get_info", "mbg_result_window_size", "mbg_result_window",
     "mbg_result_window_probs", "mbg_result_entropies", "mbg_result_final_size",
     "mbg_result_final", "mbg_result_fixed_size", "mbg_result_fixed",
@@ -62,7 +63,7 @@ class ModelInfo(ctypes.Structure):
 class CompileOpts(ctypes.Structure):
     _fields_ = [
         ("entropy_formula", ctypes.c_int32), ("no_cache", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("with_gradient", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -102,6 +103,22 @@ class BatchInfo(ctypes.Structure):
         ("transitions", ctypes.c_uint64), ("block_size", ctypes.c_int32),
         ("blocks_per_cu", ctypes.c_int32), ("traj_per_lane", ctypes.c_int32),
         ("used_graph", ctypes.c_int32),
+    ]
+
+
+class GradParams(ctypes.Structure):
+    _fields_ = [
+        ("n_trajectories", ctypes.c_int64), ("first_id", ctypes.c_uint64),
+        ("seed", ctypes.c_int64), ("variant", ctypes.c_uint32),
+        ("max_steps", ctypes.c_uint32), ("block_size", ctypes.c_int32),
+        ("blocks_per_cu", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6),
+    ]
+
+
+class GradInfo(ctypes.Structure):
+    _fields_ = [
+        ("kernel_ms", ctypes.c_double), ("transitions", ctypes.c_uint64),
+        ("block_size", ctypes.c_int32), ("grid", ctypes.c_int32),
     ]
 
 
@@ -150,6 +167,9 @@ def _declare(lib):
         "mbg_sim_params_default": ([ctypes.POINTER(SimParams)], None),
         "mbg_simulate": ([P, ctypes.POINTER(SimParams), P, pp], ctypes.c_int),
         "mbg_batch_params_default": ([ctypes.POINTER(BatchParams)], None),
+        "mbg_grad_params_default": ([ctypes.POINTER(GradParams)], None),
+        "mbg_grad_check": ([P], ctypes.c_int),
+        "mbg_backward": ([P, ctypes.POINTER(GradParams), P, P, P, P, ctypes.POINTER(GradInfo)], ctypes.c_int),
         "mbg_simulate_batched": ([P, ctypes.POINTER(BatchParams), i32, ctypes.POINTER(Variant), P, pp,
                                   ctypes.POINTER(BatchInfo)], ctypes.c_int),
         "mbg_result_free": ([P], None),

@@ -1,0 +1,209 @@
+// Likelihood-ratio gradient kernel for gfx950 (no counterpart in the
+// reference).  Compiled by hipRTC together with a model's generated code,
+// which provides MBG_W, MBG_GRAD_NP, mbg_eval_p, mbg_obs_index and
+// mbg_grad_dwell, so the per-parameter accumulators are statically indexed
+// registers.
+//
+// Estimator.  P_w(x) = E[(1/tick) * time spent in window w with observed
+// state x].  For an upstream gradient G[w][x] = dL/dP_w(x),
+//   dL/dtheta_j = 1/(n tick) * sum_traj sum_w Gw * S_j(e_w)
+// where Gw = sum of G[w][obs] * slice over the trajectory's dwell slices in
+// window w, e_w is the end of window w (or of the trajectory) and S_j is the
+// score of the path up to e_w:
+//   S_j(t) = sum_{jumps before t} 1/theta_j [jump uses theta_j]
+//            - integral_0^t c_j(s(u)) du,   c_j = d total / d theta_j.
+// Gw of later windows is not known when a jump or dwell happens, so the
+// kernel uses the suffix form of the same sum: with C(u) = sum of Gw over the
+// windows completed by time u and Gtot = C at the trajectory's end,
+//   sum_w Gw S_j(e_w) = Gtot * (n_j / theta_j - T_j)
+//                       - sum_{jumps of j} C(t_k) / theta_j + BT_j,
+//   T_j = integral c_j du,  BT_j = integral c_j C du.
+// Per lane and parameter T_j and BT_j live in registers (float, one
+// trajectory), n_j in LDS (u16, lane private); the C(t_k) term and the
+// trajectory totals go to per-block double accumulators in LDS, and the
+// block totals to the output with one atomic per parameter.
+//
+// The trajectory is the forward kernel's: same Philox stream (seed, id,
+// variant), same rates (runtime values through mbg_eval_p), same time and
+// flip arithmetic, so the sample is the one the forward statistics came from.
+#pragma once
+
+#include "mbg_kernel_args.h"
+#include "mbg_lane_core.h"
+
+#ifndef MBG_GRAD_NP
+#error "MBG_GRAD_NP must be defined by the generated code"
+#endif
+
+extern "C" __global__ void __launch_bounds__(256) simulate_grad(mbg_grad_args a)
+{
+	constexpr int NP = MBG_GRAD_NP;
+	constexpr int W = MBG_W;
+	extern __shared__ double glds[];
+	// LDS: [NP] block accumulators, [NP] 1/theta, [cells] G (optional),
+	// [2N] jump parameters (u16), [NP][blockDim] jump counts (u16)
+	double* acc = glds;
+	double* invt = glds + NP;
+	const uint64_t cells = (uint64_t)a.windows << a.n_obs;
+	double* gtab = glds + 2 * NP;
+	unsigned short* jp = (unsigned short*)(gtab + (a.g_lds ? cells : 0));
+	unsigned short* cnt = jp + ((2 * a.n_nodes + 1) & ~1);
+	for (int i = threadIdx.x; i < NP; i += blockDim.x)
+	{
+		acc[i] = 0.0;
+		const float r = a.pv.rate[i];
+		invt[i] = r > 0.0f ? 1.0 / (double)r : 0.0;
+	}
+	if (a.g_lds)
+		for (uint64_t i = threadIdx.x; i < cells; i += blockDim.x)
+			gtab[i] = a.gprob[i];
+	for (int i = threadIdx.x; i < 2 * a.n_nodes; i += blockDim.x)
+		jp[i] = a.jparam[i];
+	for (int i = threadIdx.x; i < NP * (int)blockDim.x; i += blockDim.x)
+		cnt[i] = 0;
+	__syncthreads();
+	const double* G = a.g_lds ? gtab : a.gprob;
+	unsigned short* mycnt = cnt + threadIdx.x; // count of parameter j at mycnt[j * blockDim.x]
+
+	const int lane = __lane_id();
+	const uint32_t B0 = ((uint32_t)a.n_nodes + 3u) >> 2;
+	uint32_t s[W];
+#pragma unroll
+	for (int w = 0; w < W; ++w)
+		s[w] = 0u;
+	float T[NP], BT[NP];
+#pragma unroll
+	for (int j = 0; j < NP; ++j)
+	{
+		T[j] = 0.0f;
+		BT[j] = 0.0f;
+	}
+	float t = 0.0f;
+	uint32_t k = 0;
+	uint64_t id = 0;
+	bool active = false, exhausted = false;
+	double C = 0.0, gw = 0.0; // completed windows' G; G of the window in progress
+	uint64_t ntrans = 0;
+
+	for (;;)
+	{
+		const uint64_t need = __ballot(!active && !exhausted);
+		if (need)
+		{
+			const int leader = __ffsll((unsigned long long)need) - 1;
+			uint64_t base = 0;
+			if (lane == leader)
+				base = atomicAdd(a.counters + MBG_CTR_QUEUE, (unsigned long long)__popcll(need));
+			base = mbg_dev::bcast64(base, leader);
+			if (!active && !exhausted)
+			{
+				const uint64_t mine = base + __popcll(need & ((1ull << lane) - 1ull));
+				if (mine < a.n_traj)
+				{
+					id = a.id_base + mine;
+					mbg_dev::init_state<W>(s, id, a.variant, a.seed_lo, a.seed_hi, a.n_nodes, a.istate);
+					t = 0.0f;
+					k = 0;
+					C = 0.0;
+					gw = 0.0;
+					active = true;
+				}
+				else
+					exhausted = true;
+			}
+		}
+		if (__ballot(active) == 0)
+			break;
+
+		bool ended = false;
+		if (active)
+		{
+			float u1, u2;
+			mbg_dev::step_uniforms(false, B0, k, id, a.variant, a.seed_lo, a.seed_hi, u1, u2);
+			mbg_eval_t e;
+			mbg_eval_p(s, u2, &a.pv, &e);
+			const float tp = t;
+			bool fixed = false;
+			t = mbg_dev::advance_time(false, t, e.total, u1, a.tick, a.max_time, fixed);
+			ntrans++;
+
+			// dwell slices of [tp, t): G weights, window completions, C-weighted dwell
+			const uint64_t obs = mbg_obs_index(s);
+			int w = (int)floorf(tp / a.tick);
+			float b = tp, dsum = 0.0f;
+			double cdt = 0.0;
+			while (t > b)
+			{
+				const float wend = (float)(w + 1) * a.tick;
+				const float en = fminf(t, wend);
+				const float sl = en - b;
+				cdt += C * (double)sl;
+				if (w < a.windows)
+					gw += G[((uint64_t)w << a.n_obs) | obs] * (double)sl;
+				dsum += sl;
+				if (en == wend)
+				{
+					C += gw;
+					gw = 0.0;
+				}
+				++w;
+				b = en;
+			}
+			mbg_grad_dwell(s, dsum, (float)cdt, T, BT);
+
+			k++;
+			if (t >= a.max_time || k >= a.max_steps)
+				ended = true;
+			else
+			{
+				const int f = e.flip;
+				const uint32_t xf = (s[f >> 5] >> (f & 31)) & 1u;
+				const unsigned p = jp[2 * f + (int)xf];
+				if (p != 0xffffu)
+				{
+					mycnt[p * blockDim.x] += 1;
+					atomicAdd(acc + p, -invt[p] * C);
+				}
+#pragma unroll
+				for (int w2 = 0; w2 < W; ++w2)
+					s[w2] ^= ((f >> 5) == w2) ? (1u << (f & 31)) : 0u;
+				// keep the 16-bit counts exact on very long trajectories
+				if ((k & 0x7fffu) == 0u)
+				{
+#pragma unroll
+					for (int j = 0; j < NP; ++j)
+					{
+						T[j] -= (float)((double)mycnt[j * blockDim.x] * invt[j]);
+						mycnt[j * blockDim.x] = 0;
+					}
+				}
+			}
+		}
+
+		if (ended)
+		{
+			// trajectory total: Gtot * (n_j / theta_j - T_j) + BT_j
+			const double gt = C + gw;
+#pragma unroll
+			for (int j = 0; j < NP; ++j)
+			{
+				const double c = gt * ((double)mycnt[j * blockDim.x] * invt[j] - (double)T[j]) + (double)BT[j];
+				mycnt[j * blockDim.x] = 0;
+				if (c != 0.0)
+					atomicAdd(acc + j, c);
+				T[j] = 0.0f;
+				BT[j] = 0.0f;
+			}
+			active = false;
+		}
+	}
+
+	for (int off = 32; off > 0; off >>= 1)
+		ntrans += __shfl_xor(ntrans, off);
+	if (lane == 0 && ntrans)
+		atomicAdd(a.counters + MBG_CTR_TRANSITIONS, (unsigned long long)ntrans);
+	__syncthreads();
+	for (int j = threadIdx.x; j < NP; j += blockDim.x)
+		if (acc[j] != 0.0)
+			atomicAdd(a.grad + j, acc[j] * a.scale);
+}

@@ -112,3 +112,24 @@ struct mbg_batch_chunk
 	unsigned int variant, pad;
 	unsigned long long begin, end;
 };
+
+// Likelihood-ratio gradient kernel (no counterpart in the reference).  One
+// lane per trajectory replays the forward trajectory (same stream, same
+// float arithmetic) and accumulates, per rate parameter, the score terms of
+// sum_w Gw * S(e_w) in the suffix form described in mbg_grad_kernel.h.
+struct mbg_grad_args
+{
+	unsigned long long n_traj, id_base;
+	unsigned int seed_lo, seed_hi, variant, max_steps;
+	int n_nodes, n_obs, windows, n_params;
+	float max_time, tick;
+	int g_lds;         // 1: the upstream gradient table is staged in LDS
+	int lds_bytes;     // dynamic LDS of a block
+	const float* istate;
+	mbg_param_view pv; // rates (runtime values), masks all enabled
+	const double* gprob;           // [windows << n_obs] dL/dP
+	const unsigned short* jparam;  // [2 * n_nodes]: parameter of node i's up (2i) / down (2i+1) jump, 0xffff none
+	double* grad;                  // [n_params], accumulated with atomics (zeroed by the host)
+	unsigned long long* counters;  // [MBG_NCOUNTERS]
+	double scale;                  // 1 / (n_traj_total * tick)
+};

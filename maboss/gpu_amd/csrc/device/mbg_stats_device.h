@@ -2,7 +2,9 @@
 // and the window / entropy / final-state histogram updates.
 #pragma once
 
+#if !defined(__HIPCC_RTC__)
 #include <hip/hip_runtime.h>
+#endif
 
 #include "mbg_kernel_args.h"
 

@@ -201,7 +201,10 @@ int mbg_generate_code(const mbg_model* mm, const mbg_compile_opts* opts, char** 
 	return guarded([&] {
 		if (!mm || !source)
 			throw mbg::error(mbg::MBG_E_ARG, "null argument");
-		std::string s = mbg::generate(*mm->m, to_gen(opts)).source;
+		mbg::generated g = mbg::generate(*mm->m, to_gen(opts));
+		std::string s = g.source;
+		if (opts && opts->with_gradient)
+			s += mbg::generate_grad(*mm->m, g).source + "\n#include \"mbg_grad_kernel.h\"\n";
 		char* c = (char*)std::malloc(s.size() + 1);
 		if (!c)
 			throw std::bad_alloc();
@@ -355,6 +358,54 @@ int mbg_simulate_batched(mbg_program* p, const mbg_batch_params* params, int32_t
 			info->blocks_per_cu = bt.per_cu;
 			info->traj_per_lane = bt.tpl;
 			info->used_graph = bt.graph ? 1 : 0;
+		}
+	});
+}
+
+void mbg_grad_params_default(mbg_grad_params* p)
+{
+	if (!p)
+		return;
+	std::memset(p, 0, sizeof(*p));
+	p->n_trajectories = -1;
+	p->seed = -1;
+}
+
+int mbg_grad_check(mbg_program* p)
+{
+	return guarded([&] {
+		if (!p)
+			throw mbg::error(mbg::MBG_E_ARG, "null program");
+		(void)p->p->grad();
+	});
+}
+
+int mbg_backward(mbg_program* p, const mbg_grad_params* params, const float* rates, const double* grad_probs,
+				 double* grad_params, void* stream, mbg_grad_info* info)
+{
+	return guarded([&] {
+		if (!p || !grad_probs || !grad_params)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		mbg::grad_params gp;
+		if (params)
+		{
+			gp.n_traj = params->n_trajectories;
+			gp.first_id = params->first_id;
+			gp.seed = params->seed;
+			gp.variant = params->variant;
+			gp.max_steps = params->max_steps;
+			gp.block = params->block_size;
+			gp.per_cu = params->blocks_per_cu;
+		}
+		mbg::grad_timing gt;
+		mbg::simulate_grad(*p->p, gp, rates, grad_probs, grad_params, (hipStream_t)stream, info ? &gt : nullptr);
+		if (info)
+		{
+			std::memset(info, 0, sizeof(*info));
+			info->kernel_ms = gt.kernel_ms;
+			info->transitions = gt.transitions;
+			info->block_size = gt.block;
+			info->grid = gt.grid;
 		}
 	});
 }

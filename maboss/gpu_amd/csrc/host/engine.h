@@ -82,6 +82,15 @@ struct program
 	uint64_t batch_graph_key = 0;
 	device_buffer b_bwin, b_bent, b_bfin, b_bfp, b_bctr, b_bvar, b_bchunk, b_btab;
 
+	// likelihood-ratio gradient kernel (compiled per model on first use)
+	std::unique_ptr<grad_code> gcode;
+	hipModule_t mod_grad = nullptr;
+	hipFunction_t fn_grad = nullptr;
+	const grad_code& grad();          // throws MBG_E_LIMIT when the model has no gradient form
+	hipFunction_t grad_function();
+	device_buffer b_grates, b_gen, b_gjp;
+	hipFunction_t fn_fill_u64 = nullptr; // utility module's fill (graph safe)
+
 	// wave-per-trajectory kernel (precompiled, table driven)
 	hipModule_t mod_wave = nullptr;
 	std::unique_ptr<wave_tables> wtab;
@@ -177,5 +186,30 @@ struct batch_timing
 std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_params& bp,
 													  const std::vector<variant_spec>& vs, hipStream_t stream,
 													  batch_timing* timing);
+
+// ---- gradient form
+struct grad_params
+{
+	int64_t n_traj = -1;
+	uint64_t first_id = 0;
+	int64_t seed = -1;
+	uint32_t variant = 0;
+	uint32_t max_steps = 0;
+	int block = 0, per_cu = 0;
+};
+
+struct grad_timing
+{
+	double kernel_ms = 0;
+	uint64_t transitions = 0;
+	int block = 0, grid = 0;
+};
+
+// dL/dtheta (device, [n_params]) from dL/dP (device, [windows << n_obs]) on
+// the trajectories of the forward run with the same ids, seed, variant and
+// rates (device [n_params], or null for the model's values).  Stream
+// ordered; with timing the stream is synchronised and timing filled.
+void simulate_grad(program& p, const grad_params& gp, const float* rates, const double* gprob, double* grad,
+				   hipStream_t stream, grad_timing* timing);
 
 } // namespace mbg

@@ -418,4 +418,72 @@ generated generate(const model& m, const gen_options& opt)
 	return g;
 }
 
+// d total / d theta_j in a state is the number of nodes whose current rate
+// is theta_j: node i in the fast form has rate theta_u (up_param) when off
+// with logic on, theta_d (down_param) when on with logic off, else 0.  The
+// emitted mbg_grad_dwell adds that indicator times the record's dwell (T) and
+// times its G-weighted dwell (BT) to per-parameter register accumulators.
+grad_code generate_grad(const model& m, const generated& g)
+{
+	const int N = m.n_nodes();
+	std::map<std::string, int> pidx;
+	for (size_t k = 0; k < g.param_names.size(); k++)
+		pidx[g.param_names[k]] = (int)k;
+	grad_code gc;
+	gc.n_params = (int)g.param_names.size();
+	if (gc.n_params == 0)
+		throw error(MBG_E_LIMIT, "the model's rates use no $variable to differentiate");
+	if (gc.n_params > 128)
+		throw error(MBG_E_LIMIT, "the gradient form keeps one register pair per parameter: at most 128 parameters");
+	if (m.discrete_time())
+		throw error(MBG_E_LIMIT, "the gradient form is defined for continuous time");
+	gc.jparam.assign(2 * (size_t)N, 0xffff);
+
+	std::ostringstream os;
+	os << "\n#define MBG_GRAD_NP " << gc.n_params << "\n\n";
+	os << "__device__ __forceinline__ void mbg_grad_dwell(const uint32_t* __restrict__ s, float dt, float cdt, "
+		  "float (&T)[MBG_GRAD_NP], float (&BT)[MBG_GRAD_NP])\n{\n";
+	for (int i = 0; i < N; i++)
+		os << "\tconst bool x" << i << " = ((s[" << i / 32 << "] >> " << i % 32 << ") & 1u) != 0u;\n";
+	for (int i = 0; i < N; i++)
+	{
+		const node_t& n = m.nodes[i];
+		const node_info& ni = g.nodes[i];
+		auto uses = [&](const char* attr) {
+			std::set<std::string> v;
+			if (const expr* e = n.attr(attr))
+				collect_vars(m, n, *e, v);
+			return !v.empty();
+		};
+		const bool up_uses = uses("rate_up"), down_uses = uses("rate_down");
+		if (!up_uses && !down_uses)
+			continue;
+		if (!ni.fast_form || (up_uses && ni.up_param < 0) || (down_uses && ni.down_param < 0))
+			throw error(MBG_E_LIMIT, "the gradient form needs node " + n.name
+										 + "'s rates to be `@logic ? $u : 0` / `@logic ? 0 : $d` or parameter-free");
+		emitter em(m, n, false, pidx);
+		const std::string logic = em.as_b(*n.attr("logic"));
+		os << "\t{ // node " << i << ": " << n.name << "\n";
+		os << "\t\tconst bool x = x" << i << ";\n";
+		os << "\t\tconst bool L = " << logic << ";\n";
+		if (ni.up_param >= 0)
+		{
+			gc.jparam[2 * (size_t)i] = (unsigned short)ni.up_param;
+			os << "\t\t{ const float c = (!x && L) ? 1.0f : 0.0f; T[" << ni.up_param << "] = fmaf(c, dt, T["
+			   << ni.up_param << "]); BT[" << ni.up_param << "] = fmaf(c, cdt, BT[" << ni.up_param << "]); }\n";
+		}
+		if (ni.down_param >= 0)
+		{
+			gc.jparam[2 * (size_t)i + 1] = (unsigned short)ni.down_param;
+			os << "\t\t{ const float c = (x && !L) ? 1.0f : 0.0f; T[" << ni.down_param << "] = fmaf(c, dt, T["
+			   << ni.down_param << "]); BT[" << ni.down_param << "] = fmaf(c, cdt, BT[" << ni.down_param
+			   << "]); }\n";
+		}
+		os << "\t}\n";
+	}
+	os << "}\n";
+	gc.source = os.str();
+	return gc;
+}
+
 } // namespace mbg

@@ -45,6 +45,18 @@ struct generated
 
 generated generate(const model& m, const gen_options& opt);
 
+// Gradient form: per-model code of the likelihood-ratio kernel (appended to
+// generated::source) and, per node, the parameter its up (2i) and down
+// (2i+1) jump uses (0xffff: none).  Every rate must be a bare $variable in
+// the fast form, or use no $variable; MBG_E_LIMIT otherwise.
+struct grad_code
+{
+	std::string source;
+	std::vector<unsigned short> jparam;
+	int n_params = 0;
+};
+grad_code generate_grad(const model& m, const generated& g);
+
 // Value of rate * log2(rate) as the device computes it (shared float code).
 float rate_log2_rate(float r);
 

@@ -23,6 +23,10 @@
 extern "C" {
 extern const char mbg_hdr_numeric[];
 extern const char mbg_hdr_abi[];
+extern const char mbg_hdr_kernel_args[];
+extern const char mbg_hdr_stats_device[];
+extern const char mbg_hdr_lane_core[];
+extern const char mbg_hdr_grad_kernel[];
 extern const unsigned char mbg_co_wave[];
 extern const unsigned char mbg_co_util[];
 }
@@ -151,6 +155,39 @@ std::vector<char> compile_to_bitcode(const std::string& source)
 	return bc;
 }
 
+// Whole kernel in one translation unit (gradient form): compile straight to
+// a code object.
+std::vector<char> compile_to_code_object(const std::string& source)
+{
+	const char* headers[] = { mbg_hdr_numeric,		mbg_hdr_abi,	   mbg_hdr_kernel_args,
+							  mbg_hdr_stats_device, mbg_hdr_lane_core, mbg_hdr_grad_kernel };
+	const char* names[] = { "mbg_numeric.h",	  "mbg_abi.h",		"mbg_kernel_args.h",
+							"mbg_stats_device.h", "mbg_lane_core.h", "mbg_grad_kernel.h" };
+	std::vector<const char*> flags;
+	for (const char* f : k_compile_flags)
+		if (std::strcmp(f, "-fgpu-rdc") != 0)
+			flags.push_back(f);
+	hiprtcProgram prog;
+	rtc_check(hiprtcCreateProgram(&prog, source.c_str(), "mbg_grad.hip", 6, headers, names), "hiprtcCreateProgram");
+	hiprtcResult r = hiprtcCompileProgram(prog, (int)flags.size(), flags.data());
+	size_t log_size = 0;
+	hiprtcGetProgramLogSize(prog, &log_size);
+	std::string log(log_size, '\0');
+	if (log_size)
+		hiprtcGetProgramLog(prog, &log[0]);
+	if (r != HIPRTC_SUCCESS)
+	{
+		hiprtcDestroyProgram(&prog);
+		throw error(MBG_E_COMPILE, std::string("hipRTC compile failed: ") + hiprtcGetErrorString(r) + "\n" + log);
+	}
+	size_t n = 0;
+	rtc_check(hiprtcGetCodeSize(prog, &n), "hiprtcGetCodeSize");
+	std::vector<char> co(n);
+	rtc_check(hiprtcGetCode(prog, co.data()), "hiprtcGetCode");
+	hiprtcDestroyProgram(&prog);
+	return co;
+}
+
 std::vector<char> link_code_object(std::vector<char>& model_bc, const embedded_bc& kbc)
 {
 	hiprtcLinkState ls;
@@ -190,6 +227,49 @@ program::~program()
 		(void)hipGraphExecDestroy(batch_graph);
 	if (mod_util)
 		(void)hipModuleUnload(mod_util);
+	if (mod_grad)
+		(void)hipModuleUnload(mod_grad);
+}
+
+const grad_code& program::grad()
+{
+	if (!gcode)
+		gcode = std::make_unique<grad_code>(generate_grad(*mdl, gen));
+	return *gcode;
+}
+
+hipFunction_t program::grad_function()
+{
+	if (fn_grad)
+		return fn_grad;
+	const grad_code& gc = grad();
+	scope_timer tm("compiler> gradient kernel");
+	const double t0 = wall_seconds();
+	const std::string src = gen.source + gc.source + "\n#include \"mbg_grad_kernel.h\"\n";
+	uint64_t h = fnv1a(src.data(), src.size());
+	for (const char* f : k_compile_flags)
+		h = fnv1a(f, std::strlen(f), h);
+	for (const char* hd : { mbg_hdr_numeric, mbg_hdr_abi, mbg_hdr_kernel_args, mbg_hdr_stats_device,
+							mbg_hdr_lane_core, mbg_hdr_grad_kernel })
+		h = fnv1a(hd, std::strlen(hd), h);
+	char name[64];
+	std::snprintf(name, sizeof(name), "%016llx_g.co", (unsigned long long)h);
+	const std::string dir = cache_dir();
+	const std::string path = dir + "/" + name;
+	std::vector<char> co;
+	if (no_cache || !read_all(path, co))
+	{
+		co = compile_to_code_object(src);
+		if (!no_cache)
+		{
+			mkdirs(dir);
+			write_atomic(path, co.data(), co.size());
+		}
+	}
+	hip_check(hipModuleLoadData(&mod_grad, co.data()), "hipModuleLoadData(simulate_grad)");
+	hip_check(hipModuleGetFunction(&fn_grad, mod_grad, "simulate_grad"), "hipModuleGetFunction(simulate_grad)");
+	compile_seconds += wall_seconds() - t0;
+	return fn_grad;
 }
 
 void program::load_util()
@@ -200,6 +280,7 @@ void program::load_util()
 	hip_check(hipModuleGetFunction(&fn_fill_slots, mod_util, "mbg_fill_slots"), "hipModuleGetFunction(mbg_fill_slots)");
 	hip_check(hipModuleGetFunction(&fn_expand_tails, mod_util, "mbg_expand_tails"),
 			  "hipModuleGetFunction(mbg_expand_tails)");
+	hip_check(hipModuleGetFunction(&fn_fill_u64, mod_util, "mbg_fill_u64"), "hipModuleGetFunction(mbg_fill_u64)");
 }
 
 hipFunction_t program::wave_function()

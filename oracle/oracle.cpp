@@ -41,6 +41,10 @@
 //   --time           print a JSON timing line to stderr
 //   --eval-states    read lines "w0 .. w{W-1} u2bits" on stdin, print
 //                    "total H flip obs" (float bits in hex) per line
+//   --grad-in PATH   likelihood-ratio gradient mode: PATH holds the upstream
+//                    gradient dL/dP as windows * 2^n_obs doubles; prints
+//                    "transitions N" and "name dL/dtheta" per parameter
+//   --grad-out PATH  where to write those lines (default stdout)
 // maboss_oracle --numeric-probe reads "P c0 c1 c2 c3 k0 k1" (Philox),
 // "L bits" (log2), "U x" (uniform) lines on stdin and prints the results.
 #include <algorithm>
@@ -1103,6 +1107,166 @@ void trajectory(const Sim& S, uint64_t id, Stats& st, std::vector<float>& r, std
 	}
 }
 
+// ------------------------------------------------------- likelihood-ratio gradient
+//
+// No counterpart in the reference.  Window probability P_w(x) = E[(1/tick)
+// * time in window w with observed state x]; for a loss with upstream
+// gradient G[w][x] = dL/dP_w(x), the score-function estimator is
+//   dL/dtheta_j = 1/(n tick) * sum_traj sum_w Gw * S_j(e_w),
+//   Gw = sum over the dwell slices of window w of G[w][obs] * slice,
+//   S_j(t) = sum_{jumps k, t_k <= t} (d r_{f_k} / d theta_j) / r_{f_k}
+//            - integral_0^t sum_i d r_i / d theta_j du,
+// e_w the end of window w (or the end of the trajectory).  Computed here
+// directly, in double precision, with derivatives of the rate expressions
+// taken by differentiating the expression trees.
+
+// d e / d theta for the variable named p, at state s (values by run()).
+double dval(const Ex* e, const uint32_t* s, const std::string& p)
+{
+	switch (e->k)
+	{
+		case VAR: return e->name == p ? 1.0 : 0.0;
+		case PAREN:
+		case POS: return dval(e->a.get(), s, p);
+		case NEG: return -dval(e->a.get(), s, p);
+		case ADD: return dval(e->a.get(), s, p) + dval(e->b.get(), s, p);
+		case SUB: return dval(e->a.get(), s, p) - dval(e->b.get(), s, p);
+		case MUL:
+			return dval(e->a.get(), s, p) * run(e->b.get(), s) + (double)run(e->a.get(), s) * dval(e->b.get(), s, p);
+		case DIV:
+		{
+			const double a = run(e->a.get(), s), b = run(e->b.get(), s);
+			return (dval(e->a.get(), s, p) * b - a * dval(e->b.get(), s, p)) / (b * b);
+		}
+		case TERN: return run(e->a.get(), s) != 0.0f ? dval(e->b.get(), s, p) : dval(e->c.get(), s, p);
+		default: return 0.0; // literals, node values and boolean results are locally constant
+	}
+}
+
+void collect_var_names(const Ex* e, std::set<std::string>& out)
+{
+	if (!e)
+		return;
+	if (e->k == VAR)
+		out.insert(e->name);
+	collect_var_names(e->a.get(), out);
+	collect_var_names(e->b.get(), out);
+	collect_var_names(e->c.get(), out);
+}
+
+struct Grad
+{
+	std::vector<std::string> names;            // parameters, sorted by name
+	std::vector<std::vector<int>> up_p, down_p; // parameters each rate tree uses
+	std::vector<double> G;                      // [windows << nobs]
+
+	void init(const Sim& S)
+	{
+		std::set<std::string> all;
+		for (int i = 0; i < S.N; i++)
+		{
+			collect_var_names(S.up[i].get(), all);
+			collect_var_names(S.down[i].get(), all);
+		}
+		names.assign(all.begin(), all.end());
+		auto uses = [&](const Ex* e) {
+			std::set<std::string> v;
+			collect_var_names(e, v);
+			std::vector<int> idx;
+			for (auto& nm : v)
+				idx.push_back((int)(std::lower_bound(names.begin(), names.end(), nm) - names.begin()));
+			return idx;
+		};
+		for (int i = 0; i < S.N; i++)
+		{
+			up_p.push_back(uses(S.up[i].get()));
+			down_p.push_back(uses(S.down[i].get()));
+		}
+	}
+};
+
+// One trajectory (the same draws and float arithmetic as trajectory()),
+// adding sum_w Gw * S(e_w) to acc.
+void trajectory_grad(const Sim& S, const Grad& g, uint64_t id, std::vector<double>& acc, std::vector<float>& r,
+					 std::vector<uint32_t>& s, uint64_t& transitions)
+{
+	const size_t P = g.names.size();
+	std::vector<double> score(P, 0.0), drate(P);
+	const uint32_t idlo = (uint32_t)id, idhi = (uint32_t)(id >> 32);
+	std::fill(s.begin(), s.end(), 0u);
+	for (int i = 0; i < S.N; i += 4)
+	{
+		u4 x = philox(u4 { { (uint32_t)(i >> 2), idlo, idhi, S.variant } }, S.seed_lo, S.seed_hi);
+		for (int j = 0; j < 4 && i + j < S.N; j++)
+			if (u01(x.v[j]) <= S.istate[i + j])
+				s[(i + j) >> 5] |= 1u << ((i + j) & 31);
+	}
+	const uint32_t B0 = ((uint32_t)S.N + 3u) >> 2;
+	double gw = 0.0; // G-weighted occupancy of the window in progress
+	float t = 0.0f;
+	for (uint32_t k = 0;;)
+	{
+		u4 x = philox(u4 { { B0 + (k >> 1), idlo, idhi, S.variant } }, S.seed_lo, S.seed_hi);
+		const float u1 = u01(x.v[2 * (k & 1)]);
+		const float u2 = u01(x.v[2 * (k & 1) + 1]);
+		S.rates(s.data(), r.data());
+		float L[64], P64[64];
+		const float total = S.total_of(r.data(), L, P64);
+		const float tp = t;
+		if (total == 0.0f)
+			t = S.max_time;
+		else
+			t = std::fmin(t + (-ln_poly(u1)) / total, S.max_time);
+		transitions++;
+		// d total / d theta in this state
+		std::fill(drate.begin(), drate.end(), 0.0);
+		for (int i = 0; i < S.N; i++)
+		{
+			const bool on = (s[i >> 5] >> (i & 31)) & 1u;
+			const Ex* e = on ? S.down[i].get() : S.up[i].get();
+			for (int j : on ? g.down_p[i] : g.up_p[i])
+				drate[j] += dval(e, s.data(), g.names[j]);
+		}
+		const uint64_t o = S.obs_index(s.data());
+		int w = (int)std::floor(tp / S.tick);
+		float b = tp;
+		double elapsed = 0.0; // dwell of this record so far
+		while (t > b)
+		{
+			const float wend = (float)(w + 1) * S.tick;
+			const float en = std::fmin(t, wend);
+			const float sl = en - b;
+			if (w < S.windows)
+				gw += g.G[((size_t)w << S.nobs) | o] * (double)sl;
+			elapsed += (double)sl;
+			if (en == wend)
+			{
+				// window w ends inside this record: S_j(e_w) = score_j - drate_j * elapsed
+				for (size_t j = 0; j < P; j++)
+					acc[j] += gw * (score[j] - drate[j] * elapsed);
+				gw = 0.0;
+			}
+			w++;
+			b = en;
+		}
+		for (size_t j = 0; j < P; j++)
+			score[j] -= drate[j] * elapsed;
+		k++;
+		if (t >= S.max_time || k >= S.max_steps)
+		{
+			for (size_t j = 0; j < P; j++)
+				acc[j] += gw * score[j];
+			return;
+		}
+		const int flip = S.select_flip(r.data(), total, u2 * total, L, P64);
+		const bool on = (s[flip >> 5] >> (flip & 31)) & 1u;
+		const Ex* e = on ? S.down[flip].get() : S.up[flip].get();
+		for (int j : on ? g.down_p[flip] : g.up_p[flip])
+			score[j] += dval(e, s.data(), g.names[j]) / (double)r[flip];
+		s[flip >> 5] ^= 1u << (flip & 31);
+	}
+}
+
 rates_fn compile_rates(const Sim& S, std::string& so_path)
 {
 	char dir[] = "/tmp/maboss_oracle_XXXXXX";
@@ -1180,7 +1344,7 @@ int main(int argc, char** argv)
 	long long n_override = -1, first = 0, seed_override = -1;
 	int threads = 0;
 	bool compiled = false, timing = false, eval_states = false;
-	std::string raw, prefix, print_path, entropy = "log_sum", order = "seq";
+	std::string raw, prefix, print_path, entropy = "log_sum", order = "seq", grad_in, grad_out;
 	unsigned variant = 0, max_steps = 1u << 30;
 	for (int i = 1; i < argc; i++)
 	{
@@ -1218,6 +1382,10 @@ int main(int argc, char** argv)
 			timing = true;
 		else if (a == "--eval-states")
 			eval_states = true;
+		else if (a == "--grad-in")
+			grad_in = next();
+		else if (a == "--grad-out")
+			grad_out = next();
 		else
 			pos.push_back(a);
 	}
@@ -1288,6 +1456,62 @@ int main(int argc, char** argv)
 			std::printf("%08x %08x %d %llx\n", fbits(total), fbits(S.entropy(r.data())), flip,
 						(unsigned long long)S.obs_index(st.data()));
 		}
+	}
+
+	if (!grad_in.empty())
+	{
+		if (S.discrete)
+			die("the gradient is defined for continuous time");
+		if (S.wave)
+			die("the gradient uses the lane order");
+		Grad g;
+		g.init(S);
+		const size_t cells = (size_t)S.windows << S.nobs;
+		g.G.resize(cells);
+		{
+			std::ifstream f(grad_in, std::ios::binary);
+			if (!f.read((char*)g.G.data(), (std::streamsize)(cells * sizeof(double))))
+				die("cannot read " + std::to_string(cells) + " doubles from " + grad_in);
+		}
+#ifdef _OPENMP
+		if (threads > 0)
+			omp_set_num_threads(threads);
+		const int gth = omp_get_max_threads();
+#else
+		const int gth = 1;
+#endif
+		std::vector<std::vector<double>> accs(gth, std::vector<double>(g.names.size(), 0.0));
+		std::vector<uint64_t> trans(gth, 0);
+#pragma omp parallel
+		{
+#ifdef _OPENMP
+			const int th = omp_get_thread_num();
+#else
+			const int th = 0;
+#endif
+			std::vector<float> r(S.N);
+			std::vector<uint32_t> st(S.W);
+#pragma omp for schedule(dynamic, 64)
+			for (long long i = 0; i < (long long)n; i++)
+				trajectory_grad(S, g, (uint64_t)first + (uint64_t)i, accs[th], r, st, trans[th]);
+		}
+		std::vector<double> tot(g.names.size(), 0.0);
+		uint64_t ntr = 0;
+		for (int th = 0; th < gth; th++)
+		{
+			for (size_t j = 0; j < tot.size(); j++)
+				tot[j] += accs[th][j];
+			ntr += trans[th];
+		}
+		FILE* f = grad_out.empty() ? stdout : std::fopen(grad_out.c_str(), "w");
+		if (!f)
+			die("cannot write " + grad_out);
+		std::fprintf(f, "transitions %llu\n", (unsigned long long)ntr);
+		for (size_t j = 0; j < tot.size(); j++)
+			std::fprintf(f, "%s %.17g\n", g.names[j].c_str(), tot[j] / ((double)n * (double)S.tick));
+		if (f != stdout)
+			std::fclose(f);
+		return 0;
 	}
 
 	std::string so;

@@ -67,7 +67,8 @@ STRUCTS = {
     "mbg_model_info": _capi.ModelInfo, "mbg_compile_opts": _capi.CompileOpts,
     "mbg_sim_params": _capi.SimParams, "mbg_result_info": _capi.ResultInfo,
     "mbg_variant": _capi.Variant, "mbg_batch_params": _capi.BatchParams,
-    "mbg_batch_info": _capi.BatchInfo,
+    "mbg_batch_info": _capi.BatchInfo, "mbg_grad_params": _capi.GradParams,
+    "mbg_grad_info": _capi.GradInfo,
 }
 
 

@@ -114,3 +114,23 @@ def test_flip_selection_brute_force(native_bin, tmp_path):
         assert flip == expect
         assert r32[flip] > 0
     assert int(out[-1].split()[2]) == 6
+
+
+def test_gradient_kernel_compiles_for_gfx950(tmp_path):
+    """The gradient kernel's translation unit (generated model code + the
+    kernel header) compiles for gfx950; models without a gradient form are
+    refused with MBG_ERR_LIMIT."""
+    d = tmp_path / "wl"
+    d.mkdir()
+    bnd, cfg = netgen.write_workload("grad_30", str(d))
+    src = tmp_path / "grad.hip"
+    src.write_text(mg.Model.load(bnd, cfg).generate_code(with_gradient=True))
+    flags = [f for f in HIPRTC_FLAGS if f != "-fgpu-rdc"]
+    p = subprocess.run(["/opt/rocm/bin/hipcc", "-x", "hip", "--cuda-device-only", "-c", *flags,
+                        "-include", "hip/hip_runtime.h", f"-I{DEVICE_INC}", str(src), "-o", str(tmp_path / "grad.o")],
+                       capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr[-3000:]
+    # cellcycle runs in discrete time: no gradient form
+    with pytest.raises(mg.MbgError) as e:
+        mg.Model.load(*model_paths("cellcycle")).generate_code(with_gradient=True)
+    assert e.value.code == 5
