@@ -21,6 +21,15 @@ from ._capi import (CLI_PATH, BatchInfo, BatchParams, CompileOpts, MbgError, Mod
 
 __all__ = ["Model", "Program", "Result", "simulate", "MbgError", "run_cli"]
 
+# Load the engine (and with it the system ROCm's HIP runtime and hipRTC) when
+# the package is imported, so that a framework imported afterwards shares
+# them rather than the other way round.  A missing library is reported by the
+# first call that needs it.
+try:
+    lib()
+except RuntimeError:
+    pass
+
 WINDOW_MODES = {"auto": -1, "dense": 0, "hash": 1, "lds": 2}
 
 

@@ -24,7 +24,9 @@
 // Built once per state-word count MBG_KW (and MBG_PARAM = 0 / 1) into LLVM
 // bitcode; hipRTC links it with the generated mbg_eval / mbg_eval_p /
 // mbg_obs_index of a model.
+#if !defined(__HIPCC_RTC__)
 #include <hip/hip_runtime.h>
+#endif
 
 #include "mbg_kernel_args.h"
 #include "mbg_lane_core.h"

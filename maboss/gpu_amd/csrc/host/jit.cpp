@@ -27,6 +27,7 @@ extern const char mbg_hdr_kernel_args[];
 extern const char mbg_hdr_stats_device[];
 extern const char mbg_hdr_lane_core[];
 extern const char mbg_hdr_grad_kernel[];
+extern const char mbg_src_lane[];
 extern const unsigned char mbg_co_wave[];
 extern const unsigned char mbg_co_util[];
 }
@@ -155,8 +156,8 @@ std::vector<char> compile_to_bitcode(const std::string& source)
 	return bc;
 }
 
-// Whole kernel in one translation unit (gradient form): compile straight to
-// a code object.
+// The whole kernel in one translation unit: compile straight to a code
+// object (gradient kernel; lane kernels when the bitcode link is refused).
 std::vector<char> compile_to_code_object(const std::string& source)
 {
 	const char* headers[] = { mbg_hdr_numeric,		mbg_hdr_abi,	   mbg_hdr_kernel_args,
@@ -326,7 +327,20 @@ hipFunction_t program::function(bool param)
 	if (no_cache || !read_all(path, co))
 	{
 		std::vector<char> bc = compile_to_bitcode(gen.source);
-		co = link_code_object(bc, kbc);
+		try
+		{
+			co = link_code_object(bc, kbc);
+		}
+		catch (const error&)
+		{
+			// The link reads the embedded bitcode with the process's code
+			// object manager; when an older one is loaded first (for example
+			// the copy bundled with a framework), it cannot read that bitcode.
+			// Compiling the kernel source and the model in one translation
+			// unit needs no bitcode from this build.
+			co = compile_to_code_object("#define MBG_KW " + std::to_string(W) + "\n#define MBG_PARAM "
+										+ (param ? "1" : "0") + "\n" + gen.source + "\n" + mbg_src_lane);
+		}
 		if (!no_cache)
 		{
 			mkdirs(dir);

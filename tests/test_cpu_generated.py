@@ -134,3 +134,26 @@ def test_gradient_kernel_compiles_for_gfx950(tmp_path):
     with pytest.raises(mg.MbgError) as e:
         mg.Model.load(*model_paths("cellcycle")).generate_code(with_gradient=True)
     assert e.value.code == 5
+
+
+def test_single_unit_kernels_compile_with_hiprtc(native_bin, tmp_path):
+    """The engine's hipRTC-only compile paths: lane kernels as one translation
+    unit (used when the process's code-object manager cannot read the
+    embedded bitcode) and the gradient kernel, compiled by hipRTC itself."""
+    exe = native_bin("rtc_compile", extra=["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-L/opt/rocm/lib",
+                                           "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"])
+    d = tmp_path / "wl"
+    d.mkdir()
+    bnd, cfg = netgen.write_workload("grad_30", str(d))
+    m = mg.Model.load(bnd, cfg)
+    lane = open(os.path.join(DEVICE_INC, "simulate_lane.hip")).read()
+    headers = [os.path.join(DEVICE_INC, h) for h in ("mbg_numeric.h", "mbg_abi.h", "mbg_kernel_args.h",
+                                                     "mbg_stats_device.h", "mbg_lane_core.h", "mbg_grad_kernel.h")]
+    units = {f"lane_p{p}": f"#define MBG_KW {m.info.n_words}\n#define MBG_PARAM {p}\n" + m.generate_code() + "\n" + lane
+             for p in (0, 1)}
+    units["grad"] = m.generate_code(with_gradient=True)
+    for name, text in units.items():
+        src = tmp_path / f"{name}.hip"
+        src.write_text(text)
+        p = subprocess.run([exe, str(src), *headers], capture_output=True, text=True, timeout=600)
+        assert p.returncode == 0 and p.stdout.startswith("OK"), (name, p.stderr[-3000:])

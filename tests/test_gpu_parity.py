@@ -235,3 +235,18 @@ def test_full_size_plain_100_properties(tmp_path, wl):
         lo = np.float32(w) * tick
         hi = min(np.float32(w + 1) * tick, np.float32(m.info.max_time))
         assert abs(tot / (n * TWO32) - float(hi - lo) / float(tick)) < 2e-6
+
+
+def test_framework_hip_loaded_first(tmp_path):
+    """With torch (and its bundled HIP runtime / code-object manager) loaded
+    before the engine, programs still compile and stay bit-exact."""
+    import sys
+    bnd, cfg = model_paths("cellcycle")
+    raw = tmp_path / "g.raw"
+    code = ("import torch; torch.cuda.init(); import maboss.gpu_amd as mg; "
+            f"r = mg.Model.load({bnd!r}, {cfg!r}).compile(no_cache=True).simulate(n_trajectories=3000); "
+            f"r.write_raw({str(raw)!r}); print('ok')")
+    p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert p.returncode == 0 and "ok" in p.stdout, p.stderr[-3000:]
+    assert_same(raw, oracle_raw(tmp_path, bnd, cfg, n=3000))
