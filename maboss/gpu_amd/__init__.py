@@ -21,14 +21,22 @@ from ._capi import (CLI_PATH, BatchInfo, BatchParams, CompileOpts, MbgError, Mod
 
 __all__ = ["Model", "Program", "Result", "simulate", "MbgError", "run_cli"]
 
-# Load the engine (and with it the system ROCm's HIP runtime and hipRTC) when
-# the package is imported, so that a framework imported afterwards shares
-# them rather than the other way round.  A missing library is reported by the
-# first call that needs it.
+# Load the engine when the package is imported (after torch, see
+# _capi.lib), so the process's HIP runtime is settled before any other
+# import can pick one.  A missing library is reported by the first call that
+# needs it.
 try:
     lib()
 except RuntimeError:
     pass
+
+
+def _release(fn_name, h):
+    """Free a handle; quiet during interpreter shutdown."""
+    try:
+        getattr(lib(), fn_name)(h)
+    except Exception:
+        pass
 
 WINDOW_MODES = {"auto": -1, "dense": 0, "hash": 1, "lds": 2}
 
@@ -54,7 +62,7 @@ class Model:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().mbg_model_free(h)
+            _release("mbg_model_free", h)
             self._h = None
 
     @property
@@ -123,7 +131,7 @@ class Program:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().mbg_program_free(h)
+            _release("mbg_program_free", h)
             self._h = None
 
     @property
@@ -222,7 +230,7 @@ class Result:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().mbg_result_free(h)
+            _release("mbg_result_free", h)
             self._h = None
 
     @property

@@ -193,6 +193,21 @@ def _declare(lib):
     _ = (i64, u64)
 
 
+def _framework_hip_first():
+    """Load torch (when installed) before the engine.
+
+    torch bundles its own HIP runtime and code-object manager under the same
+    sonames as the system ROCm the engine is built against.  Whichever copy
+    a process loads first is the one every later library binds to, and torch
+    does not see the GPU through the system copy, so torch goes first; the
+    engine works with either (its JIT compiles from source when the loaded
+    code-object manager cannot read its embedded bitcode)."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """The loaded C-ABI library; raises if it has not been built."""
     global _lib
@@ -202,6 +217,7 @@ def lib():
                 raise RuntimeError(
                     f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                     "or `make -C maboss/gpu_amd/csrc`")
+            _framework_hip_first()
             handle = ctypes.CDLL(LIB_PATH)
             _declare(handle)
             _lib = handle

@@ -78,6 +78,11 @@ struct mbg_sim_args
 	// (first full window, observed state) here and expanded after the run.
 	unsigned long long* tail_table; // hash slots {key, count}; null: per-window adds
 	unsigned long long tail_mask;
+	// Block-private LDS caches in front of the tail and fixed-point tables:
+	// trajectories of a block that end in the same state (attractors) are
+	// counted in LDS and reach the global table once per block.
+	int tail_lds_off, tail_lds_slots; // u64 offset, slots {key + 1, count} (power of two; 0: none)
+	int fp_lds_off, fp_lds_slots;     // u64 offset, slots of (2 + key words) u32 (power of two; 0: none)
 	unsigned long long* counters;   // [MBG_NCOUNTERS]
 	const mbg_param_view* pv;       // per-variant tables (batched/gradient); null for plain
 	mbg_wave_tables wt;             // wave-per-trajectory kernel only

@@ -51,6 +51,127 @@ __device__ __forceinline__ void hash_add(unsigned long long* table, uint64_t mas
 	atomicAdd(overflow, 1ull);
 }
 
+// Block-private count cache in LDS: slots {key + 1, count} (0 = empty),
+// nslots a power of two, a few probes.  False when no slot was found (the
+// caller then updates the global table directly).
+__device__ __forceinline__ bool lds_count_add(unsigned long long* slots, int nslots, uint64_t key, uint64_t q)
+{
+	const unsigned long long k1 = (unsigned long long)key + 1ull;
+	uint32_t s = (uint32_t)mix64(key) & (uint32_t)(nslots - 1);
+	for (int p = 0; p < 8; ++p)
+	{
+		unsigned long long* e = slots + 2 * s;
+		const unsigned long long old = atomicCAS(e, 0ull, k1);
+		if (old == 0ull || old == k1)
+		{
+			atomicAdd(e + 1, (unsigned long long)q);
+			return true;
+		}
+		s = (s + 1u) & (uint32_t)(nslots - 1);
+	}
+	return false;
+}
+
+// Fixed-point table insert of cnt trajectories.  Slots hold {tag, count, key
+// words}; the key is the state (plus the variant in the batched form).
+// Callers let one lane of a wave insert at a time, so the only wait (for a
+// slot another wave is publishing) is on a wave that has claimed the slot
+// and only has stores left; the wait is bounded.
+template <int KW>
+__device__ void fp_insert(unsigned int* table, uint64_t mask, const uint32_t (&key)[KW], unsigned int cnt,
+						  unsigned long long* counters)
+{
+	constexpr int stride = 2 + KW;
+	uint64_t h = 0x9e3779b97f4a7c15ull;
+#pragma unroll
+	for (int w = 0; w < KW; ++w)
+		h = mix64(h ^ key[w]);
+	uint64_t slot = h & mask;
+	for (uint64_t p = 0; p <= mask; ++p)
+	{
+		unsigned int* e = table + slot * stride;
+		unsigned int tag = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+		if (tag == 0u)
+		{
+			tag = atomicCAS(e, 0u, 1u);
+			if (tag == 0u)
+			{
+#pragma unroll
+				for (int w = 0; w < KW; ++w)
+					e[2 + w] = key[w];
+				__hip_atomic_store(e, 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+				atomicAdd(e + 1, cnt);
+				return;
+			}
+		}
+		for (int spin = 0; tag == 1u && spin < (1 << 22); ++spin)
+		{
+			__builtin_amdgcn_s_sleep(2);
+			tag = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+		}
+		if (tag != 2u)
+		{
+			atomicAdd(counters + MBG_CTR_FP_TIMEOUT, 1ull);
+			return;
+		}
+		bool same = true;
+#pragma unroll
+		for (int w = 0; w < KW; ++w)
+			same &= __hip_atomic_load(e + 2 + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == key[w];
+		if (same)
+		{
+			atomicAdd(e + 1, cnt);
+			return;
+		}
+		slot = (slot + 1) & mask;
+	}
+	atomicAdd(counters + MBG_CTR_FP_OVERFLOW, 1ull);
+}
+
+// Block-private fixed-point cache in LDS, same slot layout as the global
+// table; one lane of a wave at a time (as fp_insert).  False when the probes
+// find no slot.
+template <int KW>
+__device__ bool fp_lds_insert(unsigned int* t, int nslots, const uint32_t (&key)[KW])
+{
+	constexpr int stride = 2 + KW;
+	uint64_t h = 0x9e3779b97f4a7c15ull;
+#pragma unroll
+	for (int w = 0; w < KW; ++w)
+		h = mix64(h ^ key[w]);
+	uint32_t slot = (uint32_t)h & (uint32_t)(nslots - 1);
+	for (int p = 0; p < 16; ++p)
+	{
+		unsigned int* e = t + slot * stride;
+		unsigned int tag = atomicCAS(e, 0u, 1u);
+		if (tag == 0u)
+		{
+#pragma unroll
+			for (int w = 0; w < KW; ++w)
+				e[2 + w] = key[w];
+			__threadfence_block();
+			atomicExch(e, 2u);
+			atomicAdd(e + 1, 1u);
+			return true;
+		}
+		for (int spin = 0; tag == 1u && spin < (1 << 16); ++spin)
+			tag = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+		if (tag != 2u)
+			return false;
+		bool same = true;
+#pragma unroll
+		for (int w = 0; w < KW; ++w)
+			same &= __hip_atomic_load(e + 2 + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == key[w];
+		if (same)
+		{
+			atomicAdd(e + 1, 1u);
+			return true;
+		}
+		slot = (slot + 1u) & (uint32_t)(nslots - 1);
+	}
+	return false;
+}
+
 struct lane_hist
 {
 	unsigned long long* lds;
@@ -78,7 +199,19 @@ struct lane_hist
 	// One more trajectory whose final record covers windows w0.. in state obs.
 	__device__ void tail(int w0, uint64_t obs) const
 	{
-		hash_add(a.tail_table, a.tail_mask, ((uint64_t)w0 << a.n_obs) | obs, 1ull, a.counters + MBG_CTR_TAIL_OVERFLOW);
+		const uint64_t key = ((uint64_t)w0 << a.n_obs) | obs;
+		if (a.tail_lds_slots && lds_count_add(lds + a.tail_lds_off, a.tail_lds_slots, key, 1ull))
+			return;
+		hash_add(a.tail_table, a.tail_mask, key, 1ull, a.counters + MBG_CTR_TAIL_OVERFLOW);
+	}
+
+	// One more trajectory ending in fixed point key (one lane of a wave at a time).
+	template <int KW>
+	__device__ void fixed(const uint32_t (&key)[KW]) const
+	{
+		if (a.fp_lds_slots && fp_lds_insert<KW>((unsigned int*)(lds + a.fp_lds_off), a.fp_lds_slots, key))
+			return;
+		fp_insert<KW>(a.fp_table, a.fp_mask, key, 1u, a.counters);
 	}
 
 	__device__ void fin(uint64_t obs) const
@@ -91,6 +224,73 @@ struct lane_hist
 			hash_add(a.fin_table, a.fin_mask, obs, 1ull, a.counters + MBG_CTR_FIN_OVERFLOW);
 	}
 };
+
+// End of a block: add its LDS histograms and caches to the global storage
+// (after a barrier).
+__device__ __forceinline__ void flush_block_hist(unsigned long long* lds, const mbg_sim_args& a)
+{
+	if (!a.lds_bytes)
+		return;
+	const uint64_t nwin = a.win_mode == MBG_HIST_LDS ? ((uint64_t)a.windows << a.n_obs) : 0;
+	for (uint64_t i = threadIdx.x; i < nwin; i += blockDim.x)
+	{
+		const unsigned long long v = lds[a.win_lds_off + i];
+		if (v)
+			atomicAdd(a.win_vals + i, v);
+	}
+	if (a.ent_lds)
+		for (int i = threadIdx.x; i < a.windows; i += blockDim.x)
+		{
+			const unsigned long long v = lds[a.ent_lds_off + i];
+			if (v)
+				atomicAdd(a.win_ent + i, v);
+		}
+	const uint64_t nfin = a.fin_mode == MBG_HIST_LDS ? (1ull << a.n_obs) : 0;
+	for (uint64_t i = threadIdx.x; i < nfin; i += blockDim.x)
+	{
+		const unsigned long long v = lds[a.fin_lds_off + i];
+		if (v)
+			atomicAdd(a.fin_vals + i, v);
+	}
+	for (int i = threadIdx.x; i < a.tail_lds_slots; i += blockDim.x)
+	{
+		const unsigned long long k1 = lds[a.tail_lds_off + 2 * i];
+		if (k1)
+			hash_add(a.tail_table, a.tail_mask, k1 - 1ull, lds[a.tail_lds_off + 2 * i + 1],
+					 a.counters + MBG_CTR_TAIL_OVERFLOW);
+	}
+}
+
+// The block's cached fixed points to the global table: one slot per wave
+// iteration, one lane inserting at a time.
+template <int KW>
+__device__ void flush_block_fp(unsigned long long* lds, const mbg_sim_args& a)
+{
+	if (!a.fp_lds_slots)
+		return;
+	const unsigned int* t = (const unsigned int*)(lds + a.fp_lds_off);
+	const int lane = __lane_id(), wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+	for (int base = wave * 64; base < a.fp_lds_slots; base += nwaves * 64)
+	{
+		const int i = base + lane;
+		const unsigned int* e = t + (size_t)i * (2 + KW);
+		const bool full = i < a.fp_lds_slots && e[0] == 2u && e[1] != 0u;
+		uint64_t m = __ballot(full);
+		while (m)
+		{
+			const int l = __ffsll((unsigned long long)m) - 1;
+			if (lane == l)
+			{
+				uint32_t key[KW];
+#pragma unroll
+				for (int w = 0; w < KW; ++w)
+					key[w] = e[2 + w];
+				fp_insert<KW>(a.fp_table, a.fp_mask, key, e[1], a.counters);
+			}
+			m &= m - 1ull;
+		}
+	}
+}
 
 __device__ __forceinline__ uint32_t pick(const mbg_u32x4& r, uint32_t i)
 {

@@ -47,61 +47,6 @@ namespace
 
 constexpr int W = MBG_KW;
 
-// Fixed-point table insert.  Slots hold {tag, count, key words}; the key is
-// the state (plus the variant in the batched form).  Called by one lane of a
-// wave at a time, so the only wait (for a slot another wave is publishing)
-// is on a wave that has already claimed the slot and only has stores left;
-// the wait is bounded.
-template <int KW>
-__device__ void fp_insert(unsigned int* table, uint64_t mask, const uint32_t (&key)[KW], unsigned long long* counters)
-{
-	constexpr int stride = 2 + KW;
-	uint64_t h = 0x9e3779b97f4a7c15ull;
-#pragma unroll
-	for (int w = 0; w < KW; ++w)
-		h = mix64(h ^ key[w]);
-	uint64_t slot = h & mask;
-	for (uint64_t p = 0; p <= mask; ++p)
-	{
-		unsigned int* e = table + slot * stride;
-		unsigned int tag = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-		if (tag == 0u)
-		{
-			tag = atomicCAS(e, 0u, 1u);
-			if (tag == 0u)
-			{
-#pragma unroll
-				for (int w = 0; w < KW; ++w)
-					e[2 + w] = key[w];
-				__hip_atomic_store(e, 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-				atomicAdd(e + 1, 1u);
-				return;
-			}
-		}
-		for (int spin = 0; tag == 1u && spin < (1 << 22); ++spin)
-		{
-			__builtin_amdgcn_s_sleep(2);
-			tag = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-		}
-		if (tag != 2u)
-		{
-			atomicAdd(counters + MBG_CTR_FP_TIMEOUT, 1ull);
-			return;
-		}
-		bool same = true;
-#pragma unroll
-		for (int w = 0; w < KW; ++w)
-			same &= __hip_atomic_load(e + 2 + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == key[w];
-		if (same)
-		{
-			atomicAdd(e + 1, 1u);
-			return;
-		}
-		slot = (slot + 1) & mask;
-	}
-	atomicAdd(counters + MBG_CTR_FP_OVERFLOW, 1ull);
-}
-
 // Wave-reduced counter update: transitions, finished and truncated
 // trajectories go to their three destinations.
 __device__ __forceinline__ void wave_add_counters(unsigned long long* d_trans, unsigned long long* d_fin,
@@ -226,7 +171,7 @@ extern "C" __global__ void __launch_bounds__(256) simulate(mbg_sim_args a)
 			{
 				const int l = __ffsll((unsigned long long)fm) - 1;
 				if (lane == l)
-					fp_insert<W>(a.fp_table, a.fp_mask, s, a.counters);
+					hist.fixed<W>(s);
 				fm &= fm - 1ull;
 			}
 			if (ended)
@@ -238,32 +183,10 @@ extern "C" __global__ void __launch_bounds__(256) simulate(mbg_sim_args a)
 	wave_add_counters(a.counters + MBG_CTR_TRANSITIONS, a.counters + MBG_CTR_FINISHED, a.counters + MBG_CTR_TRUNCATED,
 					  ntrans, nfinished, ntrunc);
 
-	// Flush the block's LDS histograms.
+	// Flush the block's LDS histograms and caches.
 	__syncthreads();
-	if (a.lds_bytes)
-	{
-		const uint64_t nwin = a.win_mode == MBG_HIST_LDS ? ((uint64_t)a.windows << a.n_obs) : 0;
-		for (uint64_t i = threadIdx.x; i < nwin; i += blockDim.x)
-		{
-			const unsigned long long v = lds[a.win_lds_off + i];
-			if (v)
-				atomicAdd(a.win_vals + i, v);
-		}
-		if (a.ent_lds)
-			for (int i = threadIdx.x; i < a.windows; i += blockDim.x)
-			{
-				const unsigned long long v = lds[a.ent_lds_off + i];
-				if (v)
-					atomicAdd(a.win_ent + i, v);
-			}
-		const uint64_t nfin = a.fin_mode == MBG_HIST_LDS ? (1ull << a.n_obs) : 0;
-		for (uint64_t i = threadIdx.x; i < nfin; i += blockDim.x)
-		{
-			const unsigned long long v = lds[a.fin_lds_off + i];
-			if (v)
-				atomicAdd(a.fin_vals + i, v);
-		}
-	}
+	flush_block_hist(lds, a);
+	flush_block_fp<W>(lds, a);
 }
 
 #else // MBG_PARAM
@@ -379,7 +302,7 @@ extern "C" __global__ void __launch_bounds__(256) simulate_batched(mbg_sim_args 
 						for (int w = 0; w < W; ++w)
 							key[w] = s[w];
 						key[W] = v;
-						fp_insert<W + 1>(a.fp_table, a.fp_mask, key, a.counters);
+						fp_insert<W + 1>(a.fp_table, a.fp_mask, key, 1u, a.counters);
 					}
 					fm &= fm - 1ull;
 				}

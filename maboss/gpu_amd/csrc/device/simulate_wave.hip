@@ -389,30 +389,7 @@ __device__ void wave_main(const mbg_sim_args& a)
 	}
 
 	__syncthreads();
-	if (a.lds_bytes)
-	{
-		const uint64_t nwin = a.win_mode == MBG_HIST_LDS ? ((uint64_t)a.windows << a.n_obs) : 0;
-		for (uint64_t i = threadIdx.x; i < nwin; i += blockDim.x)
-		{
-			const unsigned long long v = lds[a.win_lds_off + i];
-			if (v)
-				atomicAdd(a.win_vals + i, v);
-		}
-		if (a.ent_lds)
-			for (int i = threadIdx.x; i < a.windows; i += blockDim.x)
-			{
-				const unsigned long long v = lds[a.ent_lds_off + i];
-				if (v)
-					atomicAdd(a.win_ent + i, v);
-			}
-		const uint64_t nfin = a.fin_mode == MBG_HIST_LDS ? (1ull << a.n_obs) : 0;
-		for (uint64_t i = threadIdx.x; i < nfin; i += blockDim.x)
-		{
-			const unsigned long long v = lds[a.fin_lds_off + i];
-			if (v)
-				atomicAdd(a.fin_vals + i, v);
-		}
-	}
+	flush_block_hist(lds, a);
 }
 
 } // namespace

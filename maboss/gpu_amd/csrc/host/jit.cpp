@@ -9,6 +9,8 @@
 // hash of the generated source, the compile options and the kernel bitcode.
 #include <hip/hiprtc.h>
 
+#include <atomic>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -326,13 +328,17 @@ hipFunction_t program::function(bool param)
 	std::vector<char> co;
 	if (no_cache || !read_all(path, co))
 	{
-		std::vector<char> bc = compile_to_bitcode(gen.source);
+		static std::atomic<bool> link_refused { false };
 		try
 		{
+			if (link_refused.load())
+				throw error(MBG_E_COMPILE, "bitcode link refused earlier in this process");
+			std::vector<char> bc = compile_to_bitcode(gen.source);
 			co = link_code_object(bc, kbc);
 		}
 		catch (const error&)
 		{
+			link_refused.store(true);
 			// The link reads the embedded bitcode with the process's code
 			// object manager; when an older one is loaded first (for example
 			// the copy bundled with a framework), it cannot read that bitcode.

@@ -76,10 +76,21 @@ struct plan
 {
 	int win_mode, fin_mode, ent_lds;
 	int win_off = 0, fin_off = 0, ent_off = 0, lds_bytes = 0;
+	int tail_off = 0, tail_slots = 0, fp_off = 0, fp_slots = 0; // LDS caches
 	uint64_t win_cells = 0, fin_cells = 0, win_cap = 0, fin_cap = 0, fp_cap = 0;
 };
 
-plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int windows, int n_obs)
+// Largest power of two <= limit of slots of the given size that fits the
+// remaining LDS budget, or 0 below min_slots.
+int cache_slots(size_t budget_left, size_t slot_bytes, int limit, int min_slots)
+{
+	int n = limit;
+	while (n >= min_slots && (size_t)n * slot_bytes > budget_left)
+		n >>= 1;
+	return n >= min_slots ? n : 0;
+}
+
+plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int windows, int n_obs, bool wave_kernel)
 {
 	plan p;
 	const uint64_t win_dense_bytes = n_obs <= 40 ? ((uint64_t)windows << n_obs) * 8 : ~0ull;
@@ -121,6 +132,20 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 			throw error(MBG_E_LIMIT, "final-state histogram does not fit in LDS");
 		p.fin_off = (int)(lds / 8);
 		lds += fin_dense_bytes;
+	}
+	// block-private caches in front of the tail and fixed-point tables
+	if (!m.discrete_time() && p.win_mode != MBG_HIST_LDS)
+	{
+		p.tail_slots = cache_slots(k_lds_budget - lds, 16, 512, 64);
+		p.tail_off = (int)(lds / 8);
+		lds += (size_t)p.tail_slots * 16;
+	}
+	if (!wave_kernel)
+	{
+		const size_t slot = 4 * (2 + (size_t)m.n_words());
+		p.fp_slots = cache_slots(k_lds_budget - lds, slot, 128, 16);
+		p.fp_off = (int)(lds / 8);
+		lds += ((size_t)p.fp_slots * slot + 7) & ~(size_t)7;
 	}
 	p.lds_bytes = (int)lds;
 
@@ -184,7 +209,7 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		throw error(MBG_E_ARG, "the wave kernel computes the log-sum entropy only");
 	hipFunction_t fn = use_wave ? p.wave_function() : p.function(false);
 	res->kernel = use_wave ? 1 : 0;
-	plan pl = make_plan(m, sp, n_traj, windows, n_obs);
+	plan pl = make_plan(m, sp, n_traj, windows, n_obs, use_wave);
 	// tail records for window storage in HBM (continuous time)
 	const bool tails = !m.discrete_time() && pl.win_mode != MBG_HIST_LDS;
 	uint64_t tail_cap = tails ? next_pow2(std::min<uint64_t>(std::max<uint64_t>(n_traj / 4, 1 << 12), 1 << 20)) : 0;
@@ -272,6 +297,10 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		a.pv = nullptr;
 		a.tail_table = tail_tab;
 		a.tail_mask = tail_cap ? tail_cap - 1 : 0;
+		a.tail_lds_off = pl.tail_off;
+		a.tail_lds_slots = tails ? pl.tail_slots : 0;
+		a.fp_lds_off = pl.fp_off;
+		a.fp_lds_slots = pl.fp_slots;
 
 		// ---- launch: a persistent grid filling every CU
 		const int block = sp.block_size > 0 ? sp.block_size : 256;
