@@ -113,9 +113,9 @@ typedef struct mbg_sim_params
 	int64_t seed;            /* <0: seed_pseudorandom of the model */
 	uint32_t variant;        /* stream tag (batched variant index) */
 	uint32_t max_steps;      /* 0: default cap on SSA steps per trajectory */
-	int32_t window_mode;     /* -1 auto, 0 dense HBM, 1 hash, 2 dense LDS */
+	int32_t window_mode;     /* -1 auto, 0 dense HBM, 1 hash, 2 dense LDS, 3 sorted pairs */
 	int32_t final_mode;      /* -1 auto, 0 dense HBM, 1 hash, 2 dense LDS */
-	uint64_t window_capacity;/* hash slots (power of two), 0 = auto */
+	uint64_t window_capacity;/* hash slots (power of two) or sorted-mode pairs, 0 = auto */
 	uint64_t final_capacity;
 	uint64_t fixed_capacity;
 	int32_t block_size;      /* 0 = 256 */

@@ -38,7 +38,7 @@ def _release(fn_name, h):
     except Exception:
         pass
 
-WINDOW_MODES = {"auto": -1, "dense": 0, "hash": 1, "lds": 2}
+WINDOW_MODES = {"auto": -1, "dense": 0, "hash": 1, "lds": 2, "sort": 3}
 
 
 class Model:

@@ -8,6 +8,7 @@
 #define MBG_HIST_DENSE 0  // cell (w << n_obs) | obs, in HBM
 #define MBG_HIST_HASH 1   // open-addressing table keyed by (w << n_obs) | obs
 #define MBG_HIST_LDS 2    // dense, privatised in LDS and flushed once per block
+#define MBG_HIST_SORT 3   // (key, value) pairs appended, then sorted and reduced by key
 
 // Counter slots (u64) written by the kernels.
 #define MBG_CTR_TRANSITIONS 0    // SSA iterations = records of the reference
@@ -19,6 +20,7 @@
 #define MBG_CTR_QUEUE 6          // trajectory work queue
 #define MBG_CTR_FINISHED 7       // trajectories ended at max_time or a fixed point
 #define MBG_CTR_TAIL_OVERFLOW 8  // tail table full
+#define MBG_CTR_PAIRS 9          // window pairs appended (MBG_HIST_SORT; may exceed the capacity)
 #define MBG_NCOUNTERS 10
 
 #define MBG_EMPTY_KEY 0xffffffffffffffffull
@@ -84,6 +86,9 @@ struct mbg_sim_args
 	int tail_lds_off, tail_lds_slots; // u64 offset, slots {key + 1, count} (power of two; 0: none)
 	int fp_lds_off, fp_lds_slots;     // u64 offset, slots of (2 + key words) u32 (power of two; 0: none)
 	unsigned long long* counters;   // [MBG_NCOUNTERS]
+	unsigned long long* pair_keys;  // MBG_HIST_SORT: appended cell keys
+	unsigned long long* pair_vals;  //                and their values
+	unsigned long long pair_cap;    //                capacity in pairs
 	const mbg_param_view* pv;       // per-variant tables (batched/gradient); null for plain
 	mbg_wave_tables wt;             // wave-per-trajectory kernel only
 	int wave_lds_off;               // byte offset of the per-wave rate arrays in LDS

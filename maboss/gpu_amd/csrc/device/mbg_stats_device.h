@@ -51,6 +51,27 @@ __device__ __forceinline__ void hash_add(unsigned long long* table, uint64_t mas
 	atomicAdd(overflow, 1ull);
 }
 
+// Append a (key, value) pair; the active lanes of the wave take consecutive
+// slots with one atomic.  Pairs beyond the capacity are counted, not stored
+// (the host then grows the buffers and runs again).
+__device__ __forceinline__ void append_pair(unsigned long long* keys, unsigned long long* vals, uint64_t cap,
+											unsigned long long* counter, uint64_t key, uint64_t q)
+{
+	const uint64_t m = __ballot(1);
+	const int leader = __ffsll((unsigned long long)m) - 1;
+	const int lane = __lane_id();
+	uint64_t base = 0;
+	if (lane == leader)
+		base = atomicAdd(counter, (unsigned long long)__popcll(m));
+	base = bcast64(base, leader);
+	const uint64_t idx = base + __popcll(m & ((1ull << lane) - 1ull));
+	if (idx < cap)
+	{
+		keys[idx] = key;
+		vals[idx] = q;
+	}
+}
+
 // Block-private count cache in LDS: slots {key + 1, count} (0 = empty),
 // nslots a power of two, a few probes.  False when no slot was found (the
 // caller then updates the global table directly).
@@ -184,6 +205,8 @@ struct lane_hist
 			atomicAdd(lds + a.win_lds_off + cell, (unsigned long long)q);
 		else if (a.win_mode == MBG_HIST_DENSE)
 			atomicAdd(a.win_vals + cell, (unsigned long long)q);
+		else if (a.win_mode == MBG_HIST_SORT)
+			append_pair(a.pair_keys, a.pair_vals, a.pair_cap, a.counters + MBG_CTR_PAIRS, cell, q);
 		else
 			hash_add(a.win_table, a.win_mask, cell, q, a.counters + MBG_CTR_WIN_OVERFLOW);
 	}

@@ -66,6 +66,8 @@ extern "C" __global__ void __launch_bounds__(256) mbg_expand_tails(mbg_sim_args 
 			const uint64_t cell = ((uint64_t)w << a.n_obs) | obs;
 			if (a.win_mode == MBG_HIST_DENSE)
 				atomicAdd(a.win_vals + cell, (unsigned long long)(q * cnt));
+			else if (a.win_mode == MBG_HIST_SORT)
+				append_pair(a.pair_keys, a.pair_vals, a.pair_cap, a.counters + MBG_CTR_PAIRS, cell, q * cnt);
 			else
 				hash_add(a.win_table, a.win_mask, cell, q * cnt, a.counters + MBG_CTR_WIN_OVERFLOW);
 		}

@@ -49,6 +49,9 @@ struct program
 {
 	// run workspace (kept between runs so repeated launches do not reallocate)
 	device_buffer b_counters, b_win_vals, b_win_keys, b_ent, b_fin_vals, b_fin_keys, b_fp, b_istate, b_pv, b_wave;
+	// sorted window mode: appended pairs, their sort scratch, unique count
+	device_buffer b_pk, b_pv2, b_pk_alt, b_pv_alt, b_sort_tmp, b_ucount;
+	uint64_t pairs_per_traj = 0; // learned from the last sorted run (capacity hint)
 
 	std::shared_ptr<const model> mdl;
 	generated gen;
@@ -186,6 +189,16 @@ struct batch_timing
 std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_params& bp,
 													  const std::vector<variant_spec>& vs, hipStream_t stream,
 													  batch_timing* timing);
+
+// Sorted window mode: radix-sort n (key, value) pairs on their low key_bits
+// bits and sum equal keys.  keys/vals hold the input, keys_alt/vals_alt are
+// scratch of the same size; *out_keys / *out_vals point at the unique keys
+// and sums (one of the two buffer pairs), their count goes to the device word
+// unique_count.  Stream ordered.
+void sort_reduce_pairs(device_buffer& temp, unsigned long long* keys, unsigned long long* vals,
+					   unsigned long long* keys_alt, unsigned long long* vals_alt, uint64_t n, int key_bits,
+					   unsigned long long* unique_count, hipStream_t stream, unsigned long long** out_keys,
+					   unsigned long long** out_vals);
 
 // ---- gradient form
 struct grad_params

@@ -108,9 +108,11 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 	if (p.win_mode < 0)
 		p.win_mode = win_dense_bytes <= 32 * 1024 ? MBG_HIST_LDS
 					 : win_dense_bytes <= k_dense_limit ? MBG_HIST_DENSE
-														: MBG_HIST_HASH;
-	if (p.win_mode != MBG_HIST_HASH && win_dense_bytes > k_dense_limit)
-		throw error(MBG_E_LIMIT, "dense window histogram too large; use the hashed mode");
+														: MBG_HIST_SORT;
+	if (p.win_mode < 0 || p.win_mode > MBG_HIST_SORT)
+		throw error(MBG_E_ARG, "unknown window mode");
+	if ((p.win_mode == MBG_HIST_DENSE || p.win_mode == MBG_HIST_LDS) && win_dense_bytes > k_dense_limit)
+		throw error(MBG_E_LIMIT, "dense window histogram too large; use the hashed or sorted mode");
 	if (p.win_mode == MBG_HIST_LDS)
 	{
 		if (win_dense_bytes > k_lds_budget - lds)
@@ -120,6 +122,8 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 	}
 
 	p.fin_mode = sp.final_mode;
+	if (p.fin_mode > MBG_HIST_LDS)
+		throw error(MBG_E_ARG, "unknown final-state mode");
 	if (p.fin_mode < 0)
 		p.fin_mode = fin_dense_bytes <= k_lds_budget - lds ? MBG_HIST_LDS
 					 : fin_dense_bytes <= (1ull << 30)	 ? MBG_HIST_DENSE
@@ -149,15 +153,14 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 	}
 	p.lds_bytes = (int)lds;
 
-	if (p.win_mode == MBG_HIST_HASH)
-	{
+	if (p.win_mode == MBG_HIST_HASH || p.win_mode == MBG_HIST_SORT)
 		if (n_obs + bits_for((uint64_t)windows) > 63)
-			throw error(MBG_E_LIMIT, "observed nodes plus window bits exceed the 63-bit hash key");
+			throw error(MBG_E_LIMIT, "observed nodes plus window bits exceed the 63-bit cell key");
+	if (p.win_mode == MBG_HIST_HASH)
 		p.win_cap = sp.window_capacity ? next_pow2(sp.window_capacity)
 									   : next_pow2(std::min<uint64_t>(std::max<uint64_t>(n_traj * windows, 1 << 16),
 																	   1ull << 28));
-	}
-	else
+	else if (p.win_mode != MBG_HIST_SORT) // the sorted mode's pair capacity is the runner's
 		p.win_cells = (uint64_t)windows << n_obs;
 	if (p.fin_mode == MBG_HIST_HASH)
 	{
@@ -213,6 +216,14 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 	// tail records for window storage in HBM (continuous time)
 	const bool tails = !m.discrete_time() && pl.win_mode != MBG_HIST_LDS;
 	uint64_t tail_cap = tails ? next_pow2(std::min<uint64_t>(std::max<uint64_t>(n_traj / 4, 1 << 12), 1 << 20)) : 0;
+	// sorted window mode: pair capacity from the caller, the last run, or windows per trajectory
+	uint64_t pair_cap = 0;
+	if (pl.win_mode == MBG_HIST_SORT)
+	{
+		const uint64_t per = p.pairs_per_traj ? p.pairs_per_traj + p.pairs_per_traj / 8 + 1
+											  : std::max<uint64_t>((uint64_t)windows, 16);
+		pair_cap = sp.window_capacity ? sp.window_capacity : std::max<uint64_t>(n_traj * per, 1 << 16);
+	}
 	p.load_util();
 
 	for (int attempt = 0;; attempt++)
@@ -223,7 +234,13 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		auto* ent = (unsigned long long*)p.b_ent.ensure((size_t)windows * 8);
 		hip_check(hipMemsetAsync(ent, 0, (size_t)windows * 8, stream), "hipMemsetAsync");
 		unsigned long long *win_vals = nullptr, *win_keys = nullptr, *fin_vals = nullptr, *fin_keys = nullptr;
-		if (pl.win_mode == MBG_HIST_HASH)
+		unsigned long long *pk = nullptr, *pv = nullptr;
+		if (pl.win_mode == MBG_HIST_SORT)
+		{
+			pk = (unsigned long long*)p.b_pk.ensure(pair_cap * 8);
+			pv = (unsigned long long*)p.b_pv2.ensure(pair_cap * 8);
+		}
+		else if (pl.win_mode == MBG_HIST_HASH)
 		{
 			win_keys = (unsigned long long*)p.b_win_keys.ensure(pl.win_cap * 16);
 			hip_fill_slots(p, win_keys, pl.win_cap, stream);
@@ -301,6 +318,9 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		a.tail_lds_slots = tails ? pl.tail_slots : 0;
 		a.fp_lds_off = pl.fp_off;
 		a.fp_lds_slots = pl.fp_slots;
+		a.pair_keys = pk;
+		a.pair_vals = pv;
+		a.pair_cap = pair_cap;
 
 		// ---- launch: a persistent grid filling every CU
 		const int block = sp.block_size > 0 ? sp.block_size : 256;
@@ -359,10 +379,32 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 						  "hipModuleLaunchKernel(mbg_expand_tails)");
 			}
 		}
+		std::vector<unsigned long long> ctr(MBG_NCOUNTERS);
+		unsigned long long *uk = nullptr, *uv = nullptr, *ucount = nullptr;
+		bool pair_over = false;
+		if (pl.win_mode == MBG_HIST_SORT)
+		{
+			// the pair count sizes the sort
+			hip_check(hipMemcpyAsync(ctr.data(), counters, MBG_NCOUNTERS * 8, hipMemcpyDeviceToHost, stream),
+					  "hipMemcpyAsync");
+			hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+			const uint64_t n_pairs = ctr[MBG_CTR_PAIRS];
+			pair_over = n_pairs > pair_cap;
+			if (!pair_over && n_pairs)
+			{
+				auto* ka = (unsigned long long*)p.b_pk_alt.ensure(pair_cap * 8);
+				auto* va = (unsigned long long*)p.b_pv_alt.ensure(pair_cap * 8);
+				ucount = (unsigned long long*)p.b_ucount.ensure(8);
+				sort_reduce_pairs(p.b_sort_tmp, pk, pv, ka, va, n_pairs,
+								  std::max(1, n_obs + bits_for((uint64_t)windows - 1)),
+								  ucount, stream, &uk, &uv);
+			}
+			if (n_traj)
+				p.pairs_per_traj = (n_pairs + n_traj - 1) / n_traj;
+		}
 		hip_check(hipEventRecord(e1, stream), "hipEventRecord");
 
 		// ---- gather
-		std::vector<unsigned long long> ctr(MBG_NCOUNTERS);
 		hip_check(hipMemcpyAsync(ctr.data(), counters, MBG_NCOUNTERS * 8, hipMemcpyDeviceToHost, stream),
 				  "hipMemcpyAsync");
 		hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -375,14 +417,21 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 				   fp_over = ctr[MBG_CTR_FP_OVERFLOW] != 0, tail_over = ctr[MBG_CTR_TAIL_OVERFLOW] != 0;
 		if (ctr[MBG_CTR_FP_TIMEOUT])
 			throw error(MBG_E_INTERNAL, "fixed-point table: a slot was never published");
-		if (win_over || fin_over || fp_over || tail_over)
+		if (win_over || fin_over || fp_over || tail_over || pair_over)
 		{
 			if (attempt >= 6)
 				throw error(MBG_E_OVERFLOW, std::string("device hash table overflow after resizing (")
 												+ (win_over ? "window " : "") + (fin_over ? "final " : "")
-												+ (fp_over ? "fixed-point " : "") + (tail_over ? "tail" : "") + ")");
+												+ (fp_over ? "fixed-point " : "") + (tail_over ? "tail " : "")
+												+ (pair_over ? "window pairs" : "") + ")");
 			if (tail_over)
 				tail_cap *= 4;
+			if (pair_over)
+			{
+				if (sp.window_capacity)
+					throw error(MBG_E_OVERFLOW, "window pair buffer overflow; raise window_capacity");
+				pair_cap = ctr[MBG_CTR_PAIRS] + ctr[MBG_CTR_PAIRS] / 16 + 1;
+			}
 			if (win_over)
 			{
 				if (sp.window_capacity)
@@ -410,7 +459,7 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		res->truncated = ctr[MBG_CTR_TRUNCATED];
 		res->window_mode = pl.win_mode;
 		res->final_mode = pl.fin_mode;
-		res->window_capacity = pl.win_cap;
+		res->window_capacity = pl.win_mode == MBG_HIST_SORT ? pair_cap : pl.win_cap;
 
 		if (sp.device_only)
 		{
@@ -423,7 +472,23 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		hip_check(hipMemcpy(res->ent.data(), ent, (size_t)windows * 8, hipMemcpyDeviceToHost), "hipMemcpy");
 
 		res->win.assign(windows, {});
-		if (pl.win_mode == MBG_HIST_HASH)
+		if (pl.win_mode == MBG_HIST_SORT)
+		{
+			unsigned long long nu = 0;
+			if (ucount)
+				hip_check(hipMemcpy(&nu, ucount, 8, hipMemcpyDeviceToHost), "hipMemcpy");
+			std::vector<unsigned long long> keys(nu), vals(nu);
+			if (nu)
+			{
+				hip_check(hipMemcpy(keys.data(), uk, nu * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+				hip_check(hipMemcpy(vals.data(), uv, nu * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+			}
+			// keys arrive sorted by (window, observed state)
+			const uint64_t omask = n_obs >= 64 ? ~0ull : ((1ull << n_obs) - 1);
+			for (uint64_t i = 0; i < nu; i++)
+				res->win[keys[i] >> n_obs].emplace_back(keys[i] & omask, vals[i]);
+		}
+		else if (pl.win_mode == MBG_HIST_HASH)
 		{
 			std::vector<unsigned long long> tab(pl.win_cap * 2);
 			hip_check(hipMemcpy(tab.data(), win_keys, pl.win_cap * 16, hipMemcpyDeviceToHost), "hipMemcpy");

@@ -134,13 +134,35 @@ def test_block_size_validated():
         assert e.value.code == 1
 
 
-@pytest.mark.parametrize("win,fin", [("dense", "dense"), ("hash", "hash"), ("lds", "lds"), ("hash", "dense")])
+@pytest.mark.parametrize("win,fin", [("dense", "dense"), ("hash", "hash"), ("lds", "lds"), ("hash", "dense"),
+                                     ("sort", "lds"), ("sort", "hash")])
 def test_storage_mode_invariance(tmp_path, win, fin):
     bnd, cfg = model_paths("cellcycle")
     a, r = gpu_raw(tmp_path, bnd, cfg, "a", n_trajectories=8000, window_mode=win, final_mode=fin)
     b, _ = gpu_raw(tmp_path, bnd, cfg, "b", n_trajectories=8000)
-    assert r.info.window_mode == {"dense": 0, "hash": 1, "lds": 2}[win]
+    assert r.info.window_mode == {"dense": 0, "hash": 1, "lds": 2, "sort": 3}[win]
     assert_same(a, b)
+
+
+@pytest.mark.parametrize("win", ["dense", "hash", "sort"])
+def test_window_modes_with_tails_bit_exact(tmp_path, wl, win):
+    """Continuous time with fixed points: tail records expanded into each storage mode."""
+    bnd, cfg = wl["plain_small"]
+    g, r = gpu_raw(tmp_path, bnd, cfg, n_trajectories=6000, window_mode=win)
+    assert r.info.window_mode == {"dense": 0, "hash": 1, "sort": 3}[win]
+    assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=6000))
+
+
+def test_sorted_pairs_capacity(tmp_path, wl):
+    """An explicit pair capacity that is too small is an error; the automatic one grows."""
+    bnd, cfg = wl["plain_100"]
+    prog = program(bnd, cfg)
+    with pytest.raises(mg.MbgError) as e:
+        prog.simulate(n_trajectories=2000, window_mode="sort", window_capacity=1000)
+    assert e.value.code == 8
+    g, r = gpu_raw(tmp_path, bnd, cfg, n_trajectories=2000, window_mode="sort")
+    assert r.info.window_capacity >= r.info.n_window_cells
+    assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=2000))
 
 
 def test_hash_growth_and_overflow_error(tmp_path, wl):
