@@ -20,7 +20,8 @@
 #define MBG_CTR_QUEUE 6          // trajectory work queue
 #define MBG_CTR_FINISHED 7       // trajectories ended at max_time or a fixed point
 #define MBG_CTR_TAIL_OVERFLOW 8  // tail table full
-#define MBG_CTR_PAIRS 9          // window pairs appended (MBG_HIST_SORT; may exceed the capacity)
+#define MBG_CTR_PAIRS 9          // window pair slots reserved (MBG_HIST_SORT; may exceed the capacity)
+#define MBG_PAIR_PAD_KEY 0xffffffffffffffffull // key of reserved slots left unused
 #define MBG_NCOUNTERS 10
 
 #define MBG_EMPTY_KEY 0xffffffffffffffffull
@@ -89,6 +90,8 @@ struct mbg_sim_args
 	unsigned long long* pair_keys;  // MBG_HIST_SORT: appended cell keys
 	unsigned long long* pair_vals;  //                and their values
 	unsigned long long pair_cap;    //                capacity in pairs
+	int pair_lds_off;               //                u64 offset of the per-wave chunk cursors in LDS
+	int pair_chunk;                 //                pairs a wave reserves at a time
 	const mbg_param_view* pv;       // per-variant tables (batched/gradient); null for plain
 	mbg_wave_tables wt;             // wave-per-trajectory kernel only
 	int wave_lds_off;               // byte offset of the per-wave rate arrays in LDS

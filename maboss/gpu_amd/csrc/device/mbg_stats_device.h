@@ -72,6 +72,63 @@ __device__ __forceinline__ void append_pair(unsigned long long* keys, unsigned l
 	}
 }
 
+// Append through a per-wave chunk of reserved slots: cur = {next index,
+// slots left} in LDS (one pair of words per wave).  The active lanes take
+// consecutive slots; the leader reserves a new chunk (one global atomic per
+// chunk) when the current one runs out, and lanes past the old chunk's end
+// continue in the new one.
+__device__ __forceinline__ void append_pair_chunked(unsigned long long* cur, unsigned long long* keys,
+													unsigned long long* vals, uint64_t cap, unsigned long long* counter,
+													uint64_t chunk, uint64_t key, uint64_t q)
+{
+	const uint64_t m = __ballot(1);
+	const int leader = __ffsll((unsigned long long)m) - 1;
+	const int lane = __lane_id();
+	const uint64_t n = (uint64_t)__popcll(m);
+	uint64_t start = 0, split = 0, nb = 0;
+	if (lane == leader)
+	{
+		const uint64_t next = cur[0], left = cur[1];
+		start = next;
+		if (n <= left)
+		{
+			split = n;
+			cur[0] = next + n;
+			cur[1] = left - n;
+		}
+		else
+		{
+			split = left;
+			nb = atomicAdd(counter, (unsigned long long)chunk);
+			cur[0] = nb + (n - left);
+			cur[1] = chunk - (n - left);
+		}
+	}
+	start = bcast64(start, leader);
+	split = bcast64(split, leader);
+	nb = bcast64(nb, leader);
+	const uint64_t rank = (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+	const uint64_t idx = rank < split ? start + rank : nb + (rank - split);
+	if (idx < cap)
+	{
+		keys[idx] = key;
+		vals[idx] = q;
+	}
+}
+
+// End of a wave's work: pad the rest of its chunk (all lanes of the wave).
+__device__ __forceinline__ void pad_pair_chunk(unsigned long long* cur, unsigned long long* keys,
+											   unsigned long long* vals, uint64_t cap)
+{
+	const uint64_t next = cur[0], left = cur[1];
+	for (uint64_t i = __lane_id(); i < left; i += 64)
+		if (next + i < cap)
+		{
+			keys[next + i] = MBG_PAIR_PAD_KEY; // sorts after every cell key
+			vals[next + i] = 0ull;
+		}
+}
+
 // Block-private count cache in LDS: slots {key + 1, count} (0 = empty),
 // nslots a power of two, a few probes.  False when no slot was found (the
 // caller then updates the global table directly).
@@ -206,7 +263,8 @@ struct lane_hist
 		else if (a.win_mode == MBG_HIST_DENSE)
 			atomicAdd(a.win_vals + cell, (unsigned long long)q);
 		else if (a.win_mode == MBG_HIST_SORT)
-			append_pair(a.pair_keys, a.pair_vals, a.pair_cap, a.counters + MBG_CTR_PAIRS, cell, q);
+			append_pair_chunked(lds + a.pair_lds_off + 2 * (threadIdx.x >> 6), a.pair_keys, a.pair_vals, a.pair_cap,
+								a.counters + MBG_CTR_PAIRS, (uint64_t)a.pair_chunk, cell, q);
 		else
 			hash_add(a.win_table, a.win_mask, cell, q, a.counters + MBG_CTR_WIN_OVERFLOW);
 	}

@@ -388,6 +388,8 @@ __device__ void wave_main(const mbg_sim_args& a)
 			atomicAdd(a.counters + MBG_CTR_TRUNCATED, (unsigned long long)ntrunc);
 	}
 
+	if (a.win_mode == MBG_HIST_SORT)
+		pad_pair_chunk(lds + a.pair_lds_off + 2 * wave, a.pair_keys, a.pair_vals, a.pair_cap);
 	__syncthreads();
 	flush_block_hist(lds, a);
 }

@@ -77,6 +77,7 @@ struct plan
 	int win_mode, fin_mode, ent_lds;
 	int win_off = 0, fin_off = 0, ent_off = 0, lds_bytes = 0;
 	int tail_off = 0, tail_slots = 0, fp_off = 0, fp_slots = 0; // LDS caches
+	int pair_off = 0;                                            // per-wave pair chunk cursors
 	uint64_t win_cells = 0, fin_cells = 0, win_cap = 0, fin_cap = 0, fp_cap = 0;
 };
 
@@ -136,6 +137,11 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 			throw error(MBG_E_LIMIT, "final-state histogram does not fit in LDS");
 		p.fin_off = (int)(lds / 8);
 		lds += fin_dense_bytes;
+	}
+	if (p.win_mode == MBG_HIST_SORT)
+	{
+		p.pair_off = (int)(lds / 8);
+		lds += 16 * 16; // {next, left} for up to 16 waves
 	}
 	// block-private caches in front of the tail and fixed-point tables
 	if (!m.discrete_time() && p.win_mode != MBG_HIST_LDS)
@@ -321,6 +327,8 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		a.pair_keys = pk;
 		a.pair_vals = pv;
 		a.pair_cap = pair_cap;
+		a.pair_lds_off = pl.pair_off;
+		a.pair_chunk = 1024;
 
 		// ---- launch: a persistent grid filling every CU
 		const int block = sp.block_size > 0 ? sp.block_size : 256;
@@ -396,7 +404,7 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 				auto* va = (unsigned long long*)p.b_pv_alt.ensure(pair_cap * 8);
 				ucount = (unsigned long long*)p.b_ucount.ensure(8);
 				sort_reduce_pairs(p.b_sort_tmp, pk, pv, ka, va, n_pairs,
-								  std::max(1, n_obs + bits_for((uint64_t)windows - 1)),
+								  n_obs + bits_for((uint64_t)windows - 1) + 1, // + the padding bit
 								  ucount, stream, &uk, &uv);
 			}
 			if (n_traj)
@@ -486,7 +494,8 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 			// keys arrive sorted by (window, observed state)
 			const uint64_t omask = n_obs >= 64 ? ~0ull : ((1ull << n_obs) - 1);
 			for (uint64_t i = 0; i < nu; i++)
-				res->win[keys[i] >> n_obs].emplace_back(keys[i] & omask, vals[i]);
+				if (keys[i] != MBG_PAIR_PAD_KEY)
+					res->win[keys[i] >> n_obs].emplace_back(keys[i] & omask, vals[i]);
 		}
 		else if (pl.win_mode == MBG_HIST_HASH)
 		{
