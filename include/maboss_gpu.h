@@ -89,7 +89,9 @@ typedef struct mbg_compile_opts
 	int32_t no_cache;        /* 1: bypass the code-object cache */
 	int32_t with_gradient;   /* mbg_generate_code only: append the gradient kernel's
 	                            model code and include (MBG_ERR_LIMIT if the model has none) */
-	int32_t reserved[5];
+	int32_t lane_occupancy;  /* waves per SIMD the lane kernels are compiled for
+	                            (register budget 512 / waves per lane); 0 = automatic */
+	int32_t reserved[4];
 } mbg_compile_opts;
 
 /* Generated HIP source of the model (caller frees with mbg_free_string). */

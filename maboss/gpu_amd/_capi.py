@@ -63,7 +63,8 @@ class ModelInfo(ctypes.Structure):
 class CompileOpts(ctypes.Structure):
     _fields_ = [
         ("entropy_formula", ctypes.c_int32), ("no_cache", ctypes.c_int32),
-        ("with_gradient", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5),
+        ("with_gradient", ctypes.c_int32), ("lane_occupancy", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 

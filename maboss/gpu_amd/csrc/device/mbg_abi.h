@@ -30,6 +30,28 @@ struct mbg_param_view
 	const uint32_t* dn_en;  // [W] bit i clear: node i can never switch off
 };
 
+// Hides a value from the optimiser (no instruction is emitted), so that
+// expressions computed from it are not merged with identical ones computed
+// from the original value.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MBG_OPAQUE(x) asm volatile("" : "+v"(x))
+#else
+#define MBG_OPAQUE(x) asm volatile("" : "+r"(x))
+#endif
+
+// Pins running sums at a point of the instruction stream (no instruction is
+// emitted): the generated rate code places one after each node, so each
+// node's terms are added before the next node's are computed, rather than
+// all of them being kept in registers for sums the optimiser sinks to the
+// end.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MBG_PIN2(a, b) asm volatile("" : "+v"(a), "+v"(b))
+#define MBG_PIN4(a, b, c, d) asm volatile("" : "+v"(a), "+v"(b), "+v"(c), "+v"(d))
+#else
+#define MBG_PIN2(a, b) ((void)0)
+#define MBG_PIN4(a, b, c, d) ((void)0)
+#endif
+
 #if defined(__HIPCC__) || defined(__HIPCC_RTC__)
 // Rates with the model's $variables baked in as exact float constants.
 extern "C" __device__ void mbg_eval(const uint32_t* __restrict__ s, float u2, mbg_eval_t* __restrict__ out);

@@ -21,9 +21,9 @@
 // so the variant's rate table is wave-uniform, and its lanes refill from the
 // chunk until it is exhausted.
 //
-// Built once per state-word count MBG_KW (and MBG_PARAM = 0 / 1) into LLVM
-// bitcode; hipRTC links it with the generated mbg_eval / mbg_eval_p /
-// mbg_obs_index of a model.
+// Compiled by hipRTC per model, in one translation unit with the generated
+// mbg_eval / mbg_eval_p / mbg_obs_index, for MBG_KW state words (and
+// MBG_PARAM = 0 plain / 1 batched).
 #if !defined(__HIPCC_RTC__)
 #include <hip/hip_runtime.h>
 #endif
@@ -41,6 +41,13 @@
 #endif
 
 using namespace mbg_dev;
+
+// Occupancy the kernel is compiled for (waves per SIMD), set per model.
+#if defined(MBG_LANE_WAVES) && MBG_LANE_WAVES > 0
+#define MBG_LANE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(MBG_LANE_WAVES, 8)))
+#else
+#define MBG_LANE_OCCUPANCY
+#endif
 
 namespace
 {
@@ -74,7 +81,7 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long* d_trans, u
 
 #if !MBG_PARAM
 
-extern "C" __global__ void __launch_bounds__(256) simulate(mbg_sim_args a)
+extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mbg_sim_args a)
 {
 	extern __shared__ unsigned long long lds[];
 	for (int i = threadIdx.x; i < a.lds_bytes / 8; i += blockDim.x)
@@ -198,7 +205,7 @@ extern "C" __global__ void __launch_bounds__(256) simulate(mbg_sim_args a)
 // dense window / entropy / final arrays; fixed points share one table keyed
 // by (state, variant); the stream of variant v is the plain run's stream
 // with variant tag v, so variant v reproduces a plain run bit for bit.
-extern "C" __global__ void __launch_bounds__(256) simulate_batched(mbg_sim_args a)
+extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_batched(mbg_sim_args a)
 {
 	const int lane = __lane_id();
 	const uint32_t B0 = ((uint32_t)a.n_nodes + 3u) >> 2;

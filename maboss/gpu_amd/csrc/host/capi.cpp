@@ -2,6 +2,7 @@
 // catches exceptions and turns them into a status code plus a thread-local
 // message; nothing here calls exit() (the reference exits on any CUDA error,
 // src/utils.cpp:3-30).
+#include <algorithm>
 #include <cstring>
 #include <fstream>
 #include <functional>
@@ -68,6 +69,8 @@ mbg::gen_options to_gen(const mbg_compile_opts* o)
 	mbg::gen_options g;
 	if (o && o->entropy_formula == 1)
 		g.entropy = mbg::entropy_formula::reference;
+	if (o && o->lane_occupancy > 0)
+		g.lane_waves = std::min(o->lane_occupancy, 8);
 	return g;
 }
 

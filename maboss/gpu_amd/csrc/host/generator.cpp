@@ -47,8 +47,9 @@ namespace
 
 enum class ty
 {
-	b,
-	f
+	b, // C++ bool expression
+	f, // float
+	u  // uint32_t 0 or 1 (node values and the logic built from them stay integers)
 };
 
 struct code
@@ -86,8 +87,10 @@ bool node_free(const model& m, const node_t& cur, const expr& e, int depth = 0)
 class emitter
 {
 public:
-	emitter(const model& m, const node_t& cur, bool param, const std::map<std::string, int>& pidx)
-		: m_(m), cur_(cur), param_(param), pidx_(pidx)
+	// Node values are read as bits of the state words named <words>0, <words>1, ...
+	emitter(const model& m, const node_t& cur, bool param, const std::map<std::string, int>& pidx,
+			const std::string& words = "sw")
+		: m_(m), cur_(cur), param_(param), pidx_(pidx), words_(words)
 	{}
 
 	int literals = 0;
@@ -102,7 +105,7 @@ public:
 				if (i < 0)
 					throw error(MBG_E_MODEL, "unknown node name: " + e.name);
 				literals++;
-				return { "x" + std::to_string(i), ty::b };
+				return { bit(i), ty::u };
 			}
 			case op::var: {
 				auto it = m_.variables.find(e.name);
@@ -130,16 +133,16 @@ public:
 			}
 			case op::neg: return { "(-" + as_f(*e.a) + ")", ty::f };
 			case op::pos: return { as_f(*e.a), ty::f };
-			case op::lnot: return { "(!" + as_b(*e.a) + ")", ty::b };
-			case op::land: return { "((" + as_b(*e.a) + " & " + as_b(*e.b) + ") != 0)", ty::b };
-			case op::lor: return { "((" + as_b(*e.a) + " | " + as_b(*e.b) + ") != 0)", ty::b };
-			case op::lxor: return { "(" + as_b(*e.a) + " != " + as_b(*e.b) + ")", ty::b };
+			case op::lnot: return { "(" + as_u(*e.a) + " ^ 1u)", ty::u };
+			case op::land: return { "(" + as_u(*e.a) + " & " + as_u(*e.b) + ")", ty::u };
+			case op::lor: return { "(" + as_u(*e.a) + " | " + as_u(*e.b) + ")", ty::u };
+			case op::lxor: return { "(" + as_u(*e.a) + " ^ " + as_u(*e.b) + ")", ty::u };
 			case op::eq:
 			case op::ne: {
 				code l = nat(*e.a), r = nat(*e.b);
 				const char* o = e.kind == op::eq ? " == " : " != ";
-				if (l.t == ty::b && r.t == ty::b)
-					return { "(" + l.s + o + r.s + ")", ty::b };
+				if (l.t != ty::f && r.t != ty::f)
+					return { "(" + to_u(l) + o + to_u(r) + ")", ty::b };
 				return { "(" + to_f(l) + o + to_f(r) + ")", ty::b };
 			}
 			case op::lt: return cmp(e, " < ");
@@ -153,8 +156,8 @@ public:
 			case op::ternary: {
 				std::string c = as_b(*e.a);
 				code l = nat(*e.b), r = nat(*e.c);
-				if (l.t == ty::b && r.t == ty::b)
-					return { "(" + c + " ? " + l.s + " : " + r.s + ")", ty::b };
+				if (l.t != ty::f && r.t != ty::f)
+					return { "(" + c + " ? " + to_u(l) + " : " + to_u(r) + ")", ty::u };
 				return { "(" + c + " ? " + to_f(l) + " : " + to_f(r) + ")", ty::f };
 			}
 		}
@@ -163,10 +166,73 @@ public:
 
 	std::string as_b(const expr& e) { return to_b(nat(e)); }
 	std::string as_f(const expr& e) { return to_f(nat(e)); }
+	std::string as_u(const expr& e) { return to_u(nat(e)); }
+
+	// The value of a boolean expression at bit t of a 32-bit word (other
+	// bits unspecified): node values are shifted from their own bit straight
+	// to bit t and combined with word-wide logic operations.  Each node's
+	// logic then uses shifts of its own (they depend on its bit position), so
+	// the compiler does not share extracted bits between nodes and keep them
+	// live across the whole rate evaluation.
+	std::string as_aligned(const expr& e, int t)
+	{
+		switch (e.kind)
+		{
+			case op::node: {
+				const int i = m_.node_index(e.name);
+				if (i < 0)
+					throw error(MBG_E_MODEL, "unknown node name: " + e.name);
+				literals++;
+				const int b = i % 32;
+				const std::string w = words_ + std::to_string(i / 32);
+				if (b > t)
+					return "(" + w + " >> " + std::to_string(b - t) + "u)";
+				if (b < t)
+					return "(" + w + " << " + std::to_string(t - b) + "u)";
+				return w;
+			}
+			case op::alias: {
+				std::string an = e.name.substr(1);
+				const expr* d = cur_.attr(an);
+				if (!d)
+					throw error(MBG_E_MODEL, "Node " + cur_.name + " does not have attribute " + an);
+				if (stack_.count(an))
+					throw error(MBG_E_MODEL, "cyclic alias @" + an + " in node " + cur_.name);
+				stack_.insert(an);
+				std::string c = as_aligned(*d, t);
+				stack_.erase(an);
+				return c;
+			}
+			case op::paren: return as_aligned(*e.a, t);
+			case op::lnot: return "(~" + as_aligned(*e.a, t) + ")";
+			case op::land: return "(" + as_aligned(*e.a, t) + " & " + as_aligned(*e.b, t) + ")";
+			case op::lor: return "(" + as_aligned(*e.a, t) + " | " + as_aligned(*e.b, t) + ")";
+			case op::lxor: return "(" + as_aligned(*e.a, t) + " ^ " + as_aligned(*e.b, t) + ")";
+			default: {
+				const std::string u = as_u(e);
+				return t ? "(" + u + " << " + std::to_string(t) + "u)" : u;
+			}
+		}
+	}
+	// value (0 or 1, uint32_t) of node i
+	std::string bit(int i) const
+	{
+		return "((" + words_ + std::to_string(i / 32) + " >> " + std::to_string(i % 32) + "u) & 1u)";
+	}
 
 private:
-	static std::string to_b(const code& c) { return c.t == ty::b ? c.s : "(" + c.s + " != 0.0f)"; }
-	static std::string to_f(const code& c) { return c.t == ty::f ? c.s : "(" + c.s + " ? 1.0f : 0.0f)"; }
+	static std::string to_b(const code& c)
+	{
+		return c.t == ty::b ? c.s : c.t == ty::u ? "(" + c.s + " != 0u)" : "(" + c.s + " != 0.0f)";
+	}
+	static std::string to_f(const code& c)
+	{
+		return c.t == ty::f ? c.s : c.t == ty::u ? "((float)" + c.s + ")" : "(" + c.s + " ? 1.0f : 0.0f)";
+	}
+	static std::string to_u(const code& c)
+	{
+		return c.t == ty::u ? c.s : c.t == ty::b ? "(" + c.s + " ? 1u : 0u)" : "((" + c.s + " != 0.0f) ? 1u : 0u)";
+	}
 
 	code cmp(const expr& e, const char* o) { return { "(" + as_f(*e.a) + o + as_f(*e.b) + ")", ty::b }; }
 	code arith(const expr& e, const char* o) { return { "(" + as_f(*e.a) + o + as_f(*e.b) + ")", ty::f }; }
@@ -175,6 +241,7 @@ private:
 	const node_t& cur_;
 	bool param_;
 	const std::map<std::string, int>& pidx_;
+	std::string words_;
 	std::set<std::string> stack_;
 };
 
@@ -231,14 +298,29 @@ void collect_vars(const model& m, const node_t& n, const expr& e, std::set<std::
 			collect_vars(m, n, *c, out, depth);
 }
 
+// Rate evaluation.  Each node's rate is one of 0, its up rate or its down
+// rate; which one is decided by its own value x and by whether its logic
+// differs from x (the node is "enabled").  The first pass evaluates every
+// node's logic once, collects the enabled bits in words E (bit i = node i
+// enabled), and sums the rates in node order (the total, and the entropy
+// sums of the observed nodes).  The flip selection needs the prefix sums
+// again once the total is known: models up to k_one_pass_nodes keep them in
+// registers, larger ones recompute them in a second pass from E and x with
+// the same float additions in the same order (so the selected node is the
+// same), reading the state through an empty asm statement so the compiler
+// does not merge the two passes' bit extractions.
+constexpr int k_one_pass_nodes = 40;
+
 void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, bool param,
 			   const std::map<std::string, int>& pidx, generated& g)
 {
-	const int N = m.n_nodes();
+	const int N = m.n_nodes(), W = m.n_words();
 	const auto obs = m.observed_nodes();
 	std::vector<bool> is_obs(N, false);
 	for (int i : obs)
 		is_obs[i] = true;
+	const bool log_sum = opt.entropy == entropy_formula::log_sum;
+	const bool two_pass = N > k_one_pass_nodes;
 
 	if (param)
 		os << "extern \"C\" __device__ void mbg_eval_p(const uint32_t* __restrict__ s, float u2, "
@@ -246,114 +328,172 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 	else
 		os << "extern \"C\" __device__ void mbg_eval(const uint32_t* __restrict__ s, float u2, "
 			  "mbg_eval_t* __restrict__ out)\n{\n";
-
-	for (int i = 0; i < N; i++)
-		os << "\tconst bool x" << i << " = ((s[" << i / 32 << "] >> " << i % 32 << ") & 1u) != 0u;\n";
+	for (int w = 0; w < W; w++)
+		os << "\tconst uint32_t sw" << w << " = s[" << w << "];\n\tuint32_t E" << w << " = 0u;\n";
 	if (param)
-	{
-		for (int w = 0; w < m.n_words(); w++)
+		for (int w = 0; w < W; w++)
 			os << "\tconst uint32_t ue" << w << " = pv->up_en[" << w << "], de" << w << " = pv->dn_en[" << w << "];\n";
-	}
 
+	struct node_code
+	{
+		bool fast = false;
+		int up_p = -1, dn_p = -1;
+		const expr *U = nullptr, *D = nullptr;
+	};
+	std::vector<node_code> nc(N);
 	for (int i = 0; i < N; i++)
 	{
 		const node_t& n = m.nodes[i];
-		emitter em(m, n, param, pidx);
-		const expr *U = nullptr, *D = nullptr;
-		const bool fast = logic_ternary(m, n, n.attr("rate_up"), true, U)
-						  && logic_ternary(m, n, n.attr("rate_down"), false, D);
-		node_info& ni = g.nodes[i];
-		ni.fast_form = fast;
-		ni.up_param = fast ? bare_param(U, pidx) : -1;
-		ni.down_param = fast ? bare_param(D, pidx) : -1;
-		os << "\t// node " << i << ": " << n.name << (is_obs[i] ? " (observed)" : "") << "\n";
-		os << "\tfloat r" << i << ";\n\t{\n";
-		std::string u_code, d_code;
-		if (fast)
+		nc[i].fast = logic_ternary(m, n, n.attr("rate_up"), true, nc[i].U)
+					 && logic_ternary(m, n, n.attr("rate_down"), false, nc[i].D);
+		if (nc[i].fast)
 		{
-			std::string logic = em.as_b(*n.attr("logic"));
-			u_code = em.as_f(*U);
-			d_code = em.as_f(*D);
-			os << "\t\tconst bool L = " << logic << ";\n";
-			os << "\t\tr" << i << " = (x" << i << " == L) ? 0.0f : (x" << i << " ? " << d_code << " : " << u_code
-			   << ");\n";
-		}
-		else
-		{
-			std::string rd = em.as_f(*n.attr("rate_down"));
-			std::string ru = em.as_f(*n.attr("rate_up"));
-			os << "\t\tr" << i << " = x" << i << " ? " << rd << " : " << ru << ";\n";
-		}
-		if (param)
-		{
-			const int w = i / 32, b = i % 32;
-			os << "\t\tconst bool en = ((x" << i << " ? de" << w << " : ue" << w << ") >> " << b << ") & 1u;\n";
-			os << "\t\tr" << i << " = en ? r" << i << " : 0.0f;\n";
-		}
-		os << "\t}\n";
-		ni.literals = em.literals;
-		g.total_literals += em.literals;
-
-		if (is_obs[i] && opt.entropy == entropy_formula::log_sum)
-		{
-			// r*log2(r) of the selected rate; constant rates are folded here
-			// with the same float code the device uses.
-			os << "\tfloat l" << i << ";\n";
-			if (fast)
-			{
-				auto lr_of = [&](const expr* v, int pidx_v) -> std::string {
-					if (param)
-					{
-						if (pidx_v >= 0)
-							return "pv->rate_lr[" + std::to_string(pidx_v) + "]";
-						emitter e2(m, n, true, pidx);
-						return "mbg_rlog2r(" + e2.as_f(*v) + ")";
-					}
-					return float_literal(rate_log2_rate(m.evaluate(*v)));
-				};
-				os << "\tl" << i << " = (r" << i << " == 0.0f) ? 0.0f : (x" << i << " ? " << lr_of(D, ni.down_param)
-				   << " : " << lr_of(U, ni.up_param) << ");\n";
-			}
-			else
-				os << "\tl" << i << " = mbg_rlog2r(r" << i << ");\n";
+			nc[i].up_p = bare_param(nc[i].U, pidx);
+			nc[i].dn_p = bare_param(nc[i].D, pidx);
 		}
 	}
 
-	// Total and prefix sums in node order (src/generator.cpp:124-144).
-	os << "\tfloat acc = 0.0f;\n";
-	for (int i = 0; i < N; i++)
-		os << "\tacc = acc + r" << i << "; const float c" << i << " = acc;\n";
-	os << "\tconst float total = acc;\n";
-	// Flip selection (src/jit_kernels/simulation.cu:9-21): number of prefix
-	// sums strictly below u2 * total.
-	os << "\tconst float rr = u2 * total;\n\tint flip = 0;\n";
-	for (int i = 0; i < N; i++)
-		os << "\tflip += (c" << i << " < rr) ? 1 : 0;\n";
-
-	// Transition entropy of the observed nodes.
-	os << "\tfloat H = 0.0f;\n";
-	if (!obs.empty())
-	{
-		if (opt.entropy == entropy_formula::log_sum)
+	// bit patterns of a fast-form node's rates (and r*log2(r) of them)
+	auto bits_of = [&](emitter& em, const expr* v, int pidx_v, bool lr) -> std::string {
+		if (param)
 		{
-			os << "\tfloat S = 0.0f, T = 0.0f;\n\tint nnz = 0;\n";
-			for (int i : obs)
-				os << "\tS = S + r" << i << "; T = T + l" << i << "; nnz += (r" << i << " != 0.0f) ? 1 : 0;\n";
-			os << "\t{\n\t\tconst float h = mbg_log2f(S) - T / S;\n";
-			os << "\t\tH = (nnz > 1 && h > 0.0f) ? h : 0.0f;\n\t}\n";
+			if (pidx_v >= 0)
+				return "mbg_f2u(pv->" + std::string(lr ? "rate_lr[" : "rate[") + std::to_string(pidx_v) + "])";
+			return lr ? "mbg_f2u(mbg_rlog2r(" + em.as_f(*v) + "))" : "mbg_f2u(" + em.as_f(*v) + ")";
+		}
+		const float val = m.evaluate(*v);
+		char buf[32];
+		std::snprintf(buf, sizeof(buf), "0x%08xu", mbg_f2u(lr ? rate_log2_rate(val) : val));
+		return buf;
+	};
+	// the selected rate r (and l) of a fast-form node from the masks xm (x)
+	// and msk (enabled), branch-free: same value as (x == L) ? 0 : (x ? D : U)
+	auto select_rate = [&](emitter& em, int i, bool with_l) {
+		const int w = i / 32, b = i % 32;
+		if (param)
+			os << "\t\tmsk &= 0u - (((ue" << w << " ^ ((ue" << w << " ^ de" << w << ") & xm)) >> " << b << "u) & 1u);\n";
+		os << "\t\tconst uint32_t ub = " << bits_of(em, nc[i].U, nc[i].up_p, false)
+		   << ", db = " << bits_of(em, nc[i].D, nc[i].dn_p, false) << ";\n";
+		os << "\t\tr = mbg_u2f((ub ^ ((ub ^ db) & xm)) & msk);\n";
+		if (with_l)
+		{
+			os << "\t\tconst uint32_t ul = " << bits_of(em, nc[i].U, nc[i].up_p, true)
+			   << ", dl = " << bits_of(em, nc[i].D, nc[i].dn_p, true) << ";\n";
+			os << "\t\tl = mbg_u2f((ul ^ ((ul ^ dl) & xm)) & msk);\n";
+		}
+	};
+	// a node outside the fast form: its rate expressions as written
+	auto general_rate = [&](emitter& em, int i, bool with_l) {
+		const node_t& n = m.nodes[i];
+		const int w = i / 32, b = i % 32;
+		const std::string rd = em.as_f(*n.attr("rate_down")), ru = em.as_f(*n.attr("rate_up"));
+		os << "\t\tconst uint32_t x = " << em.bit(i) << ";\n";
+		os << "\t\tr = x ? " << rd << " : " << ru << ";\n";
+		if (param)
+			os << "\t\tif ((((x ? de" << w << " : ue" << w << ") >> " << b << "u) & 1u) == 0u) r = 0.0f;\n";
+		if (with_l)
+			os << "\t\tl = mbg_rlog2r(r);\n";
+	};
+
+	os << "\tfloat acc = 0.0f, S = 0.0f, T = 0.0f;\n\tint nnz = 0;\n";
+	os << "\t// pass 1: logic, enabled bits, total and entropy sums\n";
+	for (int i = 0; i < N; i++)
+	{
+		const node_t& n = m.nodes[i];
+		const int w = i / 32, b = i % 32;
+		const bool lw = is_obs[i] && log_sum;
+		emitter em(m, n, param, pidx, "sw");
+		os << "\t// node " << i << ": " << n.name << (is_obs[i] ? " (observed)" : "") << "\n";
+		if (!two_pass)
+			os << "\tfloat r" << i << ";\n";
+		os << "\t{\n\t\tfloat r" << (lw ? ", l" : "") << ";\n";
+		if (nc[i].fast)
+		{
+			os << "\t\tconst uint32_t ew = (sw" << w << " ^ " << em.as_aligned(*n.attr("logic"), b) << ") & " << (1u << b)
+			   << "u;\n";
+			os << "\t\tE" << w << " |= ew;\n";
+			os << "\t\tconst uint32_t xm = 0u - ((sw" << w << " >> " << b << "u) & 1u);\n";
+			os << "\t\tuint32_t msk = 0u - (ew >> " << b << "u);\n";
+			select_rate(em, i, lw);
 		}
 		else
+			general_rate(em, i, lw);
+		if (!two_pass)
+			os << "\t\tr" << i << " = r;\n";
+		os << "\t\tacc = acc + r;\n";
+		if (lw)
+			os << "\t\tS = S + r; T = T + l; nnz += (r != 0.0f) ? 1 : 0;\n";
+		else if (is_obs[i])
+			os << "\t\tS = S + r;\n";
+		if (two_pass)
+			os << "\t\tMBG_PIN4(acc, S, T, nnz);\n";
+		os << "\t}\n";
+		if (!two_pass)
+			os << "\tconst float c" << i << " = acc;\n";
+
+		node_info& ni = g.nodes[i];
+		ni.fast_form = nc[i].fast;
+		ni.up_param = nc[i].up_p;
+		ni.down_param = nc[i].dn_p;
+		ni.literals = em.literals;
+		g.total_literals += em.literals;
+	}
+	os << "\tconst float total = acc;\n";
+	// flip selection (src/jit_kernels/simulation.cu:9-21): the number of
+	// prefix sums strictly below u2 * total
+	os << "\tconst float rr = u2 * total;\n\tint flip = 0;\n";
+	if (!log_sum && !obs.empty())
+		os << "\tfloat H = 0.0f;\n";
+	if (!two_pass)
+	{
+		for (int i = 0; i < N; i++)
+			os << "\tflip += (c" << i << " < rr) ? 1 : 0;\n";
+		if (!log_sum && !obs.empty())
 		{
-			os << "\tfloat S = 0.0f;\n";
-			for (int i : obs)
-				os << "\tS = S + r" << i << ";\n";
 			os << "\tif (S != 0.0f)\n\t{\n\t\tfloat p;\n";
 			for (int i : obs)
 				os << "\t\tp = r" << i << " / S; H = H - ((p == 0.0f) ? 0.0f : mbg_log2f(p) * p);\n";
 			os << "\t}\n";
 		}
 	}
-	os << "\tout->total = total;\n\tout->H = H;\n\tout->flip = flip;\n\tout->pad = 0;\n}\n\n";
+	else
+	{
+		os << "\t// pass 2: the same prefix sums from E and the state\n";
+		for (int w = 0; w < W; w++)
+			os << "\tuint32_t sv" << w << " = s[" << w << "];\n\tMBG_OPAQUE(sv" << w << ");\n";
+		os << "\tfloat acc2 = 0.0f;\n";
+		for (int i = 0; i < N; i++)
+		{
+			const node_t& n = m.nodes[i];
+			const int w = i / 32, b = i % 32;
+			emitter em(m, n, param, pidx, "sv");
+			os << "\t{\n\t\tfloat r;\n";
+			if (nc[i].fast)
+			{
+				os << "\t\tconst uint32_t xm = 0u - ((sv" << w << " >> " << b << "u) & 1u);\n";
+				os << "\t\tuint32_t msk = 0u - ((E" << w << " >> " << b << "u) & 1u);\n";
+				select_rate(em, i, false);
+			}
+			else
+				general_rate(em, i, false);
+			os << "\t\tacc2 = acc2 + r;\n\t\tflip += (acc2 < rr) ? 1 : 0;\n";
+			if (!log_sum && is_obs[i])
+				os << "\t\tif (S != 0.0f) { const float p = r / S; H = H - ((p == 0.0f) ? 0.0f : mbg_log2f(p) * p); }\n";
+			os << "\t\tMBG_PIN2(acc2, flip);\n\t}\n";
+		}
+	}
+
+	// transition entropy of the observed nodes (src/generator.cpp:146-175)
+	if (obs.empty())
+		os << "\tconst float Hout = 0.0f;\n";
+	else if (log_sum)
+		os << "\tconst float h = mbg_log2f(S) - T / S;\n\tconst float Hout = (nnz > 1 && h > 0.0f) ? h : 0.0f;\n";
+	else
+		os << "\tconst float Hout = H;\n";
+	os << "\t(void)T;\n\t(void)nnz;\n";
+	for (int w = 0; w < W; w++)
+		os << "\t(void)E" << w << ";\n";
+	os << "\tout->total = total;\n\tout->H = Hout;\n\tout->flip = flip;\n\tout->pad = 0;\n}\n\n";
 }
 
 void emit_obs_index(std::ostringstream& os, const model& m)
@@ -443,8 +583,8 @@ grad_code generate_grad(const model& m, const generated& g)
 	os << "\n#define MBG_GRAD_NP " << gc.n_params << "\n\n";
 	os << "__device__ __forceinline__ void mbg_grad_dwell(const uint32_t* __restrict__ s, float dt, float cdt, "
 		  "float (&T)[MBG_GRAD_NP], float (&BT)[MBG_GRAD_NP])\n{\n";
-	for (int i = 0; i < N; i++)
-		os << "\tconst bool x" << i << " = ((s[" << i / 32 << "] >> " << i % 32 << ") & 1u) != 0u;\n";
+	for (int w = 0; w < m.n_words(); w++)
+		os << "\tconst uint32_t sw" << w << " = s[" << w << "];\n";
 	for (int i = 0; i < N; i++)
 	{
 		const node_t& n = m.nodes[i];
@@ -462,10 +602,10 @@ grad_code generate_grad(const model& m, const generated& g)
 			throw error(MBG_E_LIMIT, "the gradient form needs node " + n.name
 										 + "'s rates to be `@logic ? $u : 0` / `@logic ? 0 : $d` or parameter-free");
 		emitter em(m, n, false, pidx);
-		const std::string logic = em.as_b(*n.attr("logic"));
+		const std::string logic = em.as_u(*n.attr("logic"));
 		os << "\t{ // node " << i << ": " << n.name << "\n";
-		os << "\t\tconst bool x = x" << i << ";\n";
-		os << "\t\tconst bool L = " << logic << ";\n";
+		os << "\t\tconst bool x = " << em.bit(i) << " != 0u;\n";
+		os << "\t\tconst bool L = " << logic << " != 0u;\n";
 		if (ni.up_param >= 0)
 		{
 			gc.jparam[2 * (size_t)i] = (unsigned short)ni.up_param;

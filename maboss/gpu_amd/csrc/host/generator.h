@@ -23,6 +23,9 @@ enum class entropy_formula
 struct gen_options
 {
 	entropy_formula entropy = entropy_formula::log_sum;
+	// Waves per SIMD the lane kernels are compiled for (the register budget:
+	// 512 / waves VGPRs per lane); 0 lets the compiler choose.
+	int lane_waves = 0;
 };
 
 // What the generator decided about each node; exported for the oracle-side

@@ -1,17 +1,19 @@
-// Kernel compiler: hipRTC compile of the generated source, link against the
-// precompiled kernel bitcode, load as a module.  Replaces
+// Kernel compiler: the generated model code and the kernel source (embedded
+// at build time) are compiled together by hipRTC into one code object per
+// (model, flavour), loaded as a module.  Replaces
 // kernel_compiler::compile_simulation (src/kernel_compiler.cpp:61-164), which
 // used NVRTC (-dlto -rdc) + nvJitLink against embedded LTO fatbins.
 //
-// The link is whole-program: hiprtcLink merges the two LLVM bitcode modules
-// and optimises them together at -O3, so the generated rate code is inlined
-// into the kernel loop.  Linked code objects are cached on disk, keyed by a
-// hash of the generated source, the compile options and the kernel bitcode.
+// One translation unit lets the generated rate code inline into the kernel
+// loop and needs no bitcode from this build, so any hipRTC / code-object
+// manager the process has loaded (the system ROCm's, or one a framework
+// bundles) can compile it.  Code objects are cached on disk, keyed by a hash
+// of the complete source and the compile options.
 #include <hip/hiprtc.h>
 
-#include <atomic>
 
 #include <cstdio>
+#include <iterator>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -37,15 +39,6 @@ extern const unsigned char mbg_co_util[];
 namespace mbg
 {
 
-// Kernel bitcode embedded at build time (see csrc/Makefile), one module per
-// (state words, parametric) pair.
-struct embedded_bc
-{
-	const unsigned char* begin;
-	const unsigned char* end;
-};
-embedded_bc kernel_bitcode(int words, bool param);
-
 void hip_check(hipError_t e, const char* what)
 {
 	if (e != hipSuccess)
@@ -56,7 +49,6 @@ namespace
 {
 
 const char* const k_compile_flags[] = { "--offload-arch=gfx950",
-										"-fgpu-rdc",
 										"-O3",
 										"-std=c++17",
 										"-ffp-contract=off",
@@ -132,44 +124,15 @@ void rtc_check(hiprtcResult r, const char* what)
 		throw error(MBG_E_COMPILE, std::string(what) + ": " + hiprtcGetErrorString(r));
 }
 
-std::vector<char> compile_to_bitcode(const std::string& source)
-{
-	const char* headers[] = { mbg_hdr_numeric, mbg_hdr_abi };
-	const char* names[] = { "mbg_numeric.h", "mbg_abi.h" };
-	hiprtcProgram prog;
-	rtc_check(hiprtcCreateProgram(&prog, source.c_str(), "mbg_model.hip", 2, headers, names), "hiprtcCreateProgram");
-	hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(k_compile_flags) / sizeof(k_compile_flags[0])),
-										  k_compile_flags);
-	size_t log_size = 0;
-	hiprtcGetProgramLogSize(prog, &log_size);
-	std::string log(log_size, '\0');
-	if (log_size)
-		hiprtcGetProgramLog(prog, &log[0]);
-	if (r != HIPRTC_SUCCESS)
-	{
-		hiprtcDestroyProgram(&prog);
-		throw error(MBG_E_COMPILE, std::string("hipRTC compile failed: ") + hiprtcGetErrorString(r) + "\n" + log);
-	}
-	size_t bc_size = 0;
-	rtc_check(hiprtcGetBitcodeSize(prog, &bc_size), "hiprtcGetBitcodeSize");
-	std::vector<char> bc(bc_size);
-	rtc_check(hiprtcGetBitcode(prog, bc.data()), "hiprtcGetBitcode");
-	hiprtcDestroyProgram(&prog);
-	return bc;
-}
-
-// The whole kernel in one translation unit: compile straight to a code
-// object (gradient kernel; lane kernels when the bitcode link is refused).
+// Compile one translation unit (generated model code + kernel source, with
+// the device headers supplied by name) straight to a code object.
 std::vector<char> compile_to_code_object(const std::string& source)
 {
 	const char* headers[] = { mbg_hdr_numeric,		mbg_hdr_abi,	   mbg_hdr_kernel_args,
 							  mbg_hdr_stats_device, mbg_hdr_lane_core, mbg_hdr_grad_kernel };
 	const char* names[] = { "mbg_numeric.h",	  "mbg_abi.h",		"mbg_kernel_args.h",
 							"mbg_stats_device.h", "mbg_lane_core.h", "mbg_grad_kernel.h" };
-	std::vector<const char*> flags;
-	for (const char* f : k_compile_flags)
-		if (std::strcmp(f, "-fgpu-rdc") != 0)
-			flags.push_back(f);
+	std::vector<const char*> flags(std::begin(k_compile_flags), std::end(k_compile_flags));
 	hiprtcProgram prog;
 	rtc_check(hiprtcCreateProgram(&prog, source.c_str(), "mbg_grad.hip", 6, headers, names), "hiprtcCreateProgram");
 	hiprtcResult r = hiprtcCompileProgram(prog, (int)flags.size(), flags.data());
@@ -189,31 +152,6 @@ std::vector<char> compile_to_code_object(const std::string& source)
 	rtc_check(hiprtcGetCode(prog, co.data()), "hiprtcGetCode");
 	hiprtcDestroyProgram(&prog);
 	return co;
-}
-
-std::vector<char> link_code_object(std::vector<char>& model_bc, const embedded_bc& kbc)
-{
-	hiprtcLinkState ls;
-	const char* isa_opts[] = { "-O3" };
-	hiprtcJIT_option lopts[] = { HIPRTC_JIT_IR_TO_ISA_OPT_EXT, HIPRTC_JIT_IR_TO_ISA_OPT_COUNT_EXT };
-	void* lvals[] = { (void*)isa_opts, (void*)(uintptr_t)1 };
-	rtc_check(hiprtcLinkCreate(2, lopts, lvals, &ls), "hiprtcLinkCreate");
-	hiprtcResult r = hiprtcLinkAddData(ls, HIPRTC_JIT_INPUT_LLVM_BITCODE, model_bc.data(), model_bc.size(),
-									   "mbg_model", 0, nullptr, nullptr);
-	if (r == HIPRTC_SUCCESS)
-		r = hiprtcLinkAddData(ls, HIPRTC_JIT_INPUT_LLVM_BITCODE, (void*)kbc.begin, (size_t)(kbc.end - kbc.begin),
-							  "mbg_kernels", 0, nullptr, nullptr);
-	void* bin = nullptr;
-	size_t bin_size = 0;
-	if (r == HIPRTC_SUCCESS)
-		r = hiprtcLinkComplete(ls, &bin, &bin_size);
-	std::vector<char> out;
-	if (r == HIPRTC_SUCCESS)
-		out.assign((const char*)bin, (const char*)bin + bin_size);
-	hiprtcLinkDestroy(ls);
-	if (r != HIPRTC_SUCCESS)
-		throw error(MBG_E_COMPILE, std::string("hipRTC link failed: ") + hiprtcGetErrorString(r));
-	return out;
 }
 
 } // namespace
@@ -310,16 +248,15 @@ hipFunction_t program::function(bool param)
 
 	scope_timer tm("compiler> whole_compilation");
 	const double t0 = wall_seconds();
-	const embedded_bc kbc = kernel_bitcode(W, param);
-	if (!kbc.begin || kbc.end <= kbc.begin)
-		throw error(MBG_E_INTERNAL, "no precompiled kernel for " + std::to_string(W) + " state words");
-
-	uint64_t h = fnv1a(gen.source.data(), gen.source.size());
+	const std::string src = "#define MBG_KW " + std::to_string(W) + "\n#define MBG_PARAM " + (param ? "1" : "0")
+							+ "\n#define MBG_LANE_WAVES " + std::to_string(opts.lane_waves) + "\n" + gen.source + "\n"
+							+ mbg_src_lane;
+	uint64_t h = fnv1a(src.data(), src.size());
 	for (const char* f : k_compile_flags)
 		h = fnv1a(f, std::strlen(f), h);
-	h = fnv1a(kbc.begin, (size_t)(kbc.end - kbc.begin), h);
-	h = fnv1a(mbg_hdr_numeric, std::strlen(mbg_hdr_numeric), h);
-	h = fnv1a(mbg_hdr_abi, std::strlen(mbg_hdr_abi), h);
+	for (const char* hd :
+		 { mbg_hdr_numeric, mbg_hdr_abi, mbg_hdr_kernel_args, mbg_hdr_stats_device, mbg_hdr_lane_core })
+		h = fnv1a(hd, std::strlen(hd), h);
 	char name[64];
 	std::snprintf(name, sizeof(name), "%016llx_%d.co", (unsigned long long)h, param ? 1 : 0);
 	const std::string dir = cache_dir();
@@ -328,25 +265,7 @@ hipFunction_t program::function(bool param)
 	std::vector<char> co;
 	if (no_cache || !read_all(path, co))
 	{
-		static std::atomic<bool> link_refused { false };
-		try
-		{
-			if (link_refused.load())
-				throw error(MBG_E_COMPILE, "bitcode link refused earlier in this process");
-			std::vector<char> bc = compile_to_bitcode(gen.source);
-			co = link_code_object(bc, kbc);
-		}
-		catch (const error&)
-		{
-			link_refused.store(true);
-			// The link reads the embedded bitcode with the process's code
-			// object manager; when an older one is loaded first (for example
-			// the copy bundled with a framework), it cannot read that bitcode.
-			// Compiling the kernel source and the model in one translation
-			// unit needs no bitcode from this build.
-			co = compile_to_code_object("#define MBG_KW " + std::to_string(W) + "\n#define MBG_PARAM "
-										+ (param ? "1" : "0") + "\n" + gen.source + "\n" + mbg_src_lane);
-		}
+		co = compile_to_code_object(src);
 		if (!no_cache)
 		{
 			mkdirs(dir);
