@@ -2,7 +2,7 @@
 
 One step = one pass of the simulate/statistics path over one batch: all
 trajectories of the workload simulated from their initial state to max_time,
-with window (hashed), final-state and fixed-point statistics accumulated in
+with window (sorted pairs), final-state and fixed-point statistics accumulated in
 HBM.  The model tables and statistics storage are resident on the device when
 the timed region starts; the timed region covers K steps (statistics storage
 cleared, persistent simulate kernel, counters read back), bracketed by
@@ -10,10 +10,12 @@ torch.cuda.synchronize().
 
 Default workload ``plain_100``: 100-node random network from seed 1 (4 state
 words, in-degree 1-4, AND/OR/NOT formulas), 50 internal nodes so the windows
-key on 50 observed bits (hashed histogram), rates lognormal(0, 1), 1e6
+key on 50 observed bits (sorted window pairs), rates lognormal(0, 1), 1e6
 trajectories, max_time 100, time_tick 1, continuous time.
 
-Prints one JSON line.  Extra fields: ``roofline`` (the simulate kernel against
+Prints one JSON line.  Extra fields: ``batched`` (256 variants in one launch
+vs launched one by one), ``grad`` (forward vs likelihood-ratio backward
+throughput), ``roofline`` (the simulate kernel against
 HBM with the reference's record traffic of 8W+16 bytes per transition),
 ``roofline_valu`` (against the VALU issue rate with the per-transition op
 count 2*literals + 4N + 5*n_obs + 40), ``cpu_baseline`` (the host oracle with
@@ -38,7 +40,7 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz = 39.3 
 METRIC = "state transitions/sec on 1 MI355X (plain vs batched vs +grad) at matched stats"
 WORKLOAD_TEXT = {
     "plain_100": "100-node random network (seed 1, 4 state words, in-degree 1-4, AND/OR/NOT), 50 observed nodes "
-                 "(hashed window histogram), lognormal(0,1) rates, 1e6 trajectories, max_time 100, tick 1, continuous",
+                 "(sorted window pairs), lognormal(0,1) rates, 1e6 trajectories, max_time 100, tick 1, continuous",
     "plain_100_discrete": "plain_100 network in discrete time",
     "plain_small": "10-node random network (seed 0), all observed, 1e4 trajectories, max_time 5, tick 0.2",
     "batched_50": "50-node random network (seed 3), base network of the batched form",
@@ -74,6 +76,70 @@ def cpu_baseline(bnd, cfg, target_s, threads):
                       f"{res['seconds']:.1f} s, formulas compiled with g++ -O3 -march=native)"}
 
 
+def bench_batched(mg, netgen, wdir, steps):
+    """Batched form: the 256 variants of the batched workload (rates x lognormal(0, 0.5), 0-3 mutants,
+    1e3-1e5 trajectories, max_time 10-200) in one graph-captured launch, against the same variants
+    launched one after another."""
+    bnd, cfg = netgen.write_workload("batched_50", wdir)
+    model = mg.Model.load(bnd, cfg)
+    prog = model.compile()
+    variants = netgen.batched_variants(list(model.param_values), model.info.n_nodes)
+    total_n = sum(v["n_trajectories"] for v in variants)
+    _, info = prog.simulate_batched(variants, device_only=True)  # autotune + graph capture
+    dev, wall, trans = 0.0, 0.0, 0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        _, info = prog.simulate_batched(variants, device_only=True)
+        wall += time.perf_counter() - t0
+        dev += info.kernel_ms / 1e3
+        trans += info.transitions
+    one = {"transitions_per_s": trans / dev, "device_ms": dev / steps * 1e3, "wall_ms": wall / steps * 1e3,
+           "block": info.block_size, "blocks_per_cu": info.blocks_per_cu, "traj_per_lane": info.traj_per_lane,
+           "graph": bool(info.used_graph)}
+    sdev, swall, strans = 0.0, 0.0, 0
+    t0 = time.perf_counter()
+    for v in variants:
+        _, vi = prog.simulate_batched([v], device_only=True, use_graph=False, block_size=info.block_size,
+                                      blocks_per_cu=info.blocks_per_cu, traj_per_lane=info.traj_per_lane)
+        sdev += vi.kernel_ms / 1e3
+        strans += vi.transitions
+    swall = time.perf_counter() - t0
+    seq = {"transitions_per_s": strans / sdev, "device_ms": sdev * 1e3, "wall_ms": swall * 1e3}
+    return {"workload": "batched_50: 256 variants, " + f"{total_n} trajectories", "one_launch": one,
+            "sequential_256": seq, "speedup_device": seq["device_ms"] / one["device_ms"],
+            "speedup_wall": seq["wall_ms"] / one["wall_ms"]}
+
+
+def bench_grad(mg, netgen, wdir, steps):
+    """Gradient form on grad_30 (1e6 trajectories): the forward window probabilities at run-time rates
+    (batched kernel, one variant) and the likelihood-ratio backward over the same trajectories."""
+    import torch
+    from maboss.gpu_amd import grad as mgrad
+    bnd, cfg = netgen.write_workload("grad_30", wdir)
+    model = mg.Model.load(bnd, cfg)
+    prog = model.compile()
+    info = model.info
+    n = info.sample_count
+    rates = list(model.param_values)
+    G = torch.randn(info.windows, 1 << info.n_observed, dtype=torch.float64,
+                    generator=torch.Generator().manual_seed(0)).cuda()
+    th = torch.tensor(rates, dtype=torch.float32).cuda()
+    prog.simulate_batched([{"n_trajectories": n, "rates": rates}], device_only=True)
+    mgrad.backward(prog, G, rates=th, n_trajectories=n, info=True)
+    fwd, bwd, ft, bt = 0.0, 0.0, 0, 0
+    for _ in range(steps):
+        _, fi = prog.simulate_batched([{"n_trajectories": n, "rates": rates}], device_only=True)
+        fwd += fi.kernel_ms / 1e3
+        ft += fi.transitions
+        _, gi = mgrad.backward(prog, G, rates=th, n_trajectories=n, info=True)
+        bwd += gi.kernel_ms / 1e3
+        bt += gi.transitions
+    return {"workload": "grad_30: 30 nodes, 60 rate parameters, 5 observed, 1e6 trajectories, max_time 20",
+            "forward_transitions_per_s": ft / fwd, "backward_transitions_per_s": bt / bwd,
+            "forward_ms": fwd / steps * 1e3, "backward_ms": bwd / steps * 1e3,
+            "backward_over_forward_throughput": (bt / bwd) / (ft / fwd), "accumulator": "f64"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +150,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the batched and gradient forms")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one plain device copy of this many MiB (calibrates HBM byte counters)")
     a = ap.parse_args()
@@ -120,16 +187,18 @@ def main():
         prog.simulate(n_trajectories=n, device_only=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    transitions, kernel_ms = 0, 0.0
+    transitions, kernel_ms, sim_ms = 0, 0.0, 0.0
     for _ in range(a.steps):
         r = prog.simulate(n_trajectories=n, device_only=True)
         ri = r.info
         transitions += ri.transitions
         kernel_ms += ri.kernel_ms
+        sim_ms += ri.simulate_ms
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     per_step = transitions / a.steps
-    kernel_s = kernel_ms / a.steps / 1e3
+    kernel_s = sim_ms / a.steps / 1e3  # the simulate kernel alone (HIP events on its stream)
+    path_s = kernel_ms / a.steps / 1e3  # simulate + tail expansion + window sort/reduce
 
     # the same step with every statistic copied back to the host (PCIe + host sort)
     t1 = time.perf_counter()
@@ -142,10 +211,12 @@ def main():
     ops_per = 2 * info.total_literals + 4 * N + 5 * nobs + 40
     achieved = bytes_per * per_step / kernel_s / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{a.workload}.json")
+    tpath = os.path.join(ROOT, "profiles", f"simulate_{a.workload}_traffic.md")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            for line in f:
+                if line.startswith("bytes_per_launch:"):
+                    traffic = float(line.split(":", 1)[1])
 
     out = {
         "metric": METRIC,
@@ -165,8 +236,10 @@ def main():
                    "form": "plain", "transitions_per_step": per_step},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "simulate", "kernel_ms": kernel_s * 1e3,
-                     "bytes_per_transition": bytes_per, "note": "effective: the fused kernel writes no records"},
+                     "kernel": "simulate", "kernel_ms": kernel_s * 1e3, "device_path_ms": path_s * 1e3,
+                     "bytes_per_transition": bytes_per,
+                     "note": "effective: the fused kernel writes no records; algorithmic bytes are the "
+                             "reference's record write + read, 8W+16 per transition"},
         "roofline_valu": {"achieved": ops_per * per_step / kernel_s / 1e12, "peak": VALU_PEAK_TOPS,
                           "unit": "T lane-ops/s", "ops_per_transition": ops_per,
                           "frac": ops_per * per_step / kernel_s / 1e12 / VALU_PEAK_TOPS},
@@ -174,6 +247,9 @@ def main():
                            "window_cells": fi.n_window_cells},
         "compile_seconds": prog.compile_seconds,
     }
+    if not a.no_forms:
+        out["batched"] = bench_batched(mg, netgen, wdir, max(1, min(a.steps, 3)))
+        out["grad"] = bench_grad(mg, netgen, wdir, max(1, min(a.steps, 3)))
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(bnd, cfg, a.cpu_seconds, threads)

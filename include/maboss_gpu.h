@@ -240,12 +240,15 @@ typedef struct mbg_result_info
 	uint64_t n_fixed_points;         /* distinct fixed points */
 	uint64_t n_final_states;         /* distinct final observed states */
 	uint64_t n_window_cells;         /* nonzero (window, state) cells */
-	double kernel_ms;                /* device time of the simulate kernel(s) */
+	double kernel_ms;                /* device time of the whole device path (simulate kernel,
+	                                    tail expansion, window sort/reduce) */
 	double total_ms;                 /* host wall time of mbg_simulate */
 	int32_t window_mode, final_mode;
 	uint64_t window_capacity;
 	int32_t kernel;                  /* 0 lane per trajectory, 1 wave per trajectory */
 	int32_t pad;
+	double simulate_ms;              /* device time of the simulate kernel alone */
+	uint64_t window_pairs;           /* sorted window mode: pair slots filled by the kernels */
 } mbg_result_info;
 
 int mbg_result_get_info(const mbg_result* r, mbg_result_info* out);

@@ -135,6 +135,7 @@ class ResultInfo(ctypes.Structure):
         ("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
         ("window_mode", ctypes.c_int32), ("final_mode", ctypes.c_int32),
         ("window_capacity", ctypes.c_uint64), ("kernel", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("simulate_ms", ctypes.c_double), ("window_pairs", ctypes.c_uint64),
     ]
 
 

@@ -441,6 +441,8 @@ int mbg_result_get_info(const mbg_result* rr, mbg_result_info* out)
 		out->n_window_cells = cells;
 		out->kernel_ms = r.kernel_ms;
 		out->total_ms = r.total_ms;
+		out->simulate_ms = r.simulate_ms;
+		out->window_pairs = r.window_pairs;
 		out->window_mode = r.window_mode;
 		out->final_mode = r.final_mode;
 		out->window_capacity = r.window_capacity;

@@ -142,7 +142,8 @@ struct result
 	std::vector<std::pair<std::vector<uint32_t>, uint64_t>> fixed; // reference order
 
 	uint64_t transitions = 0, finished = 0, truncated = 0;
-	double kernel_ms = 0, total_ms = 0;
+	double kernel_ms = 0, total_ms = 0, simulate_ms = 0;
+	uint64_t window_pairs = 0;
 	int window_mode = 0, final_mode = 0;
 	int kernel = 0; // 0 lane, 1 wave
 	uint64_t window_capacity = 0;

@@ -371,15 +371,17 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		uint64_t blocks_needed = (n_traj + work_per_block - 1) / work_per_block;
 		uint64_t grid = std::min<uint64_t>((uint64_t)p.num_cus * per_cu, std::max<uint64_t>(blocks_needed, 1));
 
-		hipEvent_t e0, e1;
+		hipEvent_t e0, e1, es;
 		hip_check(hipEventCreate(&e0), "hipEventCreate");
 		hip_check(hipEventCreate(&e1), "hipEventCreate");
+		hip_check(hipEventCreate(&es), "hipEventCreate");
 		hip_check(hipEventRecord(e0, stream), "hipEventRecord");
 		if (n_traj > 0)
 		{
 			void* args[] = { &a };
 			hip_check(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, block, 1, 1, lds_total, stream, args, nullptr),
 					  "hipModuleLaunchKernel(simulate)");
+			hip_check(hipEventRecord(es, stream), "hipEventRecord");
 			if (tails)
 			{
 				const unsigned eg = (unsigned)std::min<uint64_t>(1024, (tail_cap + 255) / 256);
@@ -416,10 +418,13 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		hip_check(hipMemcpyAsync(ctr.data(), counters, MBG_NCOUNTERS * 8, hipMemcpyDeviceToHost, stream),
 				  "hipMemcpyAsync");
 		hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-		float ms = 0;
+		float ms = 0, sim_ms = 0;
 		hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+		if (n_traj > 0)
+			hip_check(hipEventElapsedTime(&sim_ms, e0, es), "hipEventElapsedTime");
 		(void)hipEventDestroy(e0);
 		(void)hipEventDestroy(e1);
+		(void)hipEventDestroy(es);
 
 		const bool win_over = ctr[MBG_CTR_WIN_OVERFLOW] != 0, fin_over = ctr[MBG_CTR_FIN_OVERFLOW] != 0,
 				   fp_over = ctr[MBG_CTR_FP_OVERFLOW] != 0, tail_over = ctr[MBG_CTR_TAIL_OVERFLOW] != 0;
@@ -462,6 +467,8 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		}
 
 		res->kernel_ms = ms;
+		res->simulate_ms = sim_ms;
+		res->window_pairs = pl.win_mode == MBG_HIST_SORT ? ctr[MBG_CTR_PAIRS] : 0;
 		res->transitions = ctr[MBG_CTR_TRANSITIONS];
 		res->finished = ctr[MBG_CTR_FINISHED];
 		res->truncated = ctr[MBG_CTR_TRUNCATED];

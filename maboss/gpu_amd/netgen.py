@@ -229,6 +229,26 @@ def write_workload(name: str, directory: str) -> tuple[str, str]:
     return workload(name).write(os.path.join(directory, name))
 
 
+def batched_variants(param_values, n_nodes: int, count: int = 256, seed: int = 3) -> list[dict]:
+    """The variants of the batched workload over the ``batched_50`` network:
+    every rate multiplied by lognormal(0, 0.5), 0-3 mutant nodes (each forced
+    on or off with probability 1/2), a trajectory count log-uniform in
+    [1e3, 1e5] and max_time uniform in [10, 200].  Drawn from a splitmix64
+    stream of its own (the network's seed with the top bit set), so they are
+    the same on every machine; returns dicts for ``Program.simulate_batched``."""
+    rng = SplitMix64(seed | (1 << 63))
+    out = []
+    for _ in range(count):
+        rates = [f32(v * f32(rng.lognormal(0.0, 0.5))) for v in param_values]
+        mut = [0] * n_nodes
+        for i in rng.sample(list(range(n_nodes)), rng.randint(0, 3)):
+            mut[i] = 1 if rng.uniform() < 0.5 else -1
+        n = int(round(math.exp(math.log(1e3) + rng.uniform() * (math.log(1e5) - math.log(1e3)))))
+        max_time = f32(10.0 + 190.0 * rng.uniform())
+        out.append({"n_trajectories": n, "rates": rates, "mutations": mut, "max_time": max_time})
+    return out
+
+
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
