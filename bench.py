@@ -112,7 +112,8 @@ def bench_batched(mg, netgen, wdir, steps):
 
 def bench_grad(mg, netgen, wdir, steps):
     """Gradient form on grad_30 (1e6 trajectories): the forward window probabilities at run-time rates
-    (batched kernel, one variant) and the likelihood-ratio backward over the same trajectories."""
+    (the simulate kernel reading the rates from a table) and the likelihood-ratio backward over the
+    same trajectories."""
     import torch
     from maboss.gpu_amd import grad as mgrad
     bnd, cfg = netgen.write_workload("grad_30", wdir)
@@ -124,11 +125,11 @@ def bench_grad(mg, netgen, wdir, steps):
     G = torch.randn(info.windows, 1 << info.n_observed, dtype=torch.float64,
                     generator=torch.Generator().manual_seed(0)).cuda()
     th = torch.tensor(rates, dtype=torch.float32).cuda()
-    prog.simulate_batched([{"n_trajectories": n, "rates": rates}], device_only=True)
+    prog.simulate(n_trajectories=n, rates=rates, device_only=True)
     mgrad.backward(prog, G, rates=th, n_trajectories=n, info=True)
     fwd, bwd, ft, bt = 0.0, 0.0, 0, 0
     for _ in range(steps):
-        _, fi = prog.simulate_batched([{"n_trajectories": n, "rates": rates}], device_only=True)
+        fi = prog.simulate(n_trajectories=n, rates=rates, device_only=True).info
         fwd += fi.kernel_ms / 1e3
         ft += fi.transitions
         _, gi = mgrad.backward(prog, G, rates=th, n_trajectories=n, info=True)

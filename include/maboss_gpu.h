@@ -125,7 +125,9 @@ typedef struct mbg_sim_params
 	double normalization;    /* <=0: n_trajectories (reference divides by sample_count) */
 	int32_t flags;           /* MBG_SIM_* bits */
 	int32_t kernel;          /* -1 auto (lane if N <= 256), 0 lane per trajectory, 1 wave per trajectory */
-	int32_t reserved[6];
+	const float* rates;      /* [n_params] run-time values of the rate parameters (mbg_model_param_name
+	                            order), host memory; NULL: the model's.  Lane kernel only. */
+	int32_t reserved[4];
 } mbg_sim_params;
 
 /* Keep the statistics in device memory and return only the counters and

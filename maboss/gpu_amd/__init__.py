@@ -142,7 +142,9 @@ class Program:
                  max_steps: int = 0, window_mode: str = "auto", final_mode: str = "auto",
                  window_capacity: int = 0, final_capacity: int = 0, fixed_capacity: int = 0,
                  block_size: int = 0, blocks_per_cu: int = 0, normalization: float = 0.0,
-                 device_only: bool = False, kernel: str = "auto", stream=None) -> "Result":
+                 device_only: bool = False, kernel: str = "auto", rates=None, stream=None) -> "Result":
+        """Simulate; ``rates`` (one float per ``Model.param_names``) replaces the model's
+        parameter values at run time (lane kernel)."""
         p = SimParams()
         lib().mbg_sim_params_default(ctypes.byref(p))
         p.n_trajectories = n_trajectories
@@ -160,6 +162,12 @@ class Program:
         p.normalization = normalization
         p.flags = 1 if device_only else 0
         p.kernel = {"auto": -1, "lane": 0, "wave": 1}[kernel]
+        keep = None
+        if rates is not None:
+            keep = np.ascontiguousarray(rates, dtype=np.float32)
+            if keep.shape != (self.model.info.n_params,):
+                raise ValueError(f"rates must have {self.model.info.n_params} entries")
+            p.rates = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         h = ctypes.c_void_p()
         s = ctypes.c_void_p(stream) if stream else None
         check(lib().mbg_simulate(self._h, ctypes.byref(p), s, ctypes.byref(h)))

@@ -77,7 +77,7 @@ class SimParams(ctypes.Structure):
         ("final_capacity", ctypes.c_uint64), ("fixed_capacity", ctypes.c_uint64),
         ("block_size", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
         ("normalization", ctypes.c_double), ("flags", ctypes.c_int32), ("kernel", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("rates", ctypes.POINTER(ctypes.c_float)), ("reserved", ctypes.c_int32 * 4),
     ]
 
 

@@ -22,8 +22,9 @@
 // chunk until it is exhausted.
 //
 // Compiled by hipRTC per model, in one translation unit with the generated
-// mbg_eval / mbg_eval_p / mbg_obs_index, for MBG_KW state words (and
-// MBG_PARAM = 0 plain / 1 batched).
+// mbg_eval / mbg_eval_p / mbg_obs_index, for MBG_KW state words and a
+// flavour MBG_PARAM: 0 plain (rates baked into mbg_eval), 1 batched,
+// 2 plain with the rates read at run time (mbg_eval_p through a.pv).
 #if !defined(__HIPCC_RTC__)
 #include <hip/hip_runtime.h>
 #endif
@@ -79,7 +80,13 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long* d_trans, u
 
 } // namespace
 
-#if !MBG_PARAM
+#if MBG_PARAM != 1
+
+#if MBG_PARAM == 2
+#define MBG_EVAL_STEP(s, u2, e) mbg_eval_p(s, u2, a.pv, e)
+#else
+#define MBG_EVAL_STEP(s, u2, e) mbg_eval(s, u2, e)
+#endif
 
 extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mbg_sim_args a)
 {
@@ -142,7 +149,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 			float u1, u2;
 			step_uniforms(a.discrete, B0, k, id, a.variant, a.seed_lo, a.seed_hi, u1, u2);
 			mbg_eval_t e;
-			mbg_eval(s, u2, &e);
+			MBG_EVAL_STEP(s, u2, &e);
 			const float tp = t;
 			t = advance_time(a.discrete, t, e.total, u1, a.tick, a.max_time, fixed);
 			const float H = fixed ? 0.0f : e.H;
@@ -199,7 +206,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 	flush_block_fp<W>(lds, a);
 }
 
-#else // MBG_PARAM
+#else // MBG_PARAM == 1
 
 // Batched variants.  Statistics of variant v live at its offsets in the
 // dense window / entropy / final arrays; fixed points share one table keyed

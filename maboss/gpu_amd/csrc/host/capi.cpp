@@ -297,6 +297,8 @@ int mbg_simulate(mbg_program* p, const mbg_sim_params* params, void* stream, mbg
 			sp.normalization = params->normalization;
 			sp.device_only = (params->flags & MBG_SIM_DEVICE_ONLY) != 0;
 			sp.kernel = params->kernel;
+			if (params->rates)
+				sp.rates.assign(params->rates, params->rates + p->p->gen.param_names.size());
 		}
 		if (sp.block_size < 0 || sp.block_size > 1024 || (sp.block_size % 64) != 0)
 			throw mbg::error(mbg::MBG_E_ARG, "block_size must be a multiple of 64 up to 1024");

@@ -57,10 +57,9 @@ struct program
 	generated gen;
 	gen_options opts;
 	int W = 1;
-	hipModule_t mod_plain = nullptr;
-	hipFunction_t fn_plain = nullptr;
-	hipModule_t mod_param = nullptr;
-	hipFunction_t fn_param = nullptr;
+	// lane kernels per flavour: 0 plain, 1 batched, 2 plain with run-time rates
+	hipModule_t mods[3] = { nullptr, nullptr, nullptr };
+	hipFunction_t fns[3] = { nullptr, nullptr, nullptr };
 	double compile_seconds = 0;
 	bool no_cache = false;
 	int device = 0;
@@ -100,7 +99,8 @@ struct program
 	std::string wave_unsupported; // reason, when the model does not fit the wave kernel
 
 	~program();
-	hipFunction_t function(bool param);
+	hipFunction_t function(int flavour);
+	device_buffer b_rate_view; // run-time rates of the plain flavour 2
 	// Kernel for the wave path at the model's nodes-per-lane (loads on first use).
 	hipFunction_t wave_function();
 };
@@ -123,6 +123,7 @@ struct sim_params
 	double normalization = 0;
 	int kernel = -1; // -1 auto, 0 lane per trajectory, 1 wave per trajectory
 	bool device_only = false; // leave the histograms in HBM
+	std::vector<float> rates;  // [n_params] run-time rate values; empty: the model's
 };
 
 struct result

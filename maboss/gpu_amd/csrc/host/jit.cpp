@@ -158,10 +158,9 @@ std::vector<char> compile_to_code_object(const std::string& source)
 
 program::~program()
 {
-	if (mod_plain)
-		(void)hipModuleUnload(mod_plain);
-	if (mod_param)
-		(void)hipModuleUnload(mod_param);
+	for (hipModule_t m : mods)
+		if (m)
+			(void)hipModuleUnload(m);
 	if (mod_wave)
 		(void)hipModuleUnload(mod_wave);
 	if (batch_graph)
@@ -239,12 +238,15 @@ hipFunction_t program::wave_function()
 	return fn;
 }
 
-hipFunction_t program::function(bool param)
+hipFunction_t program::function(int flavour)
 {
-	hipFunction_t& fn = param ? fn_param : fn_plain;
+	if (flavour < 0 || flavour > 2)
+		throw error(MBG_E_INTERNAL, "unknown kernel flavour");
+	hipFunction_t& fn = fns[flavour];
 	if (fn)
 		return fn;
-	hipModule_t& mod = param ? mod_param : mod_plain;
+	hipModule_t& mod = mods[flavour];
+	const bool param = flavour == 1;
 
 	scope_timer tm("compiler> whole_compilation");
 	const double t0 = wall_seconds();
@@ -254,7 +256,7 @@ hipFunction_t program::function(bool param)
 	// nodes and were neutral below 40 nodes.
 	const int n_nodes = mdl->n_nodes();
 	const int waves = opts.lane_waves > 0 ? opts.lane_waves : n_nodes > 160 ? 2 : n_nodes > 40 ? 3 : 0;
-	const std::string src = "#define MBG_KW " + std::to_string(W) + "\n#define MBG_PARAM " + (param ? "1" : "0")
+	const std::string src = "#define MBG_KW " + std::to_string(W) + "\n#define MBG_PARAM " + std::to_string(flavour)
 							+ "\n#define MBG_LANE_WAVES " + std::to_string(waves) + "\n" + gen.source + "\n"
 							+ mbg_src_lane;
 	uint64_t h = fnv1a(src.data(), src.size());
@@ -264,7 +266,7 @@ hipFunction_t program::function(bool param)
 		 { mbg_hdr_numeric, mbg_hdr_abi, mbg_hdr_kernel_args, mbg_hdr_stats_device, mbg_hdr_lane_core })
 		h = fnv1a(hd, std::strlen(hd), h);
 	char name[64];
-	std::snprintf(name, sizeof(name), "%016llx_%d.co", (unsigned long long)h, param ? 1 : 0);
+	std::snprintf(name, sizeof(name), "%016llx_%d.co", (unsigned long long)h, flavour);
 	const std::string dir = cache_dir();
 	const std::string path = dir + "/" + name;
 
@@ -319,7 +321,7 @@ std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const g
 		throw error(MBG_E_HIP, std::string("this build targets gfx950; current device is ") + prop.gcnArchName);
 	p->num_cus = prop.multiProcessorCount;
 	if (p->W <= max_lane_words)
-		p->function(false);
+		p->function(0);
 	else
 		p->wave_function();
 	return p;

@@ -216,7 +216,36 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		throw error(MBG_E_LIMIT, "the lane kernel serves up to " + std::to_string(32 * max_lane_words) + " nodes");
 	if (use_wave && p.opts.entropy == entropy_formula::reference)
 		throw error(MBG_E_ARG, "the wave kernel computes the log-sum entropy only");
-	hipFunction_t fn = use_wave ? p.wave_function() : p.function(false);
+	if (!sp.rates.empty() && use_wave)
+		throw error(MBG_E_LIMIT, "run-time rates are served by the lane kernel (up to 256 nodes)");
+	if (!sp.rates.empty() && sp.rates.size() != p.gen.param_names.size())
+		throw error(MBG_E_ARG, "expected " + std::to_string(p.gen.param_names.size()) + " rates");
+	hipFunction_t fn = use_wave ? p.wave_function() : p.function(sp.rates.empty() ? 0 : 2);
+	// run-time rates: a parameter view {rate, rate*log2(rate), enable masks} in device memory
+	const mbg_param_view* d_view = nullptr;
+	if (!sp.rates.empty())
+	{
+		const size_t np = sp.rates.size();
+		const size_t off_rate = (sizeof(mbg_param_view) + 15) & ~(size_t)15;
+		const size_t off_lr = off_rate + np * 4, off_en = off_lr + np * 4;
+		std::vector<char> img(off_en + 8 * (size_t)W, 0);
+		char* base = (char*)p.b_rate_view.ensure(img.size());
+		mbg_param_view v;
+		v.rate = (const float*)(base + off_rate);
+		v.rate_lr = (const float*)(base + off_lr);
+		v.up_en = (const uint32_t*)(base + off_en);
+		v.dn_en = (const uint32_t*)(base + off_en + 4 * (size_t)W);
+		std::memcpy(img.data(), &v, sizeof(v));
+		for (size_t k = 0; k < np; k++)
+		{
+			const float lr = rate_log2_rate(sp.rates[k]);
+			std::memcpy(img.data() + off_rate + 4 * k, &sp.rates[k], 4);
+			std::memcpy(img.data() + off_lr + 4 * k, &lr, 4);
+		}
+		std::memset(img.data() + off_en, 0xff, 8 * (size_t)W);
+		hip_check(hipMemcpyAsync(base, img.data(), img.size(), hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
+		d_view = (const mbg_param_view*)base;
+	}
 	res->kernel = use_wave ? 1 : 0;
 	plan pl = make_plan(m, sp, n_traj, windows, n_obs, use_wave);
 	// tail records for window storage in HBM (continuous time)
@@ -317,7 +346,7 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		a.fp_table = fp;
 		a.fp_mask = pl.fp_cap - 1;
 		a.counters = counters;
-		a.pv = nullptr;
+		a.pv = d_view;
 		a.tail_table = tail_tab;
 		a.tail_mask = tail_cap ? tail_cap - 1 : 0;
 		a.tail_lds_off = pl.tail_off;

@@ -2,8 +2,8 @@
 engine's C-ABI.
 
 No counterpart in the reference (its rates are compile-time constants and it
-differentiates nothing).  ``forward`` runs the batched kernel with the rate
-vector ``theta`` (one variant, stream tag 0) and returns the window
+differentiates nothing).  ``forward`` runs the simulate kernel with the rate
+vector ``theta`` given at run time (stream tag 0) and returns the window
 probabilities ``P[w, x]`` (observed state ``x``) as a float64 tensor;
 ``backward`` hands ``dL/dP`` to ``mbg_backward``, which replays the same
 trajectories and returns the likelihood-ratio estimate of ``dL/dtheta``.
@@ -85,9 +85,8 @@ class WindowProbabilities:
                 def forward(ctx, theta, program, n_trajectories, seed, max_steps):
                     m = program.model.info
                     th = theta.detach().to(torch.float32).cpu().numpy()
-                    results, _ = program.simulate_batched([{"n_trajectories": n_trajectories, "rates": th}],
-                                                          seed=seed, max_steps=max_steps)
-                    P = _dense_probs(results[0], m.windows, m.n_observed)
+                    res = program.simulate(n_trajectories=n_trajectories, seed=seed, max_steps=max_steps, rates=th)
+                    P = _dense_probs(res, m.windows, m.n_observed)
                     ctx.program = program
                     ctx.n = n_trajectories
                     ctx.seed = seed

@@ -272,3 +272,25 @@ def test_framework_hip_loaded_first(tmp_path):
                        cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     assert p.returncode == 0 and "ok" in p.stdout, p.stderr[-3000:]
     assert_same(raw, oracle_raw(tmp_path, bnd, cfg, n=3000))
+
+
+def test_runtime_rates_bit_exact(tmp_path, wl):
+    """Rates given at run time (plain kernel, third flavour) equal rates edited into the model."""
+    import re
+    bnd, cfg = wl["grad_30"]
+    prog = program(bnd, cfg)
+    names = prog.model.param_names
+    base = prog.model.param_values
+    same = gpu_raw(tmp_path, bnd, cfg, "s", n_trajectories=4000, rates=base)[0]
+    assert_same(same, gpu_raw(tmp_path, bnd, cfg, "p", n_trajectories=4000)[0])
+    rates = (base * np.float32(0.7)).astype(np.float32)
+    text = open(cfg).read()
+    for k, nm in enumerate(names):
+        text, c = re.subn(rf"^\{nm}\s*=.*;$", f"{nm} = {float(rates[k]):.9g};", text, flags=re.M)
+        assert c == 1
+    cfg2 = tmp_path / "scaled.cfg"
+    cfg2.write_text(text)
+    g, _ = gpu_raw(tmp_path, bnd, cfg, "r", n_trajectories=4000, rates=rates)
+    assert_same(g, oracle_raw(tmp_path, bnd, str(cfg2), n=4000))
+    with pytest.raises(ValueError):
+        prog.simulate(n_trajectories=10, rates=rates[:3])
