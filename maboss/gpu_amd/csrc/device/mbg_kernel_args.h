@@ -92,6 +92,8 @@ struct mbg_sim_args
 	unsigned long long pair_cap;    //                capacity in pairs
 	int pair_lds_off;               //                u64 offset of the per-wave chunk cursors in LDS
 	int pair_chunk;                 //                pairs a wave reserves at a time
+	int pair_stage_off;             //                u64 offset of the per-wave staging buffers in LDS
+	int pair_stage;                 //                pairs a wave stages (0: write directly)
 	const mbg_param_view* pv;       // per-variant tables (batched/gradient); null for plain
 	mbg_wave_tables wt;             // wave-per-trajectory kernel only
 	int wave_lds_off;               // byte offset of the per-wave rate arrays in LDS

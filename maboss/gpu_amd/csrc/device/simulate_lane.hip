@@ -191,14 +191,16 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 			if (ended)
 				active = false;
 		}
+		flush_staged_pairs(lds, a, false);
 	}
 
 	pend.flush_all(fw, fe);
+	flush_staged_pairs(lds, a, true);
 	wave_add_counters(a.counters + MBG_CTR_TRANSITIONS, a.counters + MBG_CTR_FINISHED, a.counters + MBG_CTR_TRUNCATED,
 					  ntrans, nfinished, ntrunc);
 
 	if (a.win_mode == MBG_HIST_SORT)
-		pad_pair_chunk(lds + a.pair_lds_off + 2 * (threadIdx.x >> 6), a.pair_keys, a.pair_vals, a.pair_cap);
+		pad_pair_chunk(lds + a.pair_lds_off + 4 * (threadIdx.x >> 6), a.pair_keys, a.pair_vals, a.pair_cap);
 
 	// Flush the block's LDS histograms and caches.
 	__syncthreads();

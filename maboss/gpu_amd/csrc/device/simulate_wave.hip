@@ -389,7 +389,7 @@ __device__ void wave_main(const mbg_sim_args& a)
 	}
 
 	if (a.win_mode == MBG_HIST_SORT)
-		pad_pair_chunk(lds + a.pair_lds_off + 2 * wave, a.pair_keys, a.pair_vals, a.pair_cap);
+		pad_pair_chunk(lds + a.pair_lds_off + 4 * wave, a.pair_keys, a.pair_vals, a.pair_cap);
 	__syncthreads();
 	flush_block_hist(lds, a);
 }

@@ -78,6 +78,7 @@ struct plan
 	int win_off = 0, fin_off = 0, ent_off = 0, lds_bytes = 0;
 	int tail_off = 0, tail_slots = 0, fp_off = 0, fp_slots = 0; // LDS caches
 	int pair_off = 0;                                            // per-wave pair chunk cursors
+	int stage_off = 0, stage = 0;                                // per-wave pair staging (lane kernel)
 	uint64_t win_cells = 0, fin_cells = 0, win_cap = 0, fin_cap = 0, fp_cap = 0;
 };
 
@@ -141,7 +142,14 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 	if (p.win_mode == MBG_HIST_SORT)
 	{
 		p.pair_off = (int)(lds / 8);
-		lds += 16 * 16; // {next, left} for up to 16 waves
+		lds += 16 * 32; // {next, left, staged count} for up to 16 waves
+		if (!wave_kernel)
+		{
+			// 128 staged pairs per wave, blocks of at most 4 waves (the lane kernel's bound)
+			p.stage = 128;
+			p.stage_off = (int)(lds / 8);
+			lds += 4 * (size_t)p.stage * 16;
+		}
 	}
 	// block-private caches in front of the tail and fixed-point tables
 	if (!m.discrete_time() && p.win_mode != MBG_HIST_LDS)
@@ -358,6 +366,8 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		a.pair_cap = pair_cap;
 		a.pair_lds_off = pl.pair_off;
 		a.pair_chunk = 1024;
+		a.pair_stage_off = pl.stage_off;
+		a.pair_stage = pl.stage;
 
 		// ---- launch: a persistent grid filling every CU
 		const int block = sp.block_size > 0 ? sp.block_size : 256;
