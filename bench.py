@@ -43,6 +43,9 @@ WORKLOAD_TEXT = {
                  "(sorted window pairs), lognormal(0,1) rates, 1e6 trajectories, max_time 100, tick 1, continuous",
     "plain_100_discrete": "plain_100 network in discrete time",
     "plain_small": "10-node random network (seed 0), all observed, 1e4 trajectories, max_time 5, tick 0.2",
+    "scale_1000": "1000-node scale-free network (seed 2, 32 state words, out-degree exponent 2.5, in-degree <= 4, "
+                  "threshold/AND-OR formulas), 20 observed nodes, Bernoulli(0.3) initial state, lognormal(0,1) "
+                  "rates, 1e6 trajectories, max_time 50, tick 0.5, continuous (one wave per trajectory)",
     "batched_50": "50-node random network (seed 3), base network of the batched form",
     "grad_30": "30-node random network (seed 4), 1e6 trajectories, max_time 20, tick 0.5",
 }

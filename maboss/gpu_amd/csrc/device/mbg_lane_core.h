@@ -37,6 +37,72 @@ __device__ __forceinline__ void init_state(uint32_t (&s)[W], uint64_t id, uint32
 	}
 }
 
+// Initial states for the lanes with `take` set, computed by the whole wave
+// (every lane must call): a trajectory needs ceil(N/4) Philox blocks, so
+// instead of one lane running them all, each lane runs one block of one
+// refilling lane's trajectory (64 / blocks trajectories per round), turns
+// its 4 uniforms into 4 state bits at bit 4*(block % 8) of its word, and the
+// words are gathered with cross-lane shuffles.  Same draws and comparisons
+// as init_state.  Lanes without `take` keep s.
+template <int W>
+__device__ void init_states_wave(uint32_t (&s)[W], bool take, uint64_t id, uint32_t variant, uint32_t seed_lo,
+								 uint32_t seed_hi, int n_nodes, const float* istate)
+{
+	const int lane = __lane_id();
+	const int B = (n_nodes + 3) >> 2; // Philox blocks per trajectory, <= 8 * W <= 64
+	const int per = 64 / B;           // trajectories per round
+	const int q = lane / B, b = lane - q * B;
+	uint64_t todo = __ballot(take);
+	while (todo)
+	{
+		// this round: the first `per` lanes of todo
+		uint64_t sel = 0, rest = todo;
+		for (int k = 0; k < per && rest; ++k)
+		{
+			sel |= rest & (0ull - rest);
+			rest &= rest - 1ull;
+		}
+		// lane L of this round's q-th trajectory
+		uint64_t u = sel;
+		for (int k = 0; k < q && u; ++k)
+			u &= u - 1ull;
+		const int target = u ? __ffsll((unsigned long long)u) - 1 : 0;
+		const bool work = u != 0ull && q < per;
+		const uint32_t lo = __shfl((uint32_t)id, target), hi = __shfl((uint32_t)(id >> 32), target);
+		uint32_t v = 0u;
+		if (work)
+		{
+			const mbg_u32x4 r = mbg_philox4x32_10(mbg_u32x4 { (uint32_t)b, lo, hi, variant }, seed_lo, seed_hi);
+			const uint32_t rw[4] = { r.x, r.y, r.z, r.w };
+#pragma unroll
+			for (int j = 0; j < 4; ++j)
+			{
+				const int i = 4 * b + j;
+				if (i < n_nodes && mbg_u01(rw[j]) <= istate[i])
+					v |= 1u << (4 * (b & 7) + j);
+			}
+		}
+		// OR the 8 blocks of each state word into the lane holding its first block
+#pragma unroll
+		for (int off = 1; off < 8; off <<= 1)
+		{
+			const uint32_t o = __shfl_down(v, off);
+			if ((b & 7) + off < 8 && b + off < B)
+				v |= o;
+		}
+		const bool mine = (sel >> lane) & 1ull;
+		const int myq = __popcll(sel & ((1ull << lane) - 1ull));
+#pragma unroll
+		for (int w = 0; w < W; ++w)
+		{
+			const uint32_t word = __shfl(v, (myq * B + 8 * w) & 63);
+			if (mine)
+				s[w] = 8 * w < B ? word : 0u;
+		}
+		todo &= ~sel;
+	}
+}
+
 // u1 (dwell) and u2 (flip) of step k; discrete time draws u2 only
 __device__ __forceinline__ void step_uniforms(bool discrete, uint32_t B0, uint32_t k, uint64_t id, uint32_t variant,
 											  uint32_t seed_lo, uint32_t seed_hi, float& u1, float& u2)

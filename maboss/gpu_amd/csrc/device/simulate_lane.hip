@@ -125,20 +125,22 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 			if (lane == leader)
 				base = atomicAdd(a.counters + MBG_CTR_QUEUE, (unsigned long long)__popcll(need));
 			base = bcast64(base, leader);
+			bool fresh = false;
 			if (!active && !exhausted)
 			{
 				const uint64_t mine = base + __popcll(need & ((1ull << lane) - 1ull));
 				if (mine < a.n_traj)
 				{
 					id = a.id_base + mine;
-					init_state<W>(s, id, a.variant, a.seed_lo, a.seed_hi, a.n_nodes, a.istate);
 					t = 0.0f;
 					k = 0;
 					active = true;
+					fresh = true;
 				}
 				else
 					exhausted = true;
 			}
+			init_states_wave<W>(s, fresh, id, a.variant, a.seed_lo, a.seed_hi, a.n_nodes, a.istate);
 		}
 		if (__ballot(active) == 0)
 			break;
@@ -261,14 +263,16 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 			if (need && next < end)
 			{
 				const uint64_t mine = next + __popcll(need & ((1ull << lane) - 1ull));
+				bool fresh = false;
 				if (!active && mine < end)
 				{
 					id = mine;
-					init_state<W>(s, id, v, a.seed_lo, a.seed_hi, a.n_nodes, istate);
 					t = 0.0f;
 					k = 0;
 					active = true;
+					fresh = true;
 				}
+				init_states_wave<W>(s, fresh, id, v, a.seed_lo, a.seed_hi, a.n_nodes, istate);
 				next = min(end, next + (uint64_t)__popcll(need));
 			}
 			if (__ballot(active) == 0)
