@@ -59,7 +59,7 @@ class Model:
         check(lib().mbg_model_load_strings(bnd.encode(), cfg.encode(), ctypes.byref(h)))
         return cls(h)
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         h = getattr(self, "_h", None)
         if h:
             _release("mbg_model_free", h)
@@ -128,7 +128,7 @@ class Program:
         self._h = h
         self.model = model
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         h = getattr(self, "_h", None)
         if h:
             _release("mbg_program_free", h)
@@ -235,7 +235,7 @@ class Result:
         self._h = handle
         self.model = model
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         h = getattr(self, "_h", None)
         if h:
             _release("mbg_result_free", h)

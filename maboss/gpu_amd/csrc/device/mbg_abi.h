@@ -1,8 +1,6 @@
 // Device-side ABI between the per-model generated code (emitted by the host
-// generator and compiled by hipRTC) and the hand-written kernels that are
-// precompiled to LLVM bitcode at build time.  hipRTC links the two bitcode
-// modules (with whole-program optimisation), so every call below is inlined
-// into the kernel.
+// generator) and the hand-written lane kernels: hipRTC compiles both in one
+// translation unit, so every call below is inlined into the kernel.
 //
 // The generated functions replace the reference's generated
 // compute_transition_rates / compute_transition_entropy /
@@ -53,6 +51,9 @@ struct mbg_param_view
 #endif
 
 #if defined(__HIPCC__) || defined(__HIPCC_RTC__)
+// Block-shared tables of mbg_eval: every thread of the block calls it before
+// the kernel's first barrier.
+extern "C" __device__ void mbg_eval_setup();
 // Rates with the model's $variables baked in as exact float constants.
 extern "C" __device__ void mbg_eval(const uint32_t* __restrict__ s, float u2, mbg_eval_t* __restrict__ out);
 // Rates read from runtime tables (batched variants, gradients).

@@ -93,6 +93,9 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 	extern __shared__ unsigned long long lds[];
 	for (int i = threadIdx.x; i < a.lds_bytes / 8; i += blockDim.x)
 		lds[i] = 0ull;
+#if MBG_PARAM == 0
+	mbg_eval_setup();
+#endif
 	__syncthreads();
 
 	const lane_hist hist { lds, a };

@@ -309,7 +309,18 @@ void collect_vars(const model& m, const node_t& n, const expr& e, std::set<std::
 // the same float additions in the same order (so the selected node is the
 // same), reading the state through an empty asm statement so the compiler
 // does not merge the two passes' bit extractions.
+//
+// Grouped second pass (baked rates, every node in the fast form, log-sum
+// entropy): the first pass also keeps the running sum at the end of every
+// group of k_group nodes.  The prefix sums are non-decreasing (rates are
+// non-negative), so the selected node lies in the first group whose end sum
+// reaches u2 * total, and every node before that group counts.  Only that
+// group is re-summed, from the kept sum at its start, with the rates read
+// from a table of (up, down) bit patterns: the same additions on the same
+// values as the full pass, so the same node is selected, for k_group nodes'
+// work instead of N.
 constexpr int k_one_pass_nodes = 40;
+constexpr int k_group = 8;
 
 void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, bool param,
 			   const std::map<std::string, int>& pidx, generated& g)
@@ -321,18 +332,6 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 		is_obs[i] = true;
 	const bool log_sum = opt.entropy == entropy_formula::log_sum;
 	const bool two_pass = N > k_one_pass_nodes;
-
-	if (param)
-		os << "extern \"C\" __device__ void mbg_eval_p(const uint32_t* __restrict__ s, float u2, "
-			  "const mbg_param_view* __restrict__ pv, mbg_eval_t* __restrict__ out)\n{\n";
-	else
-		os << "extern \"C\" __device__ void mbg_eval(const uint32_t* __restrict__ s, float u2, "
-			  "mbg_eval_t* __restrict__ out)\n{\n";
-	for (int w = 0; w < W; w++)
-		os << "\tconst uint32_t sw" << w << " = s[" << w << "];\n\tuint32_t E" << w << " = 0u;\n";
-	if (param)
-		for (int w = 0; w < W; w++)
-			os << "\tconst uint32_t ue" << w << " = pv->up_en[" << w << "], de" << w << " = pv->dn_en[" << w << "];\n";
 
 	struct node_code
 	{
@@ -351,6 +350,39 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 			nc[i].up_p = bare_param(nc[i].U, pidx);
 			nc[i].dn_p = bare_param(nc[i].D, pidx);
 		}
+	}
+
+	bool all_fast = true;
+	for (int i = 0; i < N; i++)
+		all_fast = all_fast && nc[i].fast;
+	const bool grouped = two_pass && log_sum && !param && all_fast;
+	const int n_groups = (N + k_group - 1) / k_group;
+	if (!param)
+	{
+		// The grouped second pass reads (down << 32 | up) rate bit patterns per
+		// node, padded to whole groups, from a block-shared copy of this table
+		// that mbg_eval_setup fills before the kernel's first barrier.
+		if (grouped)
+		{
+			os << "__attribute__((aligned(16))) static __device__ const unsigned long long mbg_ud[" << n_groups * k_group
+			   << "] = {";
+			for (int i = 0; i < n_groups * k_group; i++)
+			{
+				uint64_t v = 0;
+				if (i < N)
+					v = (uint64_t)mbg_f2u(m.evaluate(*nc[i].D)) << 32 | mbg_f2u(m.evaluate(*nc[i].U));
+				char buf[48];
+				std::snprintf(buf, sizeof(buf), "%s0x%016llxull", i == 0 ? "\n\t" : i % 4 ? ", " : ",\n\t",
+							  (unsigned long long)v);
+				os << buf;
+			}
+			os << "};\n__attribute__((aligned(16))) static __shared__ unsigned long long mbg_ud_lds["
+			   << n_groups * k_group << "];\n\n";
+			os << "extern \"C\" __device__ void mbg_eval_setup()\n{\n\tfor (int i = threadIdx.x; i < "
+			   << n_groups * k_group << "; i += blockDim.x)\n\t\tmbg_ud_lds[i] = mbg_ud[i];\n}\n\n";
+		}
+		else
+			os << "extern \"C\" __device__ void mbg_eval_setup() {}\n\n";
 	}
 
 	// bit patterns of a fast-form node's rates (and r*log2(r) of them)
@@ -395,6 +427,20 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 			os << "\t\tl = mbg_rlog2r(r);\n";
 	};
 
+	if (param)
+		os << "extern \"C\" __device__ void mbg_eval_p(const uint32_t* __restrict__ s, float u2, "
+			  "const mbg_param_view* __restrict__ pv, mbg_eval_t* __restrict__ out)\n{\n";
+	else
+		os << "extern \"C\" __device__ void mbg_eval(const uint32_t* __restrict__ s, float u2, "
+			  "mbg_eval_t* __restrict__ out)\n{\n";
+	for (int w = 0; w < W; w++)
+		os << "\tconst uint32_t sw" << w << " = s[" << w << "];\n\tuint32_t E" << w << " = 0u;\n";
+	if (param)
+		for (int w = 0; w < W; w++)
+			os << "\tconst uint32_t ue" << w << " = pv->up_en[" << w << "], de" << w << " = pv->dn_en[" << w << "];\n";
+	// with baked rates and log-sum entropy, the count of observed nodes with a
+	// nonzero rate comes from E, x and masks of the nonzero up / down rates
+	const bool nnz_masks = log_sum && !param && all_fast;
 	os << "\tfloat acc = 0.0f, S = 0.0f, T = 0.0f;\n\tint nnz = 0;\n";
 	os << "\t// pass 1: logic, enabled bits, total and entropy sums\n";
 	for (int i = 0; i < N; i++)
@@ -421,7 +467,9 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 		if (!two_pass)
 			os << "\t\tr" << i << " = r;\n";
 		os << "\t\tacc = acc + r;\n";
-		if (lw)
+		if (lw && nnz_masks)
+			os << "\t\tS = S + r; T = T + l;\n";
+		else if (lw)
 			os << "\t\tS = S + r; T = T + l; nnz += (r != 0.0f) ? 1 : 0;\n";
 		else if (is_obs[i])
 			os << "\t\tS = S + r;\n";
@@ -430,6 +478,8 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 		os << "\t}\n";
 		if (!two_pass)
 			os << "\tconst float c" << i << " = acc;\n";
+		if (grouped && i % k_group == k_group - 1 && i / k_group < n_groups - 1)
+			os << "\tconst float g" << i / k_group << " = acc;\n";
 
 		node_info& ni = g.nodes[i];
 		ni.fast_form = nc[i].fast;
@@ -439,6 +489,33 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 		g.total_literals += em.literals;
 	}
 	os << "\tconst float total = acc;\n";
+	if (nnz_masks && !obs.empty())
+	{
+		// observed, enabled, and the rate in the node's direction nonzero
+		// (+-0 count as zero, as r != 0.0f does)
+		os << "\tnnz = 0";
+		for (int w = 0; w < W; w++)
+		{
+			uint32_t om = 0, unz = 0, dnz = 0;
+			for (int i = w * 32; i < std::min(N, w * 32 + 32); i++)
+			{
+				const uint32_t bit = 1u << (i % 32);
+				if (is_obs[i])
+					om |= bit;
+				if (mbg_f2u(m.evaluate(*nc[i].U)) & 0x7fffffffu)
+					unz |= bit;
+				if (mbg_f2u(m.evaluate(*nc[i].D)) & 0x7fffffffu)
+					dnz |= bit;
+			}
+			if (!om)
+				continue;
+			char buf[160];
+			std::snprintf(buf, sizeof(buf), " + __builtin_popcount(E%d & ((sw%d & 0x%08xu) | (~sw%d & 0x%08xu)) & 0x%08xu)", w, w,
+						  dnz & om, w, unz & om, om);
+			os << buf;
+		}
+		os << ";\n";
+	}
 	// flip selection (src/jit_kernels/simulation.cu:9-21): the number of
 	// prefix sums strictly below u2 * total
 	os << "\tconst float rr = u2 * total;\n\tint flip = 0;\n";
@@ -455,6 +532,31 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 				os << "\t\tp = r" << i << " / S; H = H - ((p == 0.0f) ? 0.0f : mbg_log2f(p) * p);\n";
 			os << "\t}\n";
 		}
+	}
+	else if (grouped)
+	{
+		os << "\t// pass 2 (grouped): the group holding the selected node, and the sum before it\n";
+		os << "\tint gsel = 0;\n\tfloat acc2 = 0.0f;\n";
+		for (int k = 0; k < n_groups - 1; k++)
+			os << "\tif (g" << k << " < rr) { gsel = " << k + 1 << "; acc2 = g" << k << "; }\n";
+		os << "\tMBG_PIN2(acc2, gsel);\n";
+		// a group lies inside one state word
+		static_assert(32 % k_group == 0, "groups must not straddle state words");
+		os << "\tconst int gw = gsel / " << 32 / k_group << ", gb = (gsel % " << 32 / k_group << ") * " << k_group
+		   << ";\n";
+		os << "\tuint32_t ge = E0, gx = sw0;\n";
+		for (int w = 1; w < W; w++)
+			os << "\tge = (gw == " << w << ") ? E" << w << " : ge;\n\tgx = (gw == " << w << ") ? sw" << w << " : gx;\n";
+		os << "\tge >>= gb;\n\tgx >>= gb;\n";
+		os << "\tconst unsigned long long* gud = mbg_ud_lds + gsel * " << k_group << ";\n";
+		os << "\tint cnt = 0;\n";
+		os << "#pragma unroll\n\tfor (int j = 0; j < " << k_group << "; ++j)\n\t{\n";
+		os << "\t\tconst unsigned long long ud = gud[j];\n";
+		os << "\t\tconst uint32_t ub = (uint32_t)ud, db = (uint32_t)(ud >> 32);\n";
+		os << "\t\tconst uint32_t xm = 0u - ((gx >> j) & 1u), msk = 0u - ((ge >> j) & 1u);\n";
+		os << "\t\tacc2 = acc2 + mbg_u2f((ub ^ ((ub ^ db) & xm)) & msk);\n";
+		os << "\t\tcnt += (acc2 < rr) ? 1 : 0;\n\t\tMBG_PIN2(acc2, cnt);\n\t}\n";
+		os << "\tflip = gsel * " << k_group << " + cnt;\n";
 	}
 	else
 	{

@@ -1,14 +1,25 @@
 // Test harness: runs a model's generated HIP source on the host.  The file is
-// compiled with -DGEN_FILE="path" and -I to the device headers; __device__ is
-// defined away so mbg_eval / mbg_obs_index become plain host functions.
+// compiled with -DGEN_FILE="path" and -I to the device headers; __device__ and
+// __shared__ are defined away so mbg_eval / mbg_obs_index become plain host
+// functions, and mbg_eval_setup fills its tables as a one-thread block.
 // stdin: lines "w0 .. w{W-1} u2bits" (hex); stdout: "total H flip obs" with
 // floats as hex bits -- the same format as the oracle's --eval-states.
 #define __device__
+#define __shared__
 #include <cstdio>
 #include <iostream>
 #include <vector>
 
 #include "mbg_abi.h"
+
+static const struct
+{
+	int x = 0;
+} threadIdx;
+static const struct
+{
+	int x = 1;
+} blockDim;
 
 #include GEN_FILE
 
@@ -16,6 +27,7 @@ int main()
 {
 	std::vector<uint32_t> s(MBG_W);
 	uint32_t u2b;
+	mbg_eval_setup();
 	while (true)
 	{
 		for (int w = 0; w < MBG_W; w++)

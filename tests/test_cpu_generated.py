@@ -116,6 +116,64 @@ def test_flip_selection_brute_force(native_bin, tmp_path):
     assert int(out[-1].split()[2]) == 6
 
 
+def test_grouped_flip_selection_brute_force(native_bin, tmp_path):
+    """Models above 40 nodes select the flip from groups of 8 nodes: the same
+    node as a full sequential float32 prefix scan, for states and u2 values
+    that put the target on group boundaries, inside runs of zero-rate nodes
+    (+0 and -0 rates) and in the last, partial group.  The observed nodes'
+    nonzero-rate count (entropy is 0 unless two or more are nonzero) is
+    checked against the same brute force."""
+    n = 45
+    rng = np.random.default_rng(11)
+    up = rng.lognormal(0, 1, n).astype(np.float32)
+    dn = rng.lognormal(0, 1, n).astype(np.float32)
+    up[[3, 8, 9, 10, 15, 16, 23, 40, 44]] = 0.0
+    dn[[0, 7, 8, 24, 31, 32, 41]] = 0.0
+    neg0 = {(5, "u"), (33, "d")}
+    up[5] = dn[33] = np.float32(-0.0)
+    lit = lambda v, neg: "-0" if neg else repr(float(v))
+    bnd = "".join(f"node X{i} {{ logic = !X{i}; rate_up = @logic ? $u{i} : 0; rate_down = @logic ? 0 : $d{i}; }}\n"
+                  for i in range(n))
+    cfg = "".join(f"$u{i} = {lit(up[i], (i, 'u') in neg0)};\n$d{i} = {lit(dn[i], (i, 'd') in neg0)};\n"
+                  for i in range(n))
+    cfg += "".join(f"X{i}.is_internal = 1;\n" for i in range(0, n, 3))
+    m = mg.Model.from_strings(bnd, cfg)
+    gen = tmp_path / "grouped.hip"
+    src = m.generate_code()
+    assert "pass 2 (grouped)" in src
+    gen.write_text(src)
+    exe = native_bin("gen_harness", extra=[f'-DGEN_FILE="{gen}"'], tag=str(gen))
+    states = rng.integers(0, 2, size=(400, n)).astype(np.uint8)
+    states[0] = 0
+    states[1] = 1
+    lines, expect = [], []
+    obs = np.array([i % 3 != 0 for i in range(n)])
+    for st in states:
+        r = np.where(st == 1, dn, up).astype(np.float32)  # every node is enabled (logic = !x)
+        pref = np.cumsum(r, dtype=np.float32)
+        total = pref[-1]
+        # u2 values landing exactly on group-end prefix sums, just above and below them, and random
+        cand = [np.float32(1.0), np.float32(2 ** -24)] + list(rng.uniform(0, 1, 6).astype(np.float32))
+        for g in range(7, n, 8):
+            if total > 0:
+                q = np.float32(pref[g] / total)
+                cand += [q, np.nextafter(q, np.float32(2)), np.nextafter(q, np.float32(0))]
+        words = [int(sum(int(b) << j for j, b in enumerate(st[w * 32:(w + 1) * 32]))) for w in range((n + 31) // 32)]
+        nnz = int(np.sum((r != 0) & obs))
+        for u in cand:
+            u = np.float32(min(max(u, np.float32(2 ** -24)), np.float32(1.0)))
+            lines.append(" ".join(f"{x:x}" for x in words) + f" {u.view(np.uint32):08x}\n")
+            expect.append((int(np.sum(pref < np.float32(u * total))), total, nnz))
+    out = pipe(exe, "".join(lines))
+    assert len(out) == len(expect)
+    for line, (flip, total, nnz) in zip(out, expect):
+        f = line.split()
+        assert np.uint32(int(f[0], 16)).view(np.float32) == total
+        assert int(f[2]) == flip, (line, flip)
+        if nnz < 2:
+            assert int(f[1], 16) == 0
+
+
 def test_gradient_kernel_compiles_for_gfx950(tmp_path):
     """The gradient kernel's translation unit (generated model code + the
     kernel header) compiles for gfx950; models without a gradient form are
