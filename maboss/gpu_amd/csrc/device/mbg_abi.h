@@ -50,12 +50,28 @@ struct mbg_param_view
 #define MBG_PIN4(a, b, c, d) ((void)0)
 #endif
 
+// Makes one lane's LDS stores visible to the other lanes of its wave.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MBG_WAVE_SYNC()                                                                                                \
+	do                                                                                                                 \
+	{                                                                                                                  \
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");                                                         \
+		__builtin_amdgcn_wave_barrier();                                                                               \
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");                                                         \
+	} while (0)
+#else
+#define MBG_WAVE_SYNC() ((void)0)
+#endif
+
 #if defined(__HIPCC__) || defined(__HIPCC_RTC__)
 // Block-shared tables of mbg_eval: every thread of the block calls it before
 // the kernel's first barrier.
 extern "C" __device__ void mbg_eval_setup();
 // Rates with the model's $variables baked in as exact float constants.
 extern "C" __device__ void mbg_eval(const uint32_t* __restrict__ s, float u2, mbg_eval_t* __restrict__ out);
+// Per-wave tables of mbg_eval_p for the parameter view pv: every lane of the
+// wave calls it (converged) before the wave's first mbg_eval_p with pv.
+extern "C" __device__ void mbg_eval_p_setup(const mbg_param_view* __restrict__ pv);
 // Rates read from runtime tables (batched variants, gradients).
 extern "C" __device__ void mbg_eval_p(const uint32_t* __restrict__ s, float u2, const mbg_param_view* __restrict__ pv,
 									  mbg_eval_t* __restrict__ out);

@@ -61,6 +61,7 @@ extern "C" __global__ void __launch_bounds__(256) simulate_grad(mbg_grad_args a)
 		jp[i] = a.jparam[i];
 	for (int i = threadIdx.x; i < NP * (int)blockDim.x; i += blockDim.x)
 		cnt[i] = 0;
+	mbg_eval_p_setup(&a.pv);
 	__syncthreads();
 	const double* G = a.g_lds ? gtab : a.gprob;
 	unsigned short* mycnt = cnt + threadIdx.x; // count of parameter j at mycnt[j * blockDim.x]

@@ -95,6 +95,8 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 		lds[i] = 0ull;
 #if MBG_PARAM == 0
 	mbg_eval_setup();
+#else
+	mbg_eval_p_setup(a.pv);
 #endif
 	__syncthreads();
 
@@ -245,6 +247,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 		const unsigned ent_off = vd.ent_off;
 		const float* istate = vd.istate;
 		const mbg_param_view pv = vd.pv;
+		mbg_eval_p_setup(&pv);
 
 		auto fw = [&](int w, uint64_t o, uint64_t q) {
 			atomicAdd(a.win_vals + win_off + (((uint64_t)w << a.n_obs) | o), (unsigned long long)q);

@@ -95,10 +95,11 @@ __device__ void wave_main(const mbg_sim_args& a)
 	const unsigned* coff = (const unsigned*)(nodes + N);
 	const unsigned short* child = (const unsigned short*)(coff + N + 1);
 	const unsigned short* obsl = child + a.wt.n_child;
+	const unsigned short* ra = obsl + a.n_obs; // rate slot of each node
+	const unsigned short* oba = ra + N;        // rate slot of each observed node
 	const int lane = __lane_id();
 	const int wave = threadIdx.x >> 6;
 	float* rates = (float*)(ldsb + a.wave_lds_off + a.wt.tab_bytes) + (size_t)wave * npl * 64;
-	auto raddr = [&](int node) { return (node % npl) * 64 + node / npl; };
 
 	const lane_hist hist { lds, a };
 	const uint32_t B0 = ((uint32_t)N + 3u) >> 2;
@@ -174,14 +175,11 @@ __device__ void wave_main(const mbg_sim_args& a)
 		for (;;)
 		{
 			// this lane's slice and the scan over lanes
-			float r[NPLMAX];
 			float L = 0.0f;
 #pragma unroll
 			for (int s = 0; s < NPLMAX; ++s)
-			{
-				r[s] = s < npl ? rates[s * 64 + lane] : 0.0f;
-				L = L + r[s];
-			}
+				if (s < npl)
+					L = L + rates[s * 64 + lane];
 			const float P = ks_scan(L, lane);
 			const float total = rdlane_f(P, 63);
 
@@ -221,21 +219,17 @@ __device__ void wave_main(const mbg_sim_args& a)
 				else
 					t = t + (-mbg_logf(u1)) / total;
 				t = fminf(t, a.max_time);
-				// transition entropy, log-sum form: observed node j adds to lane j % 64
-				float Sv = 0.0f, Tv = 0.0f;
-				int cnt = 0;
-				for (int j = lane; j < a.n_obs; j += 64)
+				// transition entropy, log-sum form: observed node j (< 64) on lane j
+				float Sv = 0.0f, Tv = 0.0f, v = 0.0f;
+				if (lane < a.n_obs)
 				{
-					const float v = rates[raddr(obsl[j])];
+					v = rates[oba[lane]];
 					Sv = Sv + v;
 					Tv = Tv + mbg_rlog2r(v);
-					cnt += v != 0.0f;
 				}
 				const float S = rdlane_f(ks_scan(Sv, lane), 63);
 				const float T = rdlane_f(ks_scan(Tv, lane), 63);
-#pragma unroll
-				for (int off = 32; off > 0; off >>= 1)
-					cnt += __shfl_xor(cnt, off);
+				const int cnt = __popcll(__ballot(v != 0.0f));
 				const float h = mbg_log2f(S) - T / S;
 				H = (cnt > 1 && h > 0.0f) ? h : 0.0f;
 			}
@@ -277,13 +271,11 @@ __device__ void wave_main(const mbg_sim_args& a)
 			{
 				if (s < npl)
 				{
-					qsum = qsum + r[s];
-					if (r[s] > 0.0f)
-					{
-						last = s;
-						if (ks < 0 && qsum >= rem)
-							ks = s;
-					}
+					const float rs = rates[s * 64 + lane];
+					qsum = qsum + rs;
+					const bool pos = rs > 0.0f;
+					last = pos ? s : last;
+					ks = (pos && ks < 0 && qsum >= rem) ? s : ks;
 				}
 			}
 			const int kstar = (int)rdlane_u((uint32_t)(ks >= 0 ? ks : last), ls);
@@ -305,7 +297,7 @@ __device__ void wave_main(const mbg_sim_args& a)
 				const int node = !valid ? 0 : (j == 0 ? f : (int)child[c0 + j - 1]);
 				const float rv = node_rate(nodes, node, valid, sw);
 				if (valid)
-					rates[raddr(node)] = rv;
+					rates[ra[node]] = rv;
 			}
 			__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 		}

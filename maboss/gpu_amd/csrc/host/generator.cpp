@@ -355,9 +355,47 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 	bool all_fast = true;
 	for (int i = 0; i < N; i++)
 		all_fast = all_fast && nc[i].fast;
-	const bool grouped = two_pass && log_sum && !param && all_fast;
+	const bool grouped = two_pass && log_sum && all_fast;
 	const int n_groups = (N + k_group - 1) / k_group;
-	if (!param)
+	if (param)
+	{
+		// The grouped second pass of mbg_eval_p reads the (down << 32 | up)
+		// rate bit patterns of the wave's rate table: one table per wave (up
+		// to 4 per block), filled by mbg_eval_p_setup from the parameter view
+		// before the wave's first mbg_eval_p with that view.
+		if (grouped)
+		{
+			os << "__attribute__((aligned(16))) static __shared__ unsigned long long mbg_udp_lds[4][" << n_groups * k_group
+			   << "];\n\n";
+			os << "extern \"C\" __device__ void mbg_eval_p_setup(const mbg_param_view* __restrict__ pv)\n{\n";
+			os << "\tunsigned long long* t = mbg_udp_lds[(threadIdx.x >> 6) & 3];\n";
+			os << "\tif ((threadIdx.x & 63) == 0)\n\t{\n";
+			for (int i = 0; i < N; i++)
+			{
+				emitter em(m, m.nodes[i], param, pidx, "sw");
+				std::set<std::string> uv, dv;
+				collect_vars(m, m.nodes[i], *nc[i].U, uv);
+				collect_vars(m, m.nodes[i], *nc[i].D, dv);
+				auto code = [&](const expr* v, int p_i, const std::set<std::string>& vars) -> std::string {
+					if (p_i >= 0)
+						return "mbg_f2u(pv->rate[" + std::to_string(p_i) + "])";
+					if (vars.empty())
+					{
+						char buf[32];
+						std::snprintf(buf, sizeof(buf), "0x%08xu", mbg_f2u(m.evaluate(*v)));
+						return buf;
+					}
+					return "mbg_f2u(" + em.as_f(*v) + ")";
+				};
+				os << "\t\tt[" << i << "] = (unsigned long long)" << code(nc[i].D, nc[i].dn_p, dv) << " << 32 | "
+				   << code(nc[i].U, nc[i].up_p, uv) << ";\n";
+			}
+			os << "\t}\n\tMBG_WAVE_SYNC();\n}\n\n";
+		}
+		else
+			os << "extern \"C\" __device__ void mbg_eval_p_setup(const mbg_param_view* __restrict__ pv) { (void)pv; }\n\n";
+	}
+	else
 	{
 		// The grouped second pass reads (down << 32 | up) rate bit patterns per
 		// node, padded to whole groups, from a block-shared copy of this table
@@ -545,10 +583,23 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 		os << "\tconst int gw = gsel / " << 32 / k_group << ", gb = (gsel % " << 32 / k_group << ") * " << k_group
 		   << ";\n";
 		os << "\tuint32_t ge = E0, gx = sw0;\n";
+		if (param)
+			os << "\tuint32_t gu = ue0, gd = de0;\n";
 		for (int w = 1; w < W; w++)
+		{
 			os << "\tge = (gw == " << w << ") ? E" << w << " : ge;\n\tgx = (gw == " << w << ") ? sw" << w << " : gx;\n";
+			if (param)
+				os << "\tgu = (gw == " << w << ") ? ue" << w << " : gu;\n\tgd = (gw == " << w << ") ? de" << w
+				   << " : gd;\n";
+		}
+		// mutants: a node that may not switch in its direction has rate 0
+		if (param)
+			os << "\tge &= (gx & gd) | (~gx & gu);\n";
 		os << "\tge >>= gb;\n\tgx >>= gb;\n";
-		os << "\tconst unsigned long long* gud = mbg_ud_lds + gsel * " << k_group << ";\n";
+		if (param)
+			os << "\tconst unsigned long long* gud = mbg_udp_lds[(threadIdx.x >> 6) & 3] + gsel * " << k_group << ";\n";
+		else
+			os << "\tconst unsigned long long* gud = mbg_ud_lds + gsel * " << k_group << ";\n";
 		os << "\tint cnt = 0;\n";
 		os << "#pragma unroll\n\tfor (int j = 0; j < " << k_group << "; ++j)\n\t{\n";
 		os << "\t\tconst unsigned long long ud = gud[j];\n";

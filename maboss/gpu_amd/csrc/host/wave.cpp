@@ -128,6 +128,15 @@ wave_tables build_wave_tables(const model& m)
 	put(t.child_off.data(), t.child_off.size() * 4);
 	put(t.child.data(), t.child.size() * 2);
 	put(t.obs.data(), t.obs.size() * 2);
+	// LDS rate slot of every node, then of every observed node (slot of node
+	// i: lane i / npl, row i % npl of the wave's [npl][64] rate array)
+	std::vector<unsigned short> ra(N), oba(obs.size());
+	for (int i = 0; i < N; i++)
+		ra[i] = (unsigned short)((i % t.npl) * 64 + i / t.npl);
+	for (size_t j = 0; j < obs.size(); j++)
+		oba[j] = ra[obs[j]];
+	put(ra.data(), ra.size() * 2);
+	put(oba.data(), oba.size() * 2);
 	while (t.image.size() % 16)
 		t.image.push_back(0);
 	return t;
