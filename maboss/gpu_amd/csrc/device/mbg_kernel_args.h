@@ -50,7 +50,7 @@ struct mbg_wave_tables
 	const unsigned short* obs;     // [n_obs] observed nodes in order
 	int n_child, npl;              // entries of child; nodes per lane
 	int tab_bytes;                 // LDS bytes of the staged tables (16-aligned)
-	int pad;
+	int obs_tab_off;               // byte offset of the observed nodes' {up, down, ul, dl}
 };
 
 struct mbg_sim_args

@@ -97,9 +97,14 @@ __device__ void wave_main(const mbg_sim_args& a)
 	const unsigned short* obsl = child + a.wt.n_child;
 	const unsigned short* ra = obsl + a.n_obs; // rate slot of each node
 	const unsigned short* oba = ra + N;        // rate slot of each observed node
+	const float4* otab = (const float4*)(ldsb + a.wave_lds_off + a.wt.obs_tab_off);
 	const int lane = __lane_id();
 	const int wave = threadIdx.x >> 6;
-	float* rates = (float*)(ldsb + a.wave_lds_off + a.wt.tab_bytes) + (size_t)wave * npl * 64;
+	// the wave's rates, NPLMAX rows of 64 lanes; rows npl.. stay zero, so the
+	// per-slot loops need no bounds
+	float* rates = (float*)(ldsb + a.wave_lds_off + a.wt.tab_bytes) + (size_t)wave * NPLMAX * 64;
+	for (int s = npl; s < NPLMAX; ++s)
+		rates[s * 64 + lane] = 0.0f;
 
 	const lane_hist hist { lds, a };
 	const uint32_t B0 = ((uint32_t)N + 3u) >> 2;
@@ -178,8 +183,7 @@ __device__ void wave_main(const mbg_sim_args& a)
 			float L = 0.0f;
 #pragma unroll
 			for (int s = 0; s < NPLMAX; ++s)
-				if (s < npl)
-					L = L + rates[s * 64 + lane];
+				L = L + rates[s * 64 + lane];
 			const float P = ks_scan(L, lane);
 			const float total = rdlane_f(P, 63);
 
@@ -224,8 +228,9 @@ __device__ void wave_main(const mbg_sim_args& a)
 				if (lane < a.n_obs)
 				{
 					v = rates[oba[lane]];
+					const float4 ot = otab[lane]; // {up, down, r log2 r of each}
 					Sv = Sv + v;
-					Tv = Tv + mbg_rlog2r(v);
+					Tv = Tv + (v == 0.0f ? 0.0f : (v == ot.x ? ot.z : ot.w));
 				}
 				const float S = rdlane_f(ks_scan(Sv, lane), 63);
 				const float T = rdlane_f(ks_scan(Tv, lane), 63);
@@ -269,14 +274,11 @@ __device__ void wave_main(const mbg_sim_args& a)
 #pragma unroll
 			for (int s = 0; s < NPLMAX; ++s)
 			{
-				if (s < npl)
-				{
-					const float rs = rates[s * 64 + lane];
-					qsum = qsum + rs;
-					const bool pos = rs > 0.0f;
-					last = pos ? s : last;
-					ks = (pos && ks < 0 && qsum >= rem) ? s : ks;
-				}
+				const float rs = rates[s * 64 + lane];
+				qsum = qsum + rs;
+				const bool pos = rs > 0.0f;
+				last = pos ? s : last;
+				ks = (pos && ks < 0 && qsum >= rem) ? s : ks;
 			}
 			const int kstar = (int)rdlane_u((uint32_t)(ks >= 0 ? ks : last), ls);
 			const int f = min(max(ls * npl + kstar, 0), N - 1);

@@ -21,12 +21,14 @@ void hip_check(hipError_t e, const char* what);
 // Tables of the wave-per-trajectory kernel (built from the model on the host).
 struct wave_tables
 {
-	int npl = 1;
+	int npl = 1;         // nodes per lane
+	int rows = 1;        // rows of a wave's rate array: npl rounded up to a power of two
+	int obs_tab_off = 0; // byte offset of the observed nodes' {up, down, ul, dl} in image
 	std::vector<mbg_wnode> nodes;
 	std::vector<unsigned> child_off;
 	std::vector<unsigned short> child;
 	std::vector<unsigned short> obs;
-	std::vector<char> image; // nodes, child_off, child, obs; padded to 16 bytes
+	std::vector<char> image; // nodes, child_off, child, obs, slots, observed rates; 16-byte padded
 };
 
 // Throws MBG_E_LIMIT with the reason when the model does not fit the wave kernel.

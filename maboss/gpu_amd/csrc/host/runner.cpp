@@ -382,7 +382,7 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 			int best_waves = 0;
 			for (int bs = 512; bs >= 64; bs -= 64)
 			{
-				const int lds = ((pl.lds_bytes + 15) & ~15) + (int)p.wtab->image.size() + (bs / 64) * p.wtab->npl * 64 * 4;
+				const int lds = ((pl.lds_bytes + 15) & ~15) + (int)p.wtab->image.size() + (bs / 64) * p.wtab->rows * 64 * 4;
 				if (lds > 160 * 1024)
 					continue;
 				int occ = 0;
@@ -417,9 +417,10 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 			a.wt.obs = a.wt.child + wt.child.size();
 			a.wt.n_child = (int)wt.child.size();
 			a.wt.npl = wt.npl;
+			a.wt.obs_tab_off = wt.obs_tab_off;
 			a.wt.tab_bytes = (int)wt.image.size();
 			a.wave_lds_off = (pl.lds_bytes + 15) & ~15;
-			lds_total = a.wave_lds_off + a.wt.tab_bytes + (block / 64) * wt.npl * 64 * 4;
+			lds_total = a.wave_lds_off + a.wt.tab_bytes + (block / 64) * wt.rows * 64 * 4;
 			if (lds_total > 160 * 1024)
 				throw error(MBG_E_LIMIT, "wave kernel tables do not fit in LDS; use a smaller block size");
 			work_per_block = (uint64_t)(block / 64);

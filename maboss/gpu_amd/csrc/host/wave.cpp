@@ -63,6 +63,8 @@ wave_tables build_wave_tables(const model& m)
 	if (N > 64 * 32)
 		throw error(MBG_E_LIMIT, "the wave kernel supports up to 2048 nodes");
 	t.npl = (N + 63) / 64;
+	while (t.rows < t.npl)
+		t.rows <<= 1;
 	std::vector<std::vector<int>> children(N);
 	t.nodes.resize(N);
 	const auto obs = m.observed_nodes();
@@ -139,6 +141,15 @@ wave_tables build_wave_tables(const model& m)
 	put(oba.data(), oba.size() * 2);
 	while (t.image.size() % 16)
 		t.image.push_back(0);
+	// per observed node {up, down, up*log2(up), down*log2(down)}: the entropy
+	// term of a nonzero rate without a log per step
+	t.obs_tab_off = (int)t.image.size();
+	for (int o : obs)
+	{
+		const mbg_wnode& w = t.nodes[o];
+		const float v[4] = { w.up, w.down, mbg_rlog2r(w.up), mbg_rlog2r(w.down) };
+		put(v, sizeof(v));
+	}
 	return t;
 }
 
