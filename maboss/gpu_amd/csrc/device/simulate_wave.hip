@@ -8,13 +8,14 @@
 // is (state == logic) ? 0 : (state ? down : up), the reference's fast form
 // (src/generator.cpp:93-107).
 //
-// Rates live in LDS (one [npl][64] array per wave).  After a flip only the
-// flipped node and its children are re-evaluated, lanes taking one of them
-// each.  Every step each lane re-sums its slice sequentially, a Kogge-Stone
-// scan over the 64 lanes gives the total and the prefix of every slice, and
-// a ballot picks the slice holding u2*total (src/jit_kernels/simulation.cu:
-// 9-21 selects by prefix; here the canonical order is slice-then-scan, and
-// zero-rate nodes are never chosen).  Time, entropy and the window, final
+// Rates live in LDS (one [rows][64] array per wave, rows = npl rounded up to
+// a power of two, the extra rows zero).  After a flip only the flipped node
+// and its children are re-evaluated, lanes taking one of them each.  Every
+// step each lane re-sums its slice sequentially, a DPP scan over the 64 lanes
+// (row_scan) gives the total and the prefix of every slice, a ballot picks
+// the slice holding u2*total, and the slice's own running sums pick the node
+// (src/jit_kernels/simulation.cu:9-21 selects by prefix; here the canonical
+// order is slice-then-scan, and zero-rate nodes are never chosen).  Time, entropy and the window, final
 // and fixed-point statistics follow src/jit_kernels/simulation.cu:106-163 and
 // src/jit_kernels/window_average_small.cu:50-136, as in the lane kernel.
 //
@@ -59,16 +60,38 @@ __device__ __forceinline__ float node_rate(const mbg_wnode* nodes, int i, bool v
 	return (!valid || on == logic) ? 0.0f : (on ? nd.down : nd.up);
 }
 
-__device__ __forceinline__ float ks_scan(float v, int lane)
+// v + (DPP-moved value of another lane, or 0 where the move has no source or
+// the row is masked off): one v_add_f32_dpp
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float add_dpp(float v)
 {
-#pragma unroll
-	for (int off = 1; off < 64; off <<= 1)
-	{
-		const float y = __shfl_up(v, off);
-		if (lane >= off)
-			v = v + y;
-	}
+	return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+
+// Inclusive scan over the 64 lanes in the DPP order: Kogge-Stone inside each
+// row of 16 lanes (row_shr 1, 2, 4, 8), then rows 1 and 3 add the last lane
+// of the row before (row_bcast:15), then rows 2 and 3 add lane 31
+// (row_bcast:31).  Lanes without a term add 0.
+__device__ __forceinline__ float row_scan(float v)
+{
+	v = add_dpp<0x111, 0xF>(v);
+	v = add_dpp<0x112, 0xF>(v);
+	v = add_dpp<0x114, 0xF>(v);
+	v = add_dpp<0x118, 0xF>(v);
+	v = add_dpp<0x142, 0xA>(v);
+	v = add_dpp<0x143, 0xC>(v);
 	return v;
+}
+
+// Running sums of row 0: lane k (< 16) ends with ((0 + r_0) + r_1) + ... + r_k,
+// the sequential sum (each step moves the row right by one lane and adds r).
+__device__ __forceinline__ float row0_running_sum(float r)
+{
+	float q = 0.0f + r;
+#pragma unroll
+	for (int j = 1; j < 16; ++j)
+		q = r + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(q), 0x111, 0xF, 0xF, false));
+	return q;
 }
 
 template <int NPLMAX>
@@ -184,7 +207,7 @@ __device__ void wave_main(const mbg_sim_args& a)
 #pragma unroll
 			for (int s = 0; s < NPLMAX; ++s)
 				L = L + rates[s * 64 + lane];
-			const float P = ks_scan(L, lane);
+			const float P = row_scan(L);
 			const float total = rdlane_f(P, 63);
 
 			// uniforms of step k: one Philox block per lane per batch
@@ -232,8 +255,8 @@ __device__ void wave_main(const mbg_sim_args& a)
 					Sv = Sv + v;
 					Tv = Tv + (v == 0.0f ? 0.0f : (v == ot.x ? ot.z : ot.w));
 				}
-				const float S = rdlane_f(ks_scan(Sv, lane), 63);
-				const float T = rdlane_f(ks_scan(Tv, lane), 63);
+				const float S = rdlane_f(row_scan(Sv), 63);
+				const float T = rdlane_f(row_scan(Tv), 63);
 				const int cnt = __popcll(__ballot(v != 0.0f));
 				const float h = mbg_log2f(S) - T / S;
 				H = (cnt > 1 && h > 0.0f) ? h : 0.0f;
@@ -267,20 +290,32 @@ __device__ void wave_main(const mbg_sim_args& a)
 			const unsigned long long c2 = __ballot(L > 0.0f);
 			// c2 is nonzero whenever total > 0; the fallback only guards NaN rates
 			const int ls = c1 ? __ffsll(c1) - 1 : (c2 ? 63 - __clzll(c2) : 0);
-			const float prevP = __shfl_up(P, 1);
-			const float rem = rr - (lane == 0 ? 0.0f : prevP);
-			float qsum = 0.0f;
-			int ks = -1, last = -1;
-#pragma unroll
-			for (int s = 0; s < NPLMAX; ++s)
+			const float rem = rr - (ls == 0 ? 0.0f : rdlane_f(P, ls - 1));
+			int kstar;
+			if (NPLMAX <= 16)
 			{
-				const float rs = rates[s * 64 + lane];
-				qsum = qsum + rs;
-				const bool pos = rs > 0.0f;
-				last = pos ? s : last;
-				ks = (pos && ks < 0 && qsum >= rem) ? s : ks;
+				// the selected slice on lanes 0..15 of row 0, running sums by DPP
+				const float rs = lane < NPLMAX ? rates[lane * 64 + ls] : 0.0f;
+				const float q = row0_running_sum(rs);
+				const bool pos = lane < 16 && rs > 0.0f;
+				const uint64_t hit = __ballot(pos && q >= rem), any = __ballot(pos);
+				kstar = hit ? __ffsll((unsigned long long)hit) - 1 : (any ? 63 - __clzll(any) : -1);
 			}
-			const int kstar = (int)rdlane_u((uint32_t)(ks >= 0 ? ks : last), ls);
+			else
+			{
+				float qsum = 0.0f;
+				int ks = -1, last = -1;
+#pragma unroll
+				for (int s = 0; s < NPLMAX; ++s)
+				{
+					const float rs = rates[s * 64 + lane];
+					qsum = qsum + rs;
+					const bool pos = rs > 0.0f;
+					last = pos ? s : last;
+					ks = (pos && ks < 0 && qsum >= rem) ? s : ks;
+				}
+				kstar = (int)rdlane_u((uint32_t)(ks >= 0 ? ks : last), ls);
+			}
 			const int f = min(max(ls * npl + kstar, 0), N - 1);
 
 			if (lane == (f >> 5))

@@ -788,20 +788,30 @@ struct Sim
 	// Wave order: the summation order of the wave-per-trajectory kernel.
 	// Node i belongs to lane i / npl (64 lanes, npl = ceil(N / 64) nodes
 	// each); lane sums are sequential over the lane's slice and combined by
-	// an inclusive Kogge-Stone scan (offsets 1, 2, 4, ..., 32).
+	// the inclusive scan of lane_scan.
 	bool wave = false;
 	int npl = 1;
 	uint32_t seed_lo = 0, seed_hi = 0, variant = 0, max_steps = 1u << 30;
 
-	static void ks_scan(float* v)
+	// The wave kernel's scan over 64 lanes in four rows of 16: a Kogge-Stone
+	// scan inside each row (offsets 1, 2, 4, 8; lanes with no partner add 0),
+	// then rows 1 and 3 add the last lane of the row before, then rows 2 and 3
+	// add lane 31 (every other lane adds 0 in those two steps).
+	static void lane_scan(float* v)
 	{
-		for (int off = 1; off < 64; off <<= 1)
+		float nv[64];
+		for (int off = 1; off < 16; off <<= 1)
 		{
-			float nv[64];
 			for (int l = 0; l < 64; l++)
-				nv[l] = l >= off ? v[l] + v[l - off] : v[l];
+				nv[l] = v[l] + ((l % 16) >= off ? v[l - off] : 0.0f);
 			std::memcpy(v, nv, sizeof(nv));
 		}
+		for (int l = 0; l < 64; l++)
+			nv[l] = v[l] + ((l / 16) % 2 == 1 ? v[16 * (l / 16) - 1] : 0.0f);
+		std::memcpy(v, nv, sizeof(nv));
+		for (int l = 0; l < 64; l++)
+			nv[l] = v[l] + (l >= 32 ? v[31] : 0.0f);
+		std::memcpy(v, nv, sizeof(nv));
 	}
 
 	// Total rate; in wave order also the lane sums L and scan P.
@@ -824,7 +834,7 @@ struct Sim
 			}
 			L[l] = P[l] = acc;
 		}
-		ks_scan(P);
+		lane_scan(P);
 		return P[63];
 	}
 
@@ -921,7 +931,7 @@ struct Sim
 		if (wave)
 		{
 			// observed node k contributes to lane k % 64 (sequentially), then
-			// Kogge-Stone scans of the lane values
+			// lane_scan of the lane values
 			float Sl[64], Tl[64];
 			for (int l = 0; l < 64; l++)
 			{
@@ -935,8 +945,8 @@ struct Sim
 					nnz += v != 0.0f;
 				}
 			}
-			ks_scan(Sl);
-			ks_scan(Tl);
+			lane_scan(Sl);
+			lane_scan(Tl);
 			S = Sl[63];
 			T = Tl[63];
 			const float h = log2_poly(S) - T / S;

@@ -94,6 +94,17 @@ def test_wave_kernel_bit_exact(tmp_path, wl, name, n):
     assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=n, extra=("--order", "wave")))
 
 
+def test_wave_kernel_bit_exact_32_rows(tmp_path):
+    """1500 nodes: 24 nodes per lane, a 32-row rate array per wave and the
+    per-lane selection loop (the 16-row kernels select with DPP running sums)."""
+    net = netgen.scale_free_network(7, 1500, 2.5, 4, 12, 0.0, 1.0, 0.4)
+    net.constants = dict(sample_count=100, max_time=4, time_tick=0.5, discrete_time=0, seed_pseudorandom=7)
+    bnd, cfg = net.write(str(tmp_path / "sf1500"))
+    g, r = gpu_raw(tmp_path, bnd, cfg, kernel="wave", n_trajectories=100)
+    assert r.info.kernel == 1 and r.info.transitions > 10000
+    assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=100, extra=("--order", "wave")))
+
+
 def test_wave_kernel_reference_model(tmp_path):
     bnd, cfg = model_paths("cellcycle")
     g, _ = gpu_raw(tmp_path, bnd, cfg, kernel="wave", n_trajectories=20000)
