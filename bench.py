@@ -214,8 +214,15 @@ def main():
     bytes_per = 8 * W + 16
     ops_per = 2 * info.total_literals + 4 * N + 5 * nobs + 40
     achieved = bytes_per * per_step / kernel_s / 1e9
+    if ri.kernel == 1:  # wave per trajectory: rate rows = nodes per lane rounded up to a power of two
+        rows = 1
+        while rows < (N + 63) // 64:
+            rows *= 2
+        kname = f"simulate_wave_n{rows}"
+    else:
+        kname = "simulate"
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"simulate_{a.workload}_traffic.md")
+    tpath = os.path.join(ROOT, "profiles", f"{kname}_{a.workload}_traffic.md")
     if os.path.exists(tpath):
         with open(tpath) as f:
             for line in f:
@@ -240,7 +247,7 @@ def main():
                    "form": "plain", "transitions_per_step": per_step},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "simulate", "kernel_ms": kernel_s * 1e3, "device_path_ms": path_s * 1e3,
+                     "kernel": kname, "kernel_ms": kernel_s * 1e3, "device_path_ms": path_s * 1e3,
                      "bytes_per_transition": bytes_per,
                      "note": "effective: the fused kernel writes no records; algorithmic bytes are the "
                              "reference's record write + read, 8W+16 per transition"},
