@@ -86,32 +86,30 @@ extern "C" __global__ void __launch_bounds__(256) simulate_grad(mbg_grad_args a)
 	double C = 0.0, gw = 0.0; // completed windows' G; G of the window in progress
 	uint64_t ntrans = 0;
 
+	mbg_dev::id_chunks queue;
 	for (;;)
 	{
 		const uint64_t need = __ballot(!active && !exhausted);
 		if (need)
 		{
-			const int leader = __ffsll((unsigned long long)need) - 1;
-			uint64_t base = 0;
-			if (lane == leader)
-				base = atomicAdd(a.counters + MBG_CTR_QUEUE, (unsigned long long)__popcll(need));
-			base = mbg_dev::bcast64(base, leader);
+			const uint64_t mine = queue.take(need, a.counters + MBG_CTR_QUEUE, a.n_traj, 64);
+			bool fresh = false;
 			if (!active && !exhausted)
 			{
-				const uint64_t mine = base + __popcll(need & ((1ull << lane) - 1ull));
 				if (mine < a.n_traj)
 				{
 					id = a.id_base + mine;
-					mbg_dev::init_state<W>(s, id, a.variant, a.seed_lo, a.seed_hi, a.n_nodes, a.istate);
 					t = 0.0f;
 					k = 0;
 					C = 0.0;
 					gw = 0.0;
 					active = true;
+					fresh = true;
 				}
 				else
 					exhausted = true;
 			}
+			mbg_dev::init_states_wave<W>(s, fresh, id, a.variant, a.seed_lo, a.seed_hi, a.n_nodes, a.istate);
 		}
 		if (__ballot(active) == 0)
 			break;

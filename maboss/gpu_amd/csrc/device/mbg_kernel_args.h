@@ -86,6 +86,7 @@ struct mbg_sim_args
 	// counted in LDS and reach the global table once per block.
 	int tail_lds_off, tail_lds_slots; // u64 offset, slots {key + 1, count} (power of two; 0: none)
 	int fp_lds_off, fp_lds_slots;     // u64 offset, slots of (2 + key words) u32 (power of two; 0: none)
+	int fin_lds_cache_off, fin_lds_cache_slots; // hashed final states: u64 offset, slots {key + 1, count}
 	unsigned long long* counters;   // [MBG_NCOUNTERS]
 	unsigned long long* pair_keys;  // MBG_HIST_SORT: appended cell keys
 	unsigned long long* pair_vals;  //                and their values

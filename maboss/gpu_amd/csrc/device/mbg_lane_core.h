@@ -103,6 +103,48 @@ __device__ void init_states_wave(uint32_t (&s)[W], bool take, uint64_t id, uint3
 	}
 }
 
+// Trajectory indices for the lanes of a wave from a queue shared by all
+// waves, reserved in chunks: the wave keeps its current chunk [next, end)
+// (wave-uniform) and takes one atomic per chunk instead of one per refill.
+// Chunks are guided: about 1/4 of the wave's share of what the queue had
+// left at its last reservation, at most max_chunk, so waves run out of work
+// at nearly the same time.  Lanes in `need` receive consecutive indices in
+// lane order; an index at or beyond n_total means the queue is drained.
+// Every lane must call.
+struct id_chunks
+{
+	uint64_t next = 0, end = 0;
+
+	__device__ uint64_t take(uint64_t need, unsigned long long* queue, uint64_t n_total, uint64_t max_chunk)
+	{
+		const int lane = __lane_id();
+		const uint64_t n = (uint64_t)__popcll(need);
+		const uint64_t r = (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
+		const uint64_t avail = end - next;
+		uint64_t mine = next + r;
+		if (n > avail)
+		{
+			const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+			const uint64_t left = end < n_total ? n_total - end : 0;
+			uint64_t chunk = left / (4 * waves);
+			chunk = chunk > max_chunk ? max_chunk : chunk;
+			const uint64_t want = n - avail > chunk ? n - avail : chunk;
+			const int leader = __ffsll((unsigned long long)need) - 1;
+			uint64_t base = 0;
+			if (lane == leader)
+				base = atomicAdd(queue, (unsigned long long)want);
+			base = bcast64(base, leader);
+			if (r >= avail)
+				mine = base + (r - avail);
+			next = base + (n - avail);
+			end = base + want;
+		}
+		else
+			next += n;
+		return mine;
+	}
+};
+
 // u1 (dwell) and u2 (flip) of step k; discrete time draws u2 only
 __device__ __forceinline__ void step_uniforms(bool discrete, uint32_t B0, uint32_t k, uint64_t id, uint32_t variant,
 											  uint32_t seed_lo, uint32_t seed_hi, float& u1, float& u2)

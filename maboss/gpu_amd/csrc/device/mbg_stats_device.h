@@ -385,7 +385,7 @@ struct lane_hist
 			atomicAdd(lds + a.fin_lds_off + obs, 1ull);
 		else if (a.fin_mode == MBG_HIST_DENSE)
 			atomicAdd(a.fin_vals + obs, 1ull);
-		else
+		else if (!a.fin_lds_cache_slots || !lds_count_add(lds + a.fin_lds_cache_off, a.fin_lds_cache_slots, obs, 1ull))
 			hash_add(a.fin_table, a.fin_mask, obs, 1ull, a.counters + MBG_CTR_FIN_OVERFLOW);
 	}
 };
@@ -435,6 +435,13 @@ __device__ __forceinline__ void flush_block_hist(unsigned long long* lds, const 
 		if (k1)
 			hash_add(a.tail_table, a.tail_mask, k1 - 1ull, lds[a.tail_lds_off + 2 * i + 1],
 					 a.counters + MBG_CTR_TAIL_OVERFLOW);
+	}
+	for (int i = threadIdx.x; i < a.fin_lds_cache_slots; i += blockDim.x)
+	{
+		const unsigned long long k1 = lds[a.fin_lds_cache_off + 2 * i];
+		if (k1)
+			hash_add(a.fin_table, a.fin_mask, k1 - 1ull, lds[a.fin_lds_cache_off + 2 * i + 1],
+					 a.counters + MBG_CTR_FIN_OVERFLOW);
 	}
 }
 

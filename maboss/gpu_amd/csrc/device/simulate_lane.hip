@@ -54,6 +54,8 @@ namespace
 {
 
 constexpr int W = MBG_KW;
+// trajectory ids a wave reserves at a time (one queue atomic per chunk)
+constexpr uint64_t k_id_chunk = 64;
 
 // Wave-reduced counter update: transitions, finished and truncated
 // trajectories go to their three destinations.
@@ -119,21 +121,17 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 	pending pend;
 	uint64_t ntrans = 0, nfinished = 0, ntrunc = 0;
 
+	id_chunks queue;
 	for (;;)
 	{
-		// Refill lanes whose trajectory ended: one atomic per wave.
+		// Refill lanes whose trajectory ended, from the wave's chunk of ids.
 		const uint64_t need = __ballot(!active && !exhausted);
 		if (need)
 		{
-			const int leader = __ffsll((unsigned long long)need) - 1;
-			uint64_t base = 0;
-			if (lane == leader)
-				base = atomicAdd(a.counters + MBG_CTR_QUEUE, (unsigned long long)__popcll(need));
-			base = bcast64(base, leader);
+			const uint64_t mine = queue.take(need, a.counters + MBG_CTR_QUEUE, a.n_traj, k_id_chunk);
 			bool fresh = false;
 			if (!active && !exhausted)
 			{
-				const uint64_t mine = base + __popcll(need & ((1ull << lane) - 1ull));
 				if (mine < a.n_traj)
 				{
 					id = a.id_base + mine;

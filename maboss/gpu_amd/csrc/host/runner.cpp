@@ -77,6 +77,7 @@ struct plan
 	int win_mode, fin_mode, ent_lds;
 	int win_off = 0, fin_off = 0, ent_off = 0, lds_bytes = 0;
 	int tail_off = 0, tail_slots = 0, fp_off = 0, fp_slots = 0; // LDS caches
+	int fin_cache_off = 0, fin_cache_slots = 0;
 	int pair_off = 0;                                            // per-wave pair chunk cursors
 	int stage_off = 0, stage = 0;                                // per-wave pair staging (lane kernel)
 	uint64_t win_cells = 0, fin_cells = 0, win_cap = 0, fin_cap = 0, fp_cap = 0;
@@ -166,6 +167,14 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 		p.fp_slots = cache_slots(k_lds_budget - lds, slot, 128, 16);
 		p.fp_off = (int)(lds / 8);
 		lds += ((size_t)p.fp_slots * slot + 7) & ~(size_t)7;
+	}
+	// block-private cache in front of a hashed final-state table: trajectories
+	// that end in the same state add to one LDS slot
+	if (p.fin_mode == MBG_HIST_HASH)
+	{
+		p.fin_cache_slots = cache_slots(k_lds_budget - lds, 16, wave_kernel ? 128 : 512, 64);
+		p.fin_cache_off = (int)(lds / 8);
+		lds += (size_t)p.fin_cache_slots * 16;
 	}
 	p.lds_bytes = (int)lds;
 
@@ -363,6 +372,8 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		a.tail_lds_slots = tails ? pl.tail_slots : 0;
 		a.fp_lds_off = pl.fp_off;
 		a.fp_lds_slots = pl.fp_slots;
+		a.fin_lds_cache_off = pl.fin_cache_off;
+		a.fin_lds_cache_slots = pl.fin_cache_slots;
 		a.pair_keys = pk;
 		a.pair_vals = pv;
 		a.pair_cap = pair_cap;
