@@ -78,4 +78,7 @@ extern "C" __device__ void mbg_eval_p(const uint32_t* __restrict__ s, float u2, 
 // Parallel bit-extract of the observed nodes, in node order (bit j of the
 // result is the j-th non-internal node).
 extern "C" __device__ uint64_t mbg_obs_index(const uint32_t* __restrict__ s);
+// The bit of node i in the observed index (0 for an internal node): flipping
+// node i toggles it.
+extern "C" __device__ uint64_t mbg_obs_flip(int i);
 #endif

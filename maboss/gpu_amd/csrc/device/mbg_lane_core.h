@@ -284,6 +284,9 @@ __device__ __forceinline__ void record(bool discrete, float tp, float t, uint64_
 		}
 		w++;
 		b = en;
+		// once the tail is recorded, the later windows only add entropy: none when H is 0
+		if (tail_done && H == 0.0f)
+			break;
 	}
 }
 

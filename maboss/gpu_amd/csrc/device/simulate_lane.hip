@@ -116,7 +116,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 		s[w] = 0u;
 	float t = 0.0f;
 	uint32_t k = 0;
-	uint64_t id = 0;
+	uint64_t id = 0, obs = 0; // obs: observed index of s, kept up to date at each flip
 	bool active = false, exhausted = false;
 	pending pend;
 	uint64_t ntrans = 0, nfinished = 0, ntrunc = 0;
@@ -144,6 +144,8 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 					exhausted = true;
 			}
 			init_states_wave<W>(s, fresh, id, a.variant, a.seed_lo, a.seed_hi, a.n_nodes, a.istate);
+			if (fresh)
+				obs = mbg_obs_index(s);
 		}
 		if (__ballot(active) == 0)
 			break;
@@ -159,7 +161,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 			t = advance_time(a.discrete, t, e.total, u1, a.tick, a.max_time, fixed);
 			const float H = fixed ? 0.0f : e.H;
 			ntrans++;
-			record(a.discrete, tp, t, mbg_obs_index(s), H, a.tick, a.qscale, a.windows, pend, fw, fe, ft,
+			record(a.discrete, tp, t, obs, H, a.tick, a.qscale, a.windows, pend, fw, fe, ft,
 				   use_tails && t >= a.max_time);
 			k++;
 			if (t >= a.max_time)
@@ -177,6 +179,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 #pragma unroll
 				for (int w = 0; w < W; ++w)
 					s[w] ^= ((e.flip >> 5) == w) ? (1u << (e.flip & 31)) : 0u;
+				obs ^= mbg_obs_flip(e.flip);
 			}
 		}
 
@@ -184,7 +187,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 		if (__ballot(ended))
 		{
 			if (ended && t >= a.max_time)
-				hist.fin(mbg_obs_index(s));
+				hist.fin(obs);
 			uint64_t fm = __ballot(ended && fixed);
 			while (fm)
 			{
@@ -256,7 +259,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 		const uint64_t end = ch.end;
 		float t = 0.0f;
 		uint32_t k = 0;
-		uint64_t id = 0;
+		uint64_t id = 0, obs = 0; // obs: observed index of s, kept up to date at each flip
 		bool active = false;
 		pending pend;
 		uint64_t ntrans = 0, nfinished = 0, ntrunc = 0;
@@ -277,6 +280,8 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 					fresh = true;
 				}
 				init_states_wave<W>(s, fresh, id, v, a.seed_lo, a.seed_hi, a.n_nodes, istate);
+				if (fresh)
+					obs = mbg_obs_index(s);
 				next = min(end, next + (uint64_t)__popcll(need));
 			}
 			if (__ballot(active) == 0)
@@ -293,8 +298,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 				t = advance_time(a.discrete, t, e.total, u1, a.tick, max_time, fixed);
 				const float H = fixed ? 0.0f : e.H;
 				ntrans++;
-				record(a.discrete, tp, t, mbg_obs_index(s), H, a.tick, a.qscale, windows, pend, fw, fe,
-					   [](int, uint64_t) {}, false);
+				record(a.discrete, tp, t, obs, H, a.tick, a.qscale, windows, pend, fw, fe, [](int, uint64_t) {}, false);
 				k++;
 				if (t >= max_time)
 				{
@@ -311,13 +315,14 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 #pragma unroll
 					for (int w = 0; w < W; ++w)
 						s[w] ^= ((e.flip >> 5) == w) ? (1u << (e.flip & 31)) : 0u;
+					obs ^= mbg_obs_flip(e.flip);
 				}
 			}
 
 			if (__ballot(ended))
 			{
 				if (ended && t >= max_time)
-					atomicAdd(a.fin_vals + fin_off + mbg_obs_index(s), 1ull);
+					atomicAdd(a.fin_vals + fin_off + obs, 1ull);
 				uint64_t fm = __ballot(ended && fixed);
 				while (fm)
 				{

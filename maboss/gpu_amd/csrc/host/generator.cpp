@@ -667,7 +667,32 @@ void emit_obs_index(std::ostringstream& os, const model& m)
 		os << "\tv |= (uint64_t)((s[" << i0 / 32 << "] >> " << i0 % 32 << ") & " << mask << "u) << " << j << ";\n";
 		j = k;
 	}
-	os << "\treturn v;\n}\n";
+	os << "\treturn v;\n}\n\n";
+	// the observed-index bit of node i: word masks of the observed nodes and
+	// the number of observed nodes in the words before
+	const int W = m.n_words(), N = m.n_nodes();
+	std::vector<bool> is_obs(N, false);
+	for (int i : obs)
+		is_obs[i] = true;
+	os << "extern \"C\" __device__ uint64_t mbg_obs_flip(int i)\n{\n";
+	os << "\tconst int w = i >> 5, b = i & 31;\n";
+	int before = 0;
+	for (int w = 0; w < W; w++)
+	{
+		uint32_t om = 0;
+		for (int i = w * 32; i < std::min(N, w * 32 + 32); i++)
+			if (is_obs[i])
+				om |= 1u << (i % 32);
+		char buf[128];
+		if (w == 0)
+			std::snprintf(buf, sizeof(buf), "\tuint32_t om = 0x%08xu;\n\tint base = 0;\n", om);
+		else
+			std::snprintf(buf, sizeof(buf), "\tom = (w == %d) ? 0x%08xu : om;\n\tbase = (w == %d) ? %d : base;\n", w, om,
+						  w, before);
+		os << buf;
+		before += __builtin_popcount(om);
+	}
+	os << "\treturn ((om >> b) & 1u) ? 1ull << (base + __builtin_popcount(om & ((1u << b) - 1u))) : 0ull;\n}\n";
 }
 
 } // namespace
