@@ -290,4 +290,95 @@ __device__ __forceinline__ void record(bool discrete, float tp, float t, uint64_
 	}
 }
 
+// record() for the whole wave at once: every lane calls (on: the lane has a
+// record), and the windows of the records are visited in a loop that is
+// uniform across the wave, so the window cell a lane completes in an
+// iteration reaches emit(has, w, obs, q) at a point where the whole wave is
+// active (emit can then stage it with wave-wide operations).  Same cells,
+// values and order per lane as record().
+template <typename EMIT, typename FE, typename FT>
+__device__ __forceinline__ void record_wave(bool on, bool discrete, float tp, float t, uint64_t obs, float H, float tick,
+											float qscale, int windows, pending& p, EMIT&& emit, FE&& fe, FT&& ft,
+											bool final_tail)
+{
+	// the lane's pending cell when the record moves to another one
+	auto merge = [&](int w, uint64_t q, bool& has, int& ow, uint64_t& oo, uint64_t& oq) {
+		if (w == p.pw && obs == p.pobs)
+			p.pq += q;
+		else
+		{
+			if (p.pw >= 0)
+			{
+				has = true;
+				ow = p.pw;
+				oo = p.pobs;
+				oq = p.pq;
+			}
+			p.pw = w;
+			p.pobs = obs;
+			p.pq = q;
+		}
+	};
+	if (discrete)
+	{
+		bool has = false;
+		int ow = 0;
+		uint64_t oo = 0, oq = 0;
+		if (on)
+		{
+			const int w = (int)roundf(tp / tick);
+			if (w < windows)
+			{
+				merge(w, 1ull, has, ow, oo, oq);
+				const uint64_t qe = (uint64_t)rintf(H * qscale);
+				if (qe)
+					p.ent(w, qe, fe);
+			}
+		}
+		emit(has, ow, oo, oq);
+		return;
+	}
+	int w = on ? (int)floorf(tp / tick) : 0;
+	const int w_first = w;
+	bool tail_done = false;
+	float b = tp;
+	bool more = on && t > b;
+	while (__ballot(more))
+	{
+		bool has = false;
+		int ow = 0;
+		uint64_t oo = 0, oq = 0;
+		if (more)
+		{
+			const float wend = (float)(w + 1) * tick;
+			const float en = fminf(t, wend);
+			const float sl = en - b;
+			if (w < windows)
+			{
+				if (final_tail && w > w_first)
+				{
+					if (!tail_done)
+					{
+						ft(w, obs);
+						tail_done = true;
+					}
+				}
+				else
+				{
+					const uint64_t q = (uint64_t)rintf(sl * qscale);
+					if (q)
+						merge(w, q, has, ow, oo, oq);
+				}
+				const uint64_t qe = (uint64_t)rintf((H * sl) * qscale);
+				if (qe)
+					p.ent(w, qe, fe);
+			}
+			w++;
+			b = en;
+			more = t > b && !(tail_done && H == 0.0f);
+		}
+		emit(has, ow, oo, oq);
+	}
+}
+
 } // namespace mbg_dev

@@ -149,6 +149,62 @@ __device__ __forceinline__ void append_pair_staged(unsigned long long* cur, unsi
 		append_pair_chunked(cur, keys, vals, cap, counter, chunk, key, q);
 }
 
+// Window pairs of one wave staged in LDS with the fill count in a register:
+// add() is called by the whole wave (lanes with has set contribute a pair);
+// when 64 or more pairs are staged, 64 of them go to HBM as whole lines and
+// the rest move down, so up to 64 new pairs always fit (capacity 128).
+struct pair_stage_reg
+{
+	unsigned long long* keys; // [cap] then values [cap]
+	int cap;
+	unsigned int n = 0;
+
+	__device__ void out64(unsigned long long* cur, const mbg_sim_args& a, unsigned int take)
+	{
+		const unsigned int lane = (unsigned int)__lane_id();
+		if (lane < take)
+			append_pair_chunked(cur, a.pair_keys, a.pair_vals, a.pair_cap, a.counters + MBG_CTR_PAIRS,
+								(uint64_t)a.pair_chunk, keys[lane], keys[cap + lane]);
+		unsigned long long k = 0, v = 0;
+		const unsigned int src = take + lane;
+		if (src < n)
+		{
+			k = keys[src];
+			v = keys[cap + src];
+		}
+		__builtin_amdgcn_wave_barrier();
+		if (src < n)
+		{
+			keys[lane] = k;
+			keys[cap + lane] = v;
+		}
+		__builtin_amdgcn_wave_barrier();
+		n -= take;
+	}
+
+	__device__ void add(unsigned long long* cur, const mbg_sim_args& a, bool has, uint64_t key, uint64_t q)
+	{
+		const uint64_t m = __ballot(has);
+		if (!m)
+			return;
+		if (n >= 64u)
+			out64(cur, a, 64u);
+		if (has)
+		{
+			const unsigned int i = n + (unsigned int)__popcll(m & ((1ull << __lane_id()) - 1ull));
+			keys[i] = key;
+			keys[cap + i] = q;
+		}
+		n += (unsigned int)__popcll(m);
+	}
+
+	__device__ void drain(unsigned long long* cur, const mbg_sim_args& a)
+	{
+		while (n > 0u)
+			out64(cur, a, n < 64u ? n : 64u);
+	}
+};
+
 // Convergent point of the whole wave: write staged pairs out, 64 per round
 // (all of them when final).
 __device__ __forceinline__ void flush_pair_stage(unsigned long long* cur, unsigned long long* stage,

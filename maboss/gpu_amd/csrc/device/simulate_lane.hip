@@ -104,11 +104,22 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 
 	const lane_hist hist { lds, a };
 	const int lane = __lane_id();
+	const int wave = threadIdx.x >> 6;
 	const uint32_t B0 = ((uint32_t)a.n_nodes + 3u) >> 2;
-	auto fw = [&](int w, uint64_t o, uint64_t q) { hist.win(w, o, q); };
 	auto fe = [&](int w, uint64_t q) { hist.ent(w, q); };
 	auto ft = [&](int w, uint64_t o) { hist.tail(w, o); };
 	const bool use_tails = a.tail_table != nullptr;
+	// window cells completed by the wave's records: staged pairs (sorted
+	// mode) or straight into the histogram
+	unsigned long long* cur = lds + a.pair_lds_off + 4 * wave;
+	const bool staged = a.win_mode == MBG_HIST_SORT && a.pair_stage;
+	pair_stage_reg stage { lds + a.pair_stage_off + 2 * a.pair_stage * wave, a.pair_stage };
+	auto emit = [&](bool has, int w, uint64_t o, uint64_t q) {
+		if (staged)
+			stage.add(cur, a, has, ((uint64_t)w << a.n_obs) | o, q);
+		else if (has)
+			hist.win(w, o, q);
+	};
 
 	uint32_t s[W];
 #pragma unroll
@@ -151,18 +162,22 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 			break;
 
 		bool ended = false, fixed = false;
+		mbg_eval_t e;
+		float tp = 0.0f, H = 0.0f;
 		if (active)
 		{
 			float u1, u2;
 			step_uniforms(a.discrete, B0, k, id, a.variant, a.seed_lo, a.seed_hi, u1, u2);
-			mbg_eval_t e;
 			MBG_EVAL_STEP(s, u2, &e);
-			const float tp = t;
+			tp = t;
 			t = advance_time(a.discrete, t, e.total, u1, a.tick, a.max_time, fixed);
-			const float H = fixed ? 0.0f : e.H;
+			H = fixed ? 0.0f : e.H;
 			ntrans++;
-			record(a.discrete, tp, t, obs, H, a.tick, a.qscale, a.windows, pend, fw, fe, ft,
-				   use_tails && t >= a.max_time);
+		}
+		record_wave(active, a.discrete, tp, t, obs, H, a.tick, a.qscale, a.windows, pend, emit, fe, ft,
+					use_tails && t >= a.max_time);
+		if (active)
+		{
 			k++;
 			if (t >= a.max_time)
 			{
@@ -199,11 +214,14 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 			if (ended)
 				active = false;
 		}
-		flush_staged_pairs(lds, a, false);
 	}
 
-	pend.flush_all(fw, fe);
-	flush_staged_pairs(lds, a, true);
+	// the lanes' last pending cells, then the rest of the stage
+	emit(pend.pw >= 0, pend.pw, pend.pobs, pend.pq);
+	if (pend.pew >= 0)
+		fe(pend.pew, pend.peq);
+	if (staged)
+		stage.drain(cur, a);
 	wave_add_counters(a.counters + MBG_CTR_TRANSITIONS, a.counters + MBG_CTR_FINISHED, a.counters + MBG_CTR_TRUNCATED,
 					  ntrans, nfinished, ntrunc);
 
