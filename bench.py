@@ -18,7 +18,8 @@ vs launched one by one), ``grad`` (forward vs likelihood-ratio backward
 throughput), ``roofline`` (the simulate kernel against
 HBM with the reference's record traffic of 8W+16 bytes per transition),
 ``roofline_valu`` (against the VALU issue rate with the per-transition op
-count 2*literals + 4N + 5*n_obs + 40), ``cpu_baseline`` (the host oracle with
+count 2*literals + 4N + 5*n_obs + 40), ``roofline_rate`` (the kernel's
+transitions/s against min(HBM/B, VALU/C)), ``cpu_baseline`` (the host oracle with
 compiled formulas on this machine's cores, bounded sample), and
 ``host_inclusive`` (one step that also copies all statistics to the host).
 """
@@ -254,6 +255,11 @@ def main():
         "roofline_valu": {"achieved": ops_per * per_step / kernel_s / 1e12, "peak": VALU_PEAK_TOPS,
                           "unit": "T lane-ops/s", "ops_per_transition": ops_per,
                           "frac": ops_per * per_step / kernel_s / 1e12 / VALU_PEAK_TOPS},
+        # the combined bound R* = min(HBM / B, VALU / C) transitions/s and the kernel's share of it
+        "roofline_rate": {"bound_transitions_per_s": min(HBM_PEAK_GBS * 1e9 / bytes_per, VALU_PEAK_TOPS * 1e12 / ops_per),
+                          "kernel_transitions_per_s": per_step / kernel_s,
+                          "frac": per_step / kernel_s / min(HBM_PEAK_GBS * 1e9 / bytes_per,
+                                                            VALU_PEAK_TOPS * 1e12 / ops_per)},
         "host_inclusive": {"value": fi.transitions / host_s, "unit": "transitions/s", "seconds": host_s,
                            "window_cells": fi.n_window_cells},
         "compile_seconds": prog.compile_seconds,
