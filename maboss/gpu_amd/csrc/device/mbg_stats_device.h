@@ -116,39 +116,6 @@ __device__ __forceinline__ void append_pair_chunked(unsigned long long* cur, uns
 	}
 }
 
-// Staged append: pairs collect in a per-wave LDS buffer (keys[cap], then
-// values[cap]) and reach HBM 64 at a time as whole lines (flush_pair_stage);
-// when the buffer cannot take the active lanes' pairs they are written
-// directly.  stage_cnt is the wave's fill count in LDS.
-__device__ __forceinline__ void append_pair_staged(unsigned long long* cur, unsigned long long* stage,
-												   unsigned int* stage_cnt, int stage_cap, unsigned long long* keys,
-												   unsigned long long* vals, uint64_t cap, unsigned long long* counter,
-												   uint64_t chunk, uint64_t key, uint64_t q)
-{
-	const uint64_t m = __ballot(1);
-	const int leader = __ffsll((unsigned long long)m) - 1;
-	const int lane = __lane_id();
-	const unsigned int n = (unsigned int)__popcll(m);
-	unsigned int base = 0, fits = 0;
-	if (lane == leader)
-	{
-		base = *stage_cnt;
-		fits = base + n <= (unsigned int)stage_cap;
-		if (fits)
-			*stage_cnt = base + n;
-	}
-	base = __builtin_amdgcn_readlane(base, leader);
-	fits = __builtin_amdgcn_readlane(fits, leader);
-	if (fits)
-	{
-		const unsigned int i = base + (unsigned int)__popcll(m & ((1ull << lane) - 1ull));
-		stage[i] = key;
-		stage[stage_cap + i] = q;
-	}
-	else
-		append_pair_chunked(cur, keys, vals, cap, counter, chunk, key, q);
-}
-
 // Window pairs of one wave staged in LDS with the fill count in a register:
 // add() is called by the whole wave (lanes with has set contribute a pair);
 // when 64 or more pairs are staged, 64 of them go to HBM as whole lines and
@@ -204,45 +171,6 @@ struct pair_stage_reg
 			out64(cur, a, n < 64u ? n : 64u);
 	}
 };
-
-// Convergent point of the whole wave: write staged pairs out, 64 per round
-// (all of them when final).
-__device__ __forceinline__ void flush_pair_stage(unsigned long long* cur, unsigned long long* stage,
-												 unsigned int* stage_cnt, int stage_cap, unsigned long long* keys,
-												 unsigned long long* vals, uint64_t cap, unsigned long long* counter,
-												 uint64_t chunk, bool final)
-{
-	const int lane = __lane_id();
-	unsigned int c = __builtin_amdgcn_readfirstlane(*stage_cnt);
-	while (c >= 64u || (final && c > 0u))
-	{
-		const unsigned int take = c >= 64u ? 64u : c;
-		if ((unsigned int)lane < take)
-			append_pair_chunked(cur, keys, vals, cap, counter, chunk, stage[lane], stage[stage_cap + lane]);
-		// move the rest down (at most stage_cap - 64 entries, in rounds of 64)
-		for (unsigned int j = 0; j + take < c; j += 64u)
-		{
-			const unsigned int src = take + j + (unsigned int)lane;
-			unsigned long long k = 0, v = 0;
-			if (src < c)
-			{
-				k = stage[src];
-				v = stage[stage_cap + src];
-			}
-			__builtin_amdgcn_wave_barrier();
-			if (src < c)
-			{
-				stage[j + lane] = k;
-				stage[stage_cap + j + lane] = v;
-			}
-			__builtin_amdgcn_wave_barrier();
-		}
-		c -= take;
-	}
-	if (lane == 0)
-		*stage_cnt = c;
-	__builtin_amdgcn_wave_barrier();
-}
 
 // End of a wave's work: pad the rest of its chunk (all lanes of the wave).
 __device__ __forceinline__ void pad_pair_chunk(unsigned long long* cur, unsigned long long* keys,
@@ -379,7 +307,6 @@ __device__ bool fp_lds_insert(unsigned int* t, int nslots, const uint32_t (&key)
 }
 
 struct lane_hist;
-__device__ __forceinline__ void flush_staged_pairs(unsigned long long* lds, const mbg_sim_args& a, bool final);
 
 struct lane_hist
 {
@@ -394,17 +321,8 @@ struct lane_hist
 		else if (a.win_mode == MBG_HIST_DENSE)
 			atomicAdd(a.win_vals + cell, (unsigned long long)q);
 		else if (a.win_mode == MBG_HIST_SORT)
-		{
-			const int wave = threadIdx.x >> 6;
-			unsigned long long* cur = lds + a.pair_lds_off + 4 * wave;
-			if (a.pair_stage)
-				append_pair_staged(cur, lds + a.pair_stage_off + 2 * a.pair_stage * wave, (unsigned int*)(cur + 2),
-								   a.pair_stage, a.pair_keys, a.pair_vals, a.pair_cap, a.counters + MBG_CTR_PAIRS,
-								   (uint64_t)a.pair_chunk, cell, q);
-			else
-				append_pair_chunked(cur, a.pair_keys, a.pair_vals, a.pair_cap, a.counters + MBG_CTR_PAIRS,
-									(uint64_t)a.pair_chunk, cell, q);
-		}
+			append_pair_chunked(lds + a.pair_lds_off + 4 * (threadIdx.x >> 6), a.pair_keys, a.pair_vals, a.pair_cap,
+								a.counters + MBG_CTR_PAIRS, (uint64_t)a.pair_chunk, cell, q);
 		else
 			hash_add(a.win_table, a.win_mask, cell, q, a.counters + MBG_CTR_WIN_OVERFLOW);
 	}
@@ -445,18 +363,6 @@ struct lane_hist
 			hash_add(a.fin_table, a.fin_mask, obs, 1ull, a.counters + MBG_CTR_FIN_OVERFLOW);
 	}
 };
-
-// The calling wave's staged window pairs (sorted mode), at a point where
-// the whole wave is active.
-__device__ __forceinline__ void flush_staged_pairs(unsigned long long* lds, const mbg_sim_args& a, bool final)
-{
-	if (a.win_mode != MBG_HIST_SORT || !a.pair_stage)
-		return;
-	const int wave = threadIdx.x >> 6;
-	unsigned long long* cur = lds + a.pair_lds_off + 4 * wave;
-	flush_pair_stage(cur, lds + a.pair_stage_off + 2 * a.pair_stage * wave, (unsigned int*)(cur + 2), a.pair_stage,
-					 a.pair_keys, a.pair_vals, a.pair_cap, a.counters + MBG_CTR_PAIRS, (uint64_t)a.pair_chunk, final);
-}
 
 // End of a block: add its LDS histograms and caches to the global storage
 // (after a barrier).

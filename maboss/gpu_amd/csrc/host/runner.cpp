@@ -143,7 +143,7 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 	if (p.win_mode == MBG_HIST_SORT)
 	{
 		p.pair_off = (int)(lds / 8);
-		lds += 16 * 32; // {next, left, staged count} for up to 16 waves
+		lds += 16 * 32; // per-wave chunk cursor {next, left} (32 bytes) for up to 16 waves
 		if (!wave_kernel)
 		{
 			// 128 staged pairs per wave, blocks of at most 4 waves (the lane kernel's bound)
