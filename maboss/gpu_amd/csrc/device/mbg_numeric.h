@@ -36,11 +36,6 @@ struct mbg_u32x4
 	uint32_t x, y, z, w;
 };
 
-MBG_HD uint32_t mbg_mulhi32(uint32_t a, uint32_t b)
-{
-	return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
-}
-
 // Philox4x32 with 10 rounds.  Known-answer vectors from the Random123
 // distribution are checked in the test suite.
 MBG_HD mbg_u32x4 mbg_philox4x32_10(mbg_u32x4 c, uint32_t k0, uint32_t k1)
@@ -50,10 +45,10 @@ MBG_HD mbg_u32x4 mbg_philox4x32_10(mbg_u32x4 c, uint32_t k0, uint32_t k1)
 #endif
 	for (int r = 0; r < 10; ++r)
 	{
-		const uint32_t lo0 = MBG_PHILOX_M0 * c.x;
-		const uint32_t hi0 = mbg_mulhi32(MBG_PHILOX_M0, c.x);
-		const uint32_t lo1 = MBG_PHILOX_M1 * c.z;
-		const uint32_t hi1 = mbg_mulhi32(MBG_PHILOX_M1, c.z);
+		// one 32 x 32 -> 64-bit product per multiplier (a single v_mad_u64_u32)
+		const uint64_t p0 = (uint64_t)MBG_PHILOX_M0 * c.x, p1 = (uint64_t)MBG_PHILOX_M1 * c.z;
+		const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+		const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
 		mbg_u32x4 n;
 		n.x = hi1 ^ c.y ^ k0;
 		n.y = lo1;
