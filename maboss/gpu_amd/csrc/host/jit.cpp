@@ -250,12 +250,21 @@ hipFunction_t program::function(int flavour)
 
 	scope_timer tm("compiler> whole_compilation");
 	const double t0 = wall_seconds();
-	// Occupancy: models evaluated in two passes hold fewer values per node
-	// but more nodes; measured on MI355X, 3 waves per SIMD (170 VGPRs) beat
-	// the compiler's own choice (one wave, 256 VGPRs) by 1.4-1.5x for 50-100
-	// nodes and were neutral below 40 nodes.
+	// Occupancy (waves per SIMD; the compiler alone picks fewer waves with
+	// more registers each).  The kernel waits on LDS, shuffles and the queue
+	// more than it issues, so resident waves pay even with a few spilled
+	// registers.  Measured on MI355X (A/B in one run each): 3 waves beat 1 by
+	// 1.5x on the 100-node model and 2 or 4 to 6 waves at 50-100 nodes (4 was
+	// within noise of 3 on the continuous 100-node model); at 30 nodes 4 waves
+	// beat the compiler's choice by 15 %; above 160 nodes 2.
 	const int n_nodes = mdl->n_nodes();
-	const int waves = opts.lane_waves > 0 ? opts.lane_waves : n_nodes > 160 ? 2 : n_nodes > 40 ? 3 : 0;
+	int waves = 4;
+	if (opts.lane_waves > 0)
+		waves = opts.lane_waves;
+	else if (n_nodes > 160)
+		waves = 2;
+	else if (n_nodes > 40)
+		waves = 3;
 	const std::string src = "#define MBG_KW " + std::to_string(W) + "\n#define MBG_PARAM " + std::to_string(flavour)
 							+ "\n#define MBG_LANE_WAVES " + std::to_string(waves) + "\n" + gen.source + "\n"
 							+ mbg_src_lane;
