@@ -2,7 +2,7 @@
 
 One step = one pass of the simulate/statistics path over one batch: all
 trajectories of the workload simulated from their initial state to max_time,
-with window (sorted pairs), final-state and fixed-point statistics accumulated in
+with window (hashed cells), final-state and fixed-point statistics accumulated in
 HBM.  The model tables and statistics storage are resident on the device when
 the timed region starts; the timed region covers K steps (statistics storage
 cleared, persistent simulate kernel, counters read back), bracketed by
@@ -10,7 +10,7 @@ torch.cuda.synchronize().
 
 Default workload ``plain_100``: 100-node random network from seed 1 (4 state
 words, in-degree 1-4, AND/OR/NOT formulas), 50 internal nodes so the windows
-key on 50 observed bits (sorted window pairs), rates lognormal(0, 1), 1e6
+key on 50 observed bits (hashed window cells), rates lognormal(0, 1), 1e6
 trajectories, max_time 100, time_tick 1, continuous time.
 
 Prints one JSON line.  Extra fields: ``batched`` (256 variants in one launch
@@ -21,7 +21,9 @@ HBM with the reference's record traffic of 8W+16 bytes per transition),
 count 2*literals + 4N + 5*n_obs + 40), ``roofline_rate`` (the kernel's
 transitions/s against min(HBM/B, VALU/C)), ``cpu_baseline`` (the host oracle with
 compiled formulas on this machine's cores, bounded sample), and
-``host_inclusive`` (one step that also copies all statistics to the host).
+``host_inclusive`` (one step that also copies all statistics to the host), and
+``sorted_output`` (one step that leaves the window cells as a compact array sorted by
+key in HBM instead of the hash table).
 """
 from __future__ import annotations
 
@@ -41,7 +43,7 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz = 39.3 
 METRIC = "state transitions/sec on 1 MI355X (plain vs batched vs +grad) at matched stats"
 WORKLOAD_TEXT = {
     "plain_100": "100-node random network (seed 1, 4 state words, in-degree 1-4, AND/OR/NOT), 50 observed nodes "
-                 "(sorted window pairs), lognormal(0,1) rates, 1e6 trajectories, max_time 100, tick 1, continuous",
+                 "(hashed window cells), lognormal(0,1) rates, 1e6 trajectories, max_time 100, tick 1, continuous",
     "plain_100_discrete": "plain_100 network in discrete time",
     "plain_small": "10-node random network (seed 0), all observed, 1e4 trajectories, max_time 5, tick 0.2",
     "scale_1000": "1000-node scale-free network (seed 2, 32 state words, out-degree exponent 2.5, in-degree <= 4, "
@@ -203,13 +205,25 @@ def main():
     elapsed = time.perf_counter() - t0
     per_step = transitions / a.steps
     kernel_s = sim_ms / a.steps / 1e3  # the simulate kernel alone (HIP events on its stream)
-    path_s = kernel_ms / a.steps / 1e3  # simulate + tail expansion + window sort/reduce
+    path_s = kernel_ms / a.steps / 1e3  # simulate + tail expansion (+ window sort/reduce in the sorted mode)
 
-    # the same step with every statistic copied back to the host (PCIe + host sort)
+    # the same step with every statistic copied back to the host (PCIe, cells in key order)
     t1 = time.perf_counter()
     full = prog.simulate(n_trajectories=n)
     host_s = time.perf_counter() - t1
     fi = full.info
+
+    # the same step with the window cells left as a compact sorted array in HBM (sorted pairs)
+    sorted_out = None
+    if ri.window_mode == 1:
+        prog.simulate(n_trajectories=n, device_only=True, window_mode="sort")
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        so = prog.simulate(n_trajectories=n, device_only=True, window_mode="sort").info
+        torch.cuda.synchronize()
+        sorted_out = {"window_mode": "sort", "value": so.transitions / (time.perf_counter() - ts),
+                      "unit": "transitions/s", "device_path_ms": so.kernel_ms, "simulate_ms": so.simulate_ms,
+                      "window_pairs": so.window_pairs}
 
     W, N, nobs = info.n_words, info.n_nodes, info.n_observed
     bytes_per = 8 * W + 16
@@ -260,6 +274,7 @@ def main():
                           "kernel_transitions_per_s": per_step / kernel_s,
                           "frac": per_step / kernel_s / min(HBM_PEAK_GBS * 1e9 / bytes_per,
                                                             VALU_PEAK_TOPS * 1e12 / ops_per)},
+        "sorted_output": sorted_out,
         "host_inclusive": {"value": fi.transitions / host_s, "unit": "transitions/s", "seconds": host_s,
                            "window_cells": fi.n_window_cells},
         "compile_seconds": prog.compile_seconds,

@@ -115,7 +115,8 @@ typedef struct mbg_sim_params
 	int64_t seed;            /* <0: seed_pseudorandom of the model */
 	uint32_t variant;        /* stream tag (batched variant index) */
 	uint32_t max_steps;      /* 0: default cap on SSA steps per trajectory */
-	int32_t window_mode;     /* -1 auto, 0 dense HBM, 1 hash, 2 dense LDS, 3 sorted pairs */
+	int32_t window_mode;     /* -1 auto (dense LDS up to 32 KB of cells, dense HBM up to 8 GB,
+	                            else hash), 0 dense HBM, 1 hash, 2 dense LDS, 3 sorted pairs */
 	int32_t final_mode;      /* -1 auto, 0 dense HBM, 1 hash, 2 dense LDS */
 	uint64_t window_capacity;/* hash slots (power of two) or sorted-mode pairs, 0 = auto */
 	uint64_t final_capacity;
@@ -243,7 +244,7 @@ typedef struct mbg_result_info
 	uint64_t n_final_states;         /* distinct final observed states */
 	uint64_t n_window_cells;         /* nonzero (window, state) cells */
 	double kernel_ms;                /* device time of the whole device path (simulate kernel,
-	                                    tail expansion, window sort/reduce) */
+	                                    tail expansion; sorted mode: window sort/reduce) */
 	double total_ms;                 /* host wall time of mbg_simulate */
 	int32_t window_mode, final_mode;
 	uint64_t window_capacity;

@@ -73,3 +73,22 @@ extern "C" __global__ void __launch_bounds__(256) mbg_expand_tails(mbg_sim_args 
 		}
 	}
 }
+
+// The occupied cells of a hashed window table as (cell key, value) pairs in
+// the sorted mode's pair buffers (one wave-aggregated append per round), so
+// the pair sort hands the host the cells in key order.
+extern "C" __global__ void __launch_bounds__(256) mbg_table_pairs(mbg_sim_args a)
+{
+	const uint64_t cap = a.win_mask + 1;
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	// every lane runs the same number of rounds, so each append sees its whole wave
+	const uint64_t rounds = (cap + stride - 1) / stride;
+	for (uint64_t r = 0; r < rounds; ++r)
+	{
+		const uint64_t i = r * stride + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+		const unsigned long long key = i < cap ? a.win_table[2 * i] : MBG_EMPTY_KEY;
+		const unsigned long long val = i < cap ? a.win_table[2 * i + 1] : 0ull;
+		if (key != MBG_EMPTY_KEY && val != 0ull)
+			append_pair(a.pair_keys, a.pair_vals, a.pair_cap, a.counters + MBG_CTR_PAIRS, key, val);
+	}
+}

@@ -71,7 +71,7 @@ struct program
 
 	// utility kernels (slot fill, tail expansion), precompiled
 	hipModule_t mod_util = nullptr;
-	hipFunction_t fn_fill_slots = nullptr, fn_expand_tails = nullptr;
+	hipFunction_t fn_fill_slots = nullptr, fn_expand_tails = nullptr, fn_table_pairs = nullptr;
 	void load_util();
 	device_buffer b_tail;
 

@@ -221,6 +221,8 @@ void program::load_util()
 	hip_check(hipModuleGetFunction(&fn_expand_tails, mod_util, "mbg_expand_tails"),
 			  "hipModuleGetFunction(mbg_expand_tails)");
 	hip_check(hipModuleGetFunction(&fn_fill_u64, mod_util, "mbg_fill_u64"), "hipModuleGetFunction(mbg_fill_u64)");
+	hip_check(hipModuleGetFunction(&fn_table_pairs, mod_util, "mbg_table_pairs"),
+			  "hipModuleGetFunction(mbg_table_pairs)");
 }
 
 hipFunction_t program::wave_function()

@@ -111,7 +111,7 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 	if (p.win_mode < 0)
 		p.win_mode = win_dense_bytes <= 32 * 1024 ? MBG_HIST_LDS
 					 : win_dense_bytes <= k_dense_limit ? MBG_HIST_DENSE
-														: MBG_HIST_SORT;
+														: MBG_HIST_HASH;
 	if (p.win_mode < 0 || p.win_mode > MBG_HIST_SORT)
 		throw error(MBG_E_ARG, "unknown window mode");
 	if ((p.win_mode == MBG_HIST_DENSE || p.win_mode == MBG_HIST_LDS) && win_dense_bytes > k_dense_limit)
@@ -581,14 +581,42 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		}
 		else if (pl.win_mode == MBG_HIST_HASH)
 		{
-			std::vector<unsigned long long> tab(pl.win_cap * 2);
-			hip_check(hipMemcpy(tab.data(), win_keys, pl.win_cap * 16, hipMemcpyDeviceToHost), "hipMemcpy");
+			// the occupied cells, compacted and sorted by key on the device
+			auto* tk = (unsigned long long*)p.b_pk.ensure(pl.win_cap * 8);
+			auto* tv = (unsigned long long*)p.b_pv2.ensure(pl.win_cap * 8);
+			auto* ka = (unsigned long long*)p.b_pk_alt.ensure(pl.win_cap * 8);
+			auto* va = (unsigned long long*)p.b_pv_alt.ensure(pl.win_cap * 8);
+			a.pair_keys = tk;
+			a.pair_vals = tv;
+			a.pair_cap = pl.win_cap;
+			hip_check(hipMemsetAsync(counters + MBG_CTR_PAIRS, 0, 8, stream), "hipMemsetAsync");
+			{
+				void* args[] = { &a };
+				const unsigned g = (unsigned)std::min<uint64_t>((uint64_t)p.num_cus * 8, (pl.win_cap + 255) / 256);
+				hip_check(hipModuleLaunchKernel(p.fn_table_pairs, g, 1, 1, 256, 1, 1, 0, stream, args, nullptr),
+						  "hipModuleLaunchKernel(mbg_table_pairs)");
+			}
+			unsigned long long n_cells = 0;
+			hip_check(hipMemcpyAsync(&n_cells, counters + MBG_CTR_PAIRS, 8, hipMemcpyDeviceToHost, stream),
+					  "hipMemcpyAsync");
+			hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+			unsigned long long nu = 0, *hk = nullptr, *hv = nullptr;
+			if (n_cells)
+			{
+				auto* hcount = (unsigned long long*)p.b_ucount.ensure(8);
+				sort_reduce_pairs(p.b_sort_tmp, tk, tv, ka, va, n_cells, n_obs + bits_for((uint64_t)windows - 1),
+								  hcount, stream, &hk, &hv);
+				hip_check(hipMemcpy(&nu, hcount, 8, hipMemcpyDeviceToHost), "hipMemcpy");
+			}
+			std::vector<unsigned long long> keys(nu), vals(nu);
+			if (nu)
+			{
+				hip_check(hipMemcpy(keys.data(), hk, nu * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+				hip_check(hipMemcpy(vals.data(), hv, nu * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+			}
 			const uint64_t omask = n_obs >= 64 ? ~0ull : ((1ull << n_obs) - 1);
-			for (uint64_t i = 0; i < pl.win_cap; i++)
-				if (tab[2 * i] != MBG_EMPTY_KEY && tab[2 * i + 1])
-					res->win[tab[2 * i] >> n_obs].emplace_back(tab[2 * i] & omask, tab[2 * i + 1]);
-			for (auto& v : res->win)
-				std::sort(v.begin(), v.end());
+			for (uint64_t i = 0; i < nu; i++)
+				res->win[keys[i] >> n_obs].emplace_back(keys[i] & omask, vals[i]);
 		}
 		else
 		{

@@ -81,7 +81,7 @@ def test_workloads_bit_exact(tmp_path, wl, name, n):
     g, r = gpu_raw(tmp_path, bnd, cfg, n_trajectories=n)
     assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=n))
     if name.startswith("plain_100"):
-        assert r.info.window_mode == 3  # 50 observed nodes: sorted pairs
+        assert r.info.window_mode == 1  # 50 observed nodes: hashed window cells
 
 
 @pytest.mark.parametrize("name,n", [("plain_small", 4000), ("plain_100", 2000), ("plain_100_discrete", 2000),
