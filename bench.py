@@ -158,6 +158,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the batched and gradient forms")
+    ap.add_argument("--no-sorted-output", action="store_true", help="skip the sorted-pairs step")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one plain device copy of this many MiB (calibrates HBM byte counters)")
     a = ap.parse_args()
@@ -215,7 +216,7 @@ def main():
 
     # the same step with the window cells left as a compact sorted array in HBM (sorted pairs)
     sorted_out = None
-    if ri.window_mode == 1:
+    if ri.window_mode == 1 and not a.no_sorted_output:
         prog.simulate(n_trajectories=n, device_only=True, window_mode="sort")
         torch.cuda.synchronize()
         ts = time.perf_counter()

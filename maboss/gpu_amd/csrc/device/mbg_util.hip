@@ -75,20 +75,54 @@ extern "C" __global__ void __launch_bounds__(256) mbg_expand_tails(mbg_sim_args 
 }
 
 // The occupied cells of a hashed window table as (cell key, value) pairs in
-// the sorted mode's pair buffers (one wave-aggregated append per round), so
-// the pair sort hands the host the cells in key order.
+// the sorted mode's pair buffers, so the pair sort hands the host the cells
+// in key order.  Each block takes 16 x 256 consecutive slots, thread t every
+// 256th from its own: threads count their occupied slots, a block scan orders
+// them, and one atomic per block reserves the output range.
+constexpr int k_compact_per_thread = 16;
 extern "C" __global__ void __launch_bounds__(256) mbg_table_pairs(mbg_sim_args a)
 {
+	__shared__ unsigned int part[256];
+	__shared__ unsigned long long base;
 	const uint64_t cap = a.win_mask + 1;
-	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-	// every lane runs the same number of rounds, so each append sees its whole wave
-	const uint64_t rounds = (cap + stride - 1) / stride;
-	for (uint64_t r = 0; r < rounds; ++r)
+	// slot j of thread t: first + j * blockDim.x + t (coalesced across the block)
+	const uint64_t first = (uint64_t)blockIdx.x * blockDim.x * k_compact_per_thread + threadIdx.x;
+	unsigned int mine = 0;
+	for (int j = 0; j < k_compact_per_thread; ++j)
 	{
-		const uint64_t i = r * stride + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-		const unsigned long long key = i < cap ? a.win_table[2 * i] : MBG_EMPTY_KEY;
-		const unsigned long long val = i < cap ? a.win_table[2 * i + 1] : 0ull;
-		if (key != MBG_EMPTY_KEY && val != 0ull)
-			append_pair(a.pair_keys, a.pair_vals, a.pair_cap, a.counters + MBG_CTR_PAIRS, key, val);
+		const uint64_t i = first + (uint64_t)j * blockDim.x;
+		if (i < cap && a.win_table[2 * i] != MBG_EMPTY_KEY && a.win_table[2 * i + 1] != 0ull)
+			mine++;
+	}
+	// inclusive scan of the per-thread counts over the block
+	part[threadIdx.x] = mine;
+	__syncthreads();
+	for (unsigned int off = 1; off < blockDim.x; off <<= 1)
+	{
+		const unsigned int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+		__syncthreads();
+		part[threadIdx.x] += v;
+		__syncthreads();
+	}
+	if (threadIdx.x == blockDim.x - 1)
+		base = atomicAdd(a.counters + MBG_CTR_PAIRS, (unsigned long long)part[threadIdx.x]);
+	__syncthreads();
+	uint64_t o = base + part[threadIdx.x] - mine;
+	for (int j = 0; j < k_compact_per_thread; ++j)
+	{
+		const uint64_t i = first + (uint64_t)j * blockDim.x;
+		if (i < cap)
+		{
+			const unsigned long long key = a.win_table[2 * i], val = a.win_table[2 * i + 1];
+			if (key != MBG_EMPTY_KEY && val != 0ull)
+			{
+				if (o < a.pair_cap)
+				{
+					a.pair_keys[o] = key;
+					a.pair_vals[o] = val;
+				}
+				o++;
+			}
+		}
 	}
 }

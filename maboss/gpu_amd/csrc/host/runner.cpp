@@ -592,7 +592,7 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 			hip_check(hipMemsetAsync(counters + MBG_CTR_PAIRS, 0, 8, stream), "hipMemsetAsync");
 			{
 				void* args[] = { &a };
-				const unsigned g = (unsigned)std::min<uint64_t>((uint64_t)p.num_cus * 8, (pl.win_cap + 255) / 256);
+				const unsigned g = (unsigned)((pl.win_cap + 256 * 16 - 1) / (256 * 16)); // 16 slots per thread
 				hip_check(hipModuleLaunchKernel(p.fn_table_pairs, g, 1, 1, 256, 1, 1, 0, stream, args, nullptr),
 						  "hipModuleLaunchKernel(mbg_table_pairs)");
 			}
