@@ -73,8 +73,10 @@ extern "C" __device__ void mbg_eval(const uint32_t* __restrict__ s, float u2, mb
 // wave calls it (converged) before the wave's first mbg_eval_p with pv.
 extern "C" __device__ void mbg_eval_p_setup(const mbg_param_view* __restrict__ pv);
 // Rates read from runtime tables (batched variants, gradients).
+// en (may be null) receives the W words of enabled bits: bit i set when node
+// i's logic differs from its value (before the mutant masks).
 extern "C" __device__ void mbg_eval_p(const uint32_t* __restrict__ s, float u2, const mbg_param_view* __restrict__ pv,
-									  mbg_eval_t* __restrict__ out);
+									  mbg_eval_t* __restrict__ out, uint32_t* __restrict__ en);
 // Parallel bit-extract of the observed nodes, in node order (bit j of the
 // result is the j-th non-internal node).
 extern "C" __device__ uint64_t mbg_obs_index(const uint32_t* __restrict__ s);

@@ -120,7 +120,8 @@ extern "C" __global__ void __launch_bounds__(256) simulate_grad(mbg_grad_args a)
 			float u1, u2;
 			mbg_dev::step_uniforms(false, B0, k, id, a.variant, a.seed_lo, a.seed_hi, u1, u2);
 			mbg_eval_t e;
-			mbg_eval_p(s, u2, &a.pv, &e);
+			uint32_t en[W]; // enabled bits: the rate of each node without re-evaluating its logic
+			mbg_eval_p(s, u2, &a.pv, &e, en);
 			const float tp = t;
 			bool fixed = false;
 			t = mbg_dev::advance_time(false, t, e.total, u1, a.tick, a.max_time, fixed);
@@ -148,7 +149,7 @@ extern "C" __global__ void __launch_bounds__(256) simulate_grad(mbg_grad_args a)
 				++w;
 				b = en;
 			}
-			mbg_grad_dwell(s, dsum, (float)cdt, T, BT);
+			mbg_grad_dwell(s, en, dsum, (float)cdt, T, BT);
 
 			k++;
 			if (t >= a.max_time || k >= a.max_steps)

@@ -85,7 +85,7 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long* d_trans, u
 #if MBG_PARAM != 1
 
 #if MBG_PARAM == 2
-#define MBG_EVAL_STEP(s, u2, e) mbg_eval_p(s, u2, a.pv, e)
+#define MBG_EVAL_STEP(s, u2, e) mbg_eval_p(s, u2, a.pv, e, nullptr)
 #else
 #define MBG_EVAL_STEP(s, u2, e) mbg_eval(s, u2, e)
 #endif
@@ -311,7 +311,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 				float u1, u2;
 				step_uniforms(a.discrete, B0, k, id, v, a.seed_lo, a.seed_hi, u1, u2);
 				mbg_eval_t e;
-				mbg_eval_p(s, u2, &pv, &e);
+				mbg_eval_p(s, u2, &pv, &e, nullptr);
 				const float tp = t;
 				t = advance_time(a.discrete, t, e.total, u1, a.tick, max_time, fixed);
 				const float H = fixed ? 0.0f : e.H;

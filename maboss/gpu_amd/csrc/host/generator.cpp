@@ -467,7 +467,7 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 
 	if (param)
 		os << "extern \"C\" __device__ void mbg_eval_p(const uint32_t* __restrict__ s, float u2, "
-			  "const mbg_param_view* __restrict__ pv, mbg_eval_t* __restrict__ out)\n{\n";
+			  "const mbg_param_view* __restrict__ pv, mbg_eval_t* __restrict__ out, uint32_t* __restrict__ en)\n{\n";
 	else
 		os << "extern \"C\" __device__ void mbg_eval(const uint32_t* __restrict__ s, float u2, "
 			  "mbg_eval_t* __restrict__ out)\n{\n";
@@ -527,6 +527,14 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 		g.total_literals += em.literals;
 	}
 	os << "\tconst float total = acc;\n";
+	if (param)
+	{
+		// the enabled bits, for callers that need them (the gradient kernel)
+		os << "\tif (en)\n\t{\n";
+		for (int w = 0; w < W; w++)
+			os << "\t\ten[" << w << "] = E" << w << ";\n";
+		os << "\t}\n";
+	}
 	if (nnz_masks && !obs.empty())
 	{
 		// observed, enabled, and the rate in the node's direction nonzero
@@ -759,10 +767,10 @@ grad_code generate_grad(const model& m, const generated& g)
 
 	std::ostringstream os;
 	os << "\n#define MBG_GRAD_NP " << gc.n_params << "\n\n";
-	os << "__device__ __forceinline__ void mbg_grad_dwell(const uint32_t* __restrict__ s, float dt, float cdt, "
-		  "float (&T)[MBG_GRAD_NP], float (&BT)[MBG_GRAD_NP])\n{\n";
+	os << "__device__ __forceinline__ void mbg_grad_dwell(const uint32_t* __restrict__ s, const uint32_t* __restrict__ en, "
+		  "float dt, float cdt, float (&T)[MBG_GRAD_NP], float (&BT)[MBG_GRAD_NP])\n{\n";
 	for (int w = 0; w < m.n_words(); w++)
-		os << "\tconst uint32_t sw" << w << " = s[" << w << "];\n";
+		os << "\tconst uint32_t sw" << w << " = s[" << w << "], ew" << w << " = en[" << w << "];\n";
 	for (int i = 0; i < N; i++)
 	{
 		const node_t& n = m.nodes[i];
@@ -779,21 +787,22 @@ grad_code generate_grad(const model& m, const generated& g)
 		if (!ni.fast_form || (up_uses && ni.up_param < 0) || (down_uses && ni.down_param < 0))
 			throw error(MBG_E_LIMIT, "the gradient form needs node " + n.name
 										 + "'s rates to be `@logic ? $u : 0` / `@logic ? 0 : $d` or parameter-free");
-		emitter em(m, n, false, pidx);
-		const std::string logic = em.as_u(*n.attr("logic"));
+		// the node's rate is its up parameter when off and enabled (logic on),
+		// its down parameter when on and enabled (logic off)
+		const int w = i / 32, b = i % 32;
 		os << "\t{ // node " << i << ": " << n.name << "\n";
-		os << "\t\tconst bool x = " << em.bit(i) << " != 0u;\n";
-		os << "\t\tconst bool L = " << logic << " != 0u;\n";
+		os << "\t\tconst bool x = ((sw" << w << " >> " << b << "u) & 1u) != 0u;\n";
+		os << "\t\tconst bool e = ((ew" << w << " >> " << b << "u) & 1u) != 0u;\n";
 		if (ni.up_param >= 0)
 		{
 			gc.jparam[2 * (size_t)i] = (unsigned short)ni.up_param;
-			os << "\t\t{ const float c = (!x && L) ? 1.0f : 0.0f; T[" << ni.up_param << "] = fmaf(c, dt, T["
+			os << "\t\t{ const float c = (!x && e) ? 1.0f : 0.0f; T[" << ni.up_param << "] = fmaf(c, dt, T["
 			   << ni.up_param << "]); BT[" << ni.up_param << "] = fmaf(c, cdt, BT[" << ni.up_param << "]); }\n";
 		}
 		if (ni.down_param >= 0)
 		{
 			gc.jparam[2 * (size_t)i + 1] = (unsigned short)ni.down_param;
-			os << "\t\t{ const float c = (x && !L) ? 1.0f : 0.0f; T[" << ni.down_param << "] = fmaf(c, dt, T["
+			os << "\t\t{ const float c = (x && e) ? 1.0f : 0.0f; T[" << ni.down_param << "] = fmaf(c, dt, T["
 			   << ni.down_param << "]); BT[" << ni.down_param << "] = fmaf(c, cdt, BT[" << ni.down_param
 			   << "]); }\n";
 		}
