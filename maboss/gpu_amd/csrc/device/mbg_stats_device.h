@@ -51,6 +51,55 @@ __device__ __forceinline__ void hash_add(unsigned long long* table, uint64_t mas
 	atomicAdd(overflow, 1ull);
 }
 
+// One lane's window-cell insert into the hashed table, split so the wait for
+// the key claim overlaps the lane's next step: issue() sends the atomicCAS of
+// the first probe; settle() (one step later) reads its result and either adds
+// the value (fire and forget) or sends the next probe.  finish() settles the
+// insert completely.  Same slots and sums as hash_add.
+struct hash_inflight
+{
+	uint64_t slot = 0, key = 0, val = 0, ret = 0;
+	uint32_t probes = 0;
+	bool busy = false;
+
+	__device__ __forceinline__ void issue(unsigned long long* table, uint64_t mask, uint64_t k, uint64_t q)
+	{
+		key = k;
+		val = q;
+		slot = mix64(k) & mask;
+		probes = 0;
+		ret = atomicCAS(table + 2 * slot, MBG_EMPTY_KEY, (unsigned long long)k);
+		busy = true;
+	}
+
+	__device__ __forceinline__ void settle(unsigned long long* table, uint64_t mask, unsigned long long* overflow)
+	{
+		if (!busy)
+			return;
+		if (ret == MBG_EMPTY_KEY || ret == key)
+		{
+			atomicAdd(table + 2 * slot + 1, (unsigned long long)val);
+			busy = false;
+		}
+		else if (++probes > mask)
+		{
+			atomicAdd(overflow, 1ull);
+			busy = false;
+		}
+		else
+		{
+			slot = (slot + 1) & mask;
+			ret = atomicCAS(table + 2 * slot, MBG_EMPTY_KEY, (unsigned long long)key);
+		}
+	}
+
+	__device__ __forceinline__ void finish(unsigned long long* table, uint64_t mask, unsigned long long* overflow)
+	{
+		while (busy)
+			settle(table, mask, overflow);
+	}
+};
+
 // Append a (key, value) pair; the active lanes of the wave take consecutive
 // slots with one atomic.  Pairs beyond the capacity are counted, not stored
 // (the host then grows the buffers and runs again).

@@ -110,13 +110,27 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 	auto ft = [&](int w, uint64_t o) { hist.tail(w, o); };
 	const bool use_tails = a.tail_table != nullptr;
 	// window cells completed by the wave's records: staged pairs (sorted
-	// mode) or straight into the histogram
+	// mode), a pipelined insert (hashed mode: the previous cell's insert is
+	// settled when the next one is issued) or straight into the histogram
 	unsigned long long* cur = lds + a.pair_lds_off + 4 * wave;
 	const bool staged = a.win_mode == MBG_HIST_SORT && a.pair_stage;
+	const bool hashed = a.win_mode == MBG_HIST_HASH;
 	pair_stage_reg stage { lds + a.pair_stage_off + 2 * a.pair_stage * wave, a.pair_stage };
+	hash_inflight ins;
+	unsigned long long* const win_over = a.counters + MBG_CTR_WIN_OVERFLOW;
 	auto emit = [&](bool has, int w, uint64_t o, uint64_t q) {
 		if (staged)
 			stage.add(cur, a, has, ((uint64_t)w << a.n_obs) | o, q);
+		else if (hashed)
+		{
+			if (has)
+			{
+				ins.finish(a.win_table, a.win_mask, win_over);
+				ins.issue(a.win_table, a.win_mask, ((uint64_t)w << a.n_obs) | o, q);
+			}
+			else
+				ins.settle(a.win_table, a.win_mask, win_over);
+		}
 		else if (has)
 			hist.win(w, o, q);
 	};
@@ -216,12 +230,14 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 		}
 	}
 
-	// the lanes' last pending cells, then the rest of the stage
+	// the lanes' last pending cells, then the rest of the stage / the inserts in flight
 	emit(pend.pw >= 0, pend.pw, pend.pobs, pend.pq);
 	if (pend.pew >= 0)
 		fe(pend.pew, pend.peq);
 	if (staged)
 		stage.drain(cur, a);
+	if (hashed)
+		ins.finish(a.win_table, a.win_mask, win_over);
 	wave_add_counters(a.counters + MBG_CTR_TRANSITIONS, a.counters + MBG_CTR_FINISHED, a.counters + MBG_CTR_TRUNCATED,
 					  ntrans, nfinished, ntrunc);
 
