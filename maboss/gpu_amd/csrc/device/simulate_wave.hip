@@ -94,6 +94,121 @@ __device__ __forceinline__ float row0_running_sum(float r)
 	return q;
 }
 
+__device__ __forceinline__ uint64_t fp_hash(uint32_t sw, int W)
+{
+	uint64_t h = 0x9e3779b97f4a7c15ull;
+	for (int w = 0; w < W; ++w)
+		h = mix64(h ^ rdlane_u(sw, w));
+	return h;
+}
+
+// Add cnt trajectories ending in the fixed point whose word w lane w holds
+// (sw; lanes >= W hold none) to the global fixed-point table: the slots,
+// hash and probing of fp_insert, done by the whole wave (lane 0 claims,
+// every lane writes or compares its word).
+__device__ void wave_fp_global(const mbg_sim_args& a, uint32_t sw, int W, int lane, uint32_t cnt)
+{
+	const int stride = 2 + W;
+	uint64_t slot = fp_hash(sw, W) & a.fp_mask;
+	for (uint64_t p = 0; p <= a.fp_mask; ++p)
+	{
+		unsigned int* e = a.fp_table + slot * stride;
+		uint32_t tag = 0, claimed = 0;
+		if (lane == 0)
+		{
+			tag = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+			if (tag == 0u)
+			{
+				tag = atomicCAS(e, 0u, 1u);
+				claimed = tag == 0u;
+			}
+		}
+		claimed = rdlane_u(claimed, 0);
+		if (claimed)
+		{
+			if (lane < W)
+				e[2 + lane] = sw;
+			__threadfence();
+			if (lane == 0)
+			{
+				__hip_atomic_store(e, 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+				atomicAdd(e + 1, cnt);
+			}
+			return;
+		}
+		if (lane == 0)
+			for (int spin = 0; tag == 1u && spin < (1 << 22); ++spin)
+			{
+				__builtin_amdgcn_s_sleep(2);
+				tag = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+			}
+		tag = rdlane_u(tag, 0);
+		if (tag != 2u)
+		{
+			if (lane == 0)
+				atomicAdd(a.counters + MBG_CTR_FP_TIMEOUT, 1ull);
+			return;
+		}
+		__threadfence();
+		const bool same = lane >= W || __hip_atomic_load(e + 2 + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == sw;
+		if (__ballot(!same) == 0ull)
+		{
+			if (lane == 0)
+				atomicAdd(e + 1, cnt);
+			return;
+		}
+		slot = (slot + 1) & a.fp_mask;
+	}
+	if (lane == 0)
+		atomicAdd(a.counters + MBG_CTR_FP_OVERFLOW, 1ull);
+}
+
+// One more trajectory ending in that fixed point, counted in the block's LDS
+// cache: nslots (a power of two) slots of (2 + W) words {tag, count, key},
+// tag 0 free, 1 being written, 2 complete.  False when 16 probes find no
+// slot (the caller then goes to the global table).
+__device__ bool wave_fp_lds(unsigned int* t, int nslots, uint32_t sw, int W, int lane)
+{
+	const int stride = 2 + W;
+	uint32_t slot = (uint32_t)fp_hash(sw, W) & (uint32_t)(nslots - 1);
+	for (int p = 0; p < 16; ++p)
+	{
+		unsigned int* e = t + slot * stride;
+		uint32_t tag = 0;
+		if (lane == 0)
+			tag = atomicCAS(e, 0u, 1u);
+		tag = rdlane_u(tag, 0);
+		if (tag == 0u)
+		{
+			if (lane < W)
+				e[2 + lane] = sw;
+			__threadfence_block();
+			if (lane == 0)
+			{
+				atomicExch(e, 2u);
+				atomicAdd(e + 1, 1u);
+			}
+			return true;
+		}
+		if (lane == 0)
+			for (int spin = 0; tag == 1u && spin < (1 << 16); ++spin)
+				tag = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+		tag = rdlane_u(tag, 0);
+		if (tag != 2u)
+			return false;
+		const bool same =
+			lane >= W || __hip_atomic_load(e + 2 + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == sw;
+		if (__ballot(!same) == 0ull)
+		{
+			if (lane == 0)
+				atomicAdd(e + 1, 1u);
+			return true;
+		}
+		slot = (slot + 1u) & (uint32_t)(nslots - 1);
+	}
+	return false;
+}
+
 template <int NPLMAX>
 __device__ void wave_main(const mbg_sim_args& a)
 {
@@ -344,67 +459,8 @@ __device__ void wave_main(const mbg_sim_args& a)
 		{
 			if (lane == 0)
 				hist.fin(obs);
-			if (fixed)
-			{
-				constexpr int stride_pad = 2;
-				const int stride = stride_pad + W;
-				uint64_t h = 0x9e3779b97f4a7c15ull;
-				for (int w = 0; w < W; ++w)
-					h = mix64(h ^ rdlane_u(sw, w));
-				uint64_t slot = h & a.fp_mask;
-				for (uint64_t p = 0; p <= a.fp_mask; ++p)
-				{
-					unsigned int* e = a.fp_table + slot * stride;
-					uint32_t tag = 0, claimed = 0;
-					if (lane == 0)
-					{
-						tag = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-						if (tag == 0u)
-						{
-							tag = atomicCAS(e, 0u, 1u);
-							claimed = tag == 0u;
-						}
-					}
-					claimed = rdlane_u(claimed, 0);
-					if (claimed)
-					{
-						if (lane < W)
-							e[2 + lane] = sw;
-						__threadfence();
-						if (lane == 0)
-						{
-							__hip_atomic_store(e, 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-							atomicAdd(e + 1, 1u);
-						}
-						break;
-					}
-					if (lane == 0)
-						for (int spin = 0; tag == 1u && spin < (1 << 22); ++spin)
-						{
-							__builtin_amdgcn_s_sleep(2);
-							tag = __hip_atomic_load(e, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-						}
-					tag = rdlane_u(tag, 0);
-					if (tag != 2u)
-					{
-						if (lane == 0)
-							atomicAdd(a.counters + MBG_CTR_FP_TIMEOUT, 1ull);
-						break;
-					}
-					__threadfence();
-					const bool same =
-						lane >= W || __hip_atomic_load(e + 2 + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == sw;
-					if (__ballot(!same) == 0ull)
-					{
-						if (lane == 0)
-							atomicAdd(e + 1, 1u);
-						break;
-					}
-					slot = (slot + 1) & a.fp_mask;
-					if (p == a.fp_mask && lane == 0)
-						atomicAdd(a.counters + MBG_CTR_FP_OVERFLOW, 1ull);
-				}
-			}
+			if (fixed && !(a.fp_lds_slots && wave_fp_lds((unsigned int*)(lds + a.fp_lds_off), a.fp_lds_slots, sw, W, lane)))
+				wave_fp_global(a, sw, W, lane, 1u);
 		}
 	}
 
@@ -421,6 +477,18 @@ __device__ void wave_main(const mbg_sim_args& a)
 		pad_pair_chunk(lds + a.pair_lds_off + 4 * wave, a.pair_keys, a.pair_vals, a.pair_cap);
 	__syncthreads();
 	flush_block_hist(lds, a);
+	// the block's cached fixed points: one slot per wave
+	if (a.fp_lds_slots)
+	{
+		const unsigned int* t = (const unsigned int*)(lds + a.fp_lds_off);
+		for (int i = wave; i < a.fp_lds_slots; i += (int)(blockDim.x >> 6))
+		{
+			const unsigned int* e = t + (size_t)i * (2 + W);
+			const uint32_t tag = rdlane_u(e[0], 0), cnt = rdlane_u(e[1], 0);
+			if (tag == 2u && cnt != 0u)
+				wave_fp_global(a, lane < W ? e[2 + lane] : 0u, W, lane, cnt);
+		}
+	}
 }
 
 } // namespace

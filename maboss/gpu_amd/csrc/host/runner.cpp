@@ -161,10 +161,10 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 		p.tail_off = (int)(lds / 8);
 		lds += (size_t)p.tail_slots * 16;
 	}
-	if (!wave_kernel)
 	{
+		// the wave kernel keeps a smaller cache: its LDS holds the node tables
 		const size_t slot = 4 * (2 + (size_t)m.n_words());
-		p.fp_slots = cache_slots(k_lds_budget - lds, slot, 128, 16);
+		p.fp_slots = wave_kernel ? cache_slots(k_lds_budget - lds, slot, 16, 8) : cache_slots(k_lds_budget - lds, slot, 128, 16);
 		p.fp_off = (int)(lds / 8);
 		lds += ((size_t)p.fp_slots * slot + 7) & ~(size_t)7;
 	}

@@ -105,6 +105,35 @@ def test_wave_kernel_bit_exact_32_rows(tmp_path):
     assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=100, extra=("--order", "wave")))
 
 
+def write_memory_model(tmp_path, n_mem=60, n_dyn=10, n=3000):
+    """n_mem internal nodes that keep their initial state (logic = self) and
+    n_dyn observed nodes that copy one of them: every trajectory ends in a
+    fixed point, nearly all of them distinct (2^n_mem initial states)."""
+    nodes = []
+    for i in range(n_mem):
+        nodes.append(f"Node X{i} {{\n  logic = X{i};\n  rate_up = @logic ? 1.0 : 0.0;\n"
+                     f"  rate_down = @logic ? 0.0 : 1.0;\n}}\n")
+    for i in range(n_dyn):
+        nodes.append(f"Node Y{i} {{\n  logic = X{i} & !X{i + 1};\n  rate_up = @logic ? 2.0 : 0.0;\n"
+                     f"  rate_down = @logic ? 0.0 : 0.5;\n}}\n")
+    bnd, cfg = tmp_path / "mem.bnd", tmp_path / "mem.cfg"
+    bnd.write_text("".join(nodes))
+    cfg.write_text(f"sample_count = {n};\nmax_time = 20;\ntime_tick = 1;\ndiscrete_time = 0;\n"
+                   f"seed_pseudorandom = 11;\n" + "".join(f"X{i}.is_internal = 1;\n" for i in range(n_mem)))
+    return str(bnd), str(cfg)
+
+
+@pytest.mark.parametrize("kernel,order", [("lane", ()), ("wave", ("--order", "wave"))])
+def test_many_distinct_fixed_points(tmp_path, kernel, order):
+    """More distinct fixed points than a block's LDS cache holds: the cache's
+    fall-through to the global table and its flush at the block's end."""
+    bnd, cfg = write_memory_model(tmp_path)
+    g, r = gpu_raw(tmp_path, bnd, cfg, kernel=kernel, n_trajectories=3000)
+    assert r.info.kernel == (1 if kernel == "wave" else 0)
+    assert len(read_raw(g)["P"]) > 2900
+    assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=3000, extra=order))
+
+
 def test_wave_kernel_reference_model(tmp_path):
     bnd, cfg = model_paths("cellcycle")
     g, _ = gpu_raw(tmp_path, bnd, cfg, kernel="wave", n_trajectories=20000)
