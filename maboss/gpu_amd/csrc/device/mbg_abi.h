@@ -37,6 +37,16 @@ struct mbg_param_view
 #define MBG_OPAQUE(x) asm volatile("" : "+r"(x))
 #endif
 
+// Hides that a value is a 0 / all-ones mask, so that a select written as
+// (u ^ ((u ^ d) & m)) & e stays two bitwise instructions on the mask instead
+// of becoming a compare and a conditional move with both constants loaded.
+// Unlike MBG_OPAQUE it does not order the code around it.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MBG_MASK(x) asm("" : "+v"(x))
+#else
+#define MBG_MASK(x) ((void)0)
+#endif
+
 // Pins running sums at a point of the instruction stream (no instruction is
 // emitted): the generated rate code places one after each node, so each
 // node's terms are added before the next node's are computed, rather than

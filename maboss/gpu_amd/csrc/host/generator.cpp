@@ -496,7 +496,7 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 			os << "\t\tconst uint32_t ew = (sw" << w << " ^ " << em.as_aligned(*n.attr("logic"), b) << ") & " << (1u << b)
 			   << "u;\n";
 			os << "\t\tE" << w << " |= ew;\n";
-			os << "\t\tconst uint32_t xm = 0u - ((sw" << w << " >> " << b << "u) & 1u);\n";
+			os << "\t\tuint32_t xm = 0u - ((sw" << w << " >> " << b << "u) & 1u);\n\t\tMBG_MASK(xm);\n";
 			os << "\t\tuint32_t msk = 0u - (ew >> " << b << "u);\n";
 			select_rate(em, i, lw);
 		}
