@@ -167,20 +167,21 @@ __device__ __forceinline__ void append_pair_chunked(unsigned long long* cur, uns
 
 // Window pairs of one wave staged in LDS with the fill count in a register:
 // add() is called by the whole wave (lanes with has set contribute a pair);
-// when 64 or more pairs are staged, 64 of them go to HBM as whole lines and
-// the rest move down, so up to 64 new pairs always fit (capacity 128).
+// when 64 or more pairs are staged, 64 of them go to the sink, one per lane
+// (sorted mode: HBM as whole lines; hashed mode: one table insert per lane),
+// and the rest move down, so up to 64 new pairs always fit (capacity 128).
 struct pair_stage_reg
 {
 	unsigned long long* keys; // [cap] then values [cap]
 	int cap;
 	unsigned int n = 0;
 
-	__device__ void out64(unsigned long long* cur, const mbg_sim_args& a, unsigned int take)
+	template <typename SINK>
+	__device__ void out64(unsigned int take, SINK&& sink)
 	{
 		const unsigned int lane = (unsigned int)__lane_id();
 		if (lane < take)
-			append_pair_chunked(cur, a.pair_keys, a.pair_vals, a.pair_cap, a.counters + MBG_CTR_PAIRS,
-								(uint64_t)a.pair_chunk, keys[lane], keys[cap + lane]);
+			sink(keys[lane], keys[cap + lane]);
 		unsigned long long k = 0, v = 0;
 		const unsigned int src = take + lane;
 		if (src < n)
@@ -198,13 +199,14 @@ struct pair_stage_reg
 		n -= take;
 	}
 
-	__device__ void add(unsigned long long* cur, const mbg_sim_args& a, bool has, uint64_t key, uint64_t q)
+	template <typename SINK>
+	__device__ void add(bool has, uint64_t key, uint64_t q, SINK&& sink)
 	{
 		const uint64_t m = __ballot(has);
 		if (!m)
 			return;
 		if (n >= 64u)
-			out64(cur, a, 64u);
+			out64(64u, sink);
 		if (has)
 		{
 			const unsigned int i = n + (unsigned int)__popcll(m & ((1ull << __lane_id()) - 1ull));
@@ -214,10 +216,11 @@ struct pair_stage_reg
 		n += (unsigned int)__popcll(m);
 	}
 
-	__device__ void drain(unsigned long long* cur, const mbg_sim_args& a)
+	template <typename SINK>
+	__device__ void drain(SINK&& sink)
 	{
 		while (n > 0u)
-			out64(cur, a, n < 64u ? n : 64u);
+			out64(n < 64u ? n : 64u, sink);
 	}
 };
 

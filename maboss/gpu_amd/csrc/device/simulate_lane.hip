@@ -109,28 +109,29 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 	auto fe = [&](int w, uint64_t q) { hist.ent(w, q); };
 	auto ft = [&](int w, uint64_t o) { hist.tail(w, o); };
 	const bool use_tails = a.tail_table != nullptr;
-	// window cells completed by the wave's records: staged pairs (sorted
-	// mode), a pipelined insert (hashed mode: the previous cell's insert is
-	// settled when the next one is issued) or straight into the histogram
+	// window cells completed by the wave's records: staged in LDS, 64 at a
+	// time to the sink (sorted mode: pair chunks in HBM; hashed mode: one
+	// pipelined table insert per lane, the claim of each batch settled when the
+	// next batch is issued), or straight into the histogram
 	unsigned long long* cur = lds + a.pair_lds_off + 4 * wave;
-	const bool staged = a.win_mode == MBG_HIST_SORT && a.pair_stage;
 	const bool hashed = a.win_mode == MBG_HIST_HASH;
+	const bool staged = (a.win_mode == MBG_HIST_SORT || hashed) && a.pair_stage;
 	pair_stage_reg stage { lds + a.pair_stage_off + 2 * a.pair_stage * wave, a.pair_stage };
 	hash_inflight ins;
 	unsigned long long* const win_over = a.counters + MBG_CTR_WIN_OVERFLOW;
+	auto sink = [&](uint64_t key, uint64_t q) {
+		if (hashed)
+		{
+			ins.finish(a.win_table, a.win_mask, win_over);
+			ins.issue(a.win_table, a.win_mask, key, q);
+		}
+		else
+			append_pair_chunked(cur, a.pair_keys, a.pair_vals, a.pair_cap, a.counters + MBG_CTR_PAIRS,
+								(uint64_t)a.pair_chunk, key, q);
+	};
 	auto emit = [&](bool has, int w, uint64_t o, uint64_t q) {
 		if (staged)
-			stage.add(cur, a, has, ((uint64_t)w << a.n_obs) | o, q);
-		else if (hashed)
-		{
-			if (has)
-			{
-				ins.finish(a.win_table, a.win_mask, win_over);
-				ins.issue(a.win_table, a.win_mask, ((uint64_t)w << a.n_obs) | o, q);
-			}
-			else
-				ins.settle(a.win_table, a.win_mask, win_over);
-		}
+			stage.add(has, ((uint64_t)w << a.n_obs) | o, q, sink);
 		else if (has)
 			hist.win(w, o, q);
 	};
@@ -235,7 +236,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 	if (pend.pew >= 0)
 		fe(pend.pew, pend.peq);
 	if (staged)
-		stage.drain(cur, a);
+		stage.drain(sink);
 	if (hashed)
 		ins.finish(a.win_table, a.win_mask, win_over);
 	wave_add_counters(a.counters + MBG_CTR_TRANSITIONS, a.counters + MBG_CTR_FINISHED, a.counters + MBG_CTR_TRUNCATED,

@@ -144,13 +144,13 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 	{
 		p.pair_off = (int)(lds / 8);
 		lds += 16 * 32; // per-wave chunk cursor {next, left} (32 bytes) for up to 16 waves
-		if (!wave_kernel)
-		{
-			// 128 staged pairs per wave, blocks of at most 4 waves (the lane kernel's bound)
-			p.stage = 128;
-			p.stage_off = (int)(lds / 8);
-			lds += 4 * (size_t)p.stage * 16;
-		}
+	}
+	if ((p.win_mode == MBG_HIST_SORT || p.win_mode == MBG_HIST_HASH) && !wave_kernel)
+	{
+		// 128 staged pairs per wave, blocks of at most 4 waves (the lane kernel's bound)
+		p.stage = 128;
+		p.stage_off = (int)(lds / 8);
+		lds += 4 * (size_t)p.stage * 16;
 	}
 	// block-private caches in front of the tail and fixed-point tables
 	if (!m.discrete_time() && p.win_mode != MBG_HIST_LDS)
