@@ -291,11 +291,15 @@ __device__ __forceinline__ void record(bool discrete, float tp, float t, uint64_
 }
 
 // record() for the whole wave at once: every lane calls (on: the lane has a
-// record), and the windows of the records are visited in a loop that is
-// uniform across the wave, so the window cell a lane completes in an
-// iteration reaches emit(has, w, obs, q) at a point where the whole wave is
-// active (emit can then stage it with wave-wide operations).  Same cells,
-// values and order per lane as record().
+// record) and every window cell completed reaches emit(has, w, obs, q) at a
+// point where the whole wave is active (emit can then stage it with
+// wave-wide operations).  Continuous time: the lane itself handles the
+// record's first and last windows (the ones its pending cell and pending
+// entropy may merge with); the full windows in between are spread over the
+// wave's lanes, one window per lane, so a record that spans many windows
+// does not keep the rest of the wave waiting.  Same cell and entropy sums as
+// record() (the same slices: for each window w after the first, the slice is
+// min(t, (w + 1) tick) - w tick in FP32), added in another order.
 template <typename EMIT, typename FE, typename FT>
 __device__ __forceinline__ void record_wave(bool on, bool discrete, float tp, float t, uint64_t obs, float H, float tick,
 											float qscale, int windows, pending& p, EMIT&& emit, FE&& fe, FT&& ft,
@@ -338,46 +342,109 @@ __device__ __forceinline__ void record_wave(bool on, bool discrete, float tp, fl
 		emit(has, ow, oo, oq);
 		return;
 	}
-	int w = on ? (int)floorf(tp / tick) : 0;
-	const int w_first = w;
-	bool tail_done = false;
-	float b = tp;
-	bool more = on && t > b;
-	while (__ballot(more))
+
+	bool has0 = false, has1 = false, cells = false;
+	int ow0 = 0, ow1 = 0;
+	uint64_t oo0 = 0, oq0 = 0, oo1 = 0, oq1 = 0;
+	int ra = 0, len = 0; // windows [ra, ra + len) go to the wave
+	if (on && t > tp)
 	{
-		bool has = false;
-		int ow = 0;
-		uint64_t oo = 0, oq = 0;
-		if (more)
+		// first window: slice [tp, min(t, end of window))
+		const int w0 = (int)floorf(tp / tick);
+		const float en0 = fminf(t, (float)(w0 + 1) * tick);
+		if (w0 < windows)
 		{
-			const float wend = (float)(w + 1) * tick;
-			const float en = fminf(t, wend);
-			const float sl = en - b;
-			if (w < windows)
+			const float sl = en0 - tp;
+			const uint64_t q = (uint64_t)rintf(sl * qscale);
+			if (q)
+				merge(w0, q, has0, ow0, oo0, oq0);
+			const uint64_t qe = (uint64_t)rintf((H * sl) * qscale);
+			if (qe)
+				p.ent(w0, qe, fe);
+		}
+		if (t > en0)
+		{
+			// the record reaches windows w0 + 1 .. wl: wl is the largest w with
+			// w tick < t in FP32 (record() visits w while its start is below t)
+			int wl = (int)floorf(t / tick);
+			while (wl > w0 + 1 && (float)wl * tick >= t)
+				--wl;
+			if (wl < w0 + 1)
+				wl = w0 + 1;
+			while ((float)(wl + 1) * tick < t)
+				++wl;
+			if (final_tail)
 			{
-				if (final_tail && w > w_first)
+				// the trajectory's last record: the tail table takes the
+				// windows' dwell; they still add entropy
+				if (w0 + 1 < windows)
 				{
-					if (!tail_done)
+					ft(w0 + 1, obs);
+					if (H != 0.0f)
 					{
-						ft(w, obs);
-						tail_done = true;
+						ra = w0 + 1;
+						len = min(wl, windows - 1) - ra + 1;
 					}
 				}
-				else
+			}
+			else
+			{
+				cells = true;
+				ra = w0 + 1;
+				len = max(min(wl - 1, windows - 1) - ra + 1, 0);
+				if (wl < windows)
 				{
+					const float sl = fminf(t, (float)(wl + 1) * tick) - (float)wl * tick;
 					const uint64_t q = (uint64_t)rintf(sl * qscale);
 					if (q)
-						merge(w, q, has, ow, oo, oq);
+						merge(wl, q, has1, ow1, oo1, oq1);
+					const uint64_t qe = (uint64_t)rintf((H * sl) * qscale);
+					if (qe)
+						p.ent(wl, qe, fe);
 				}
-				const uint64_t qe = (uint64_t)rintf((H * sl) * qscale);
-				if (qe)
-					p.ent(w, qe, fe);
 			}
-			w++;
-			b = en;
-			more = t > b && !(tail_done && H == 0.0f);
 		}
-		emit(has, ow, oo, oq);
+	}
+	emit(has0, ow0, oo0, oq0);
+	emit(has1, ow1, oo1, oq1);
+	if (__ballot(len > 0) == 0ull)
+		return;
+
+	// the windows in between, 64 at a time: item j of the wave's concatenated
+	// ranges goes to lane j % 64
+	const int lane = __lane_id();
+	int incl = len;
+#pragma unroll
+	for (int d = 1; d < 64; d <<= 1)
+	{
+		const int v = __shfl_up(incl, (unsigned int)d);
+		if (lane >= d)
+			incl += v;
+	}
+	const int total = __shfl(incl, 63);
+	for (int base = 0; base < total; base += 64)
+	{
+		const int j = base + lane;
+		// owner: the number of lanes whose range ends at or before j
+		int src = 0;
+#pragma unroll
+		for (int step = 32; step > 0; step >>= 1)
+			src += (__shfl(incl, src + step - 1) <= j) ? step : 0;
+		const bool valid = j < total;
+		src = valid ? src : 0;
+		const int w = __shfl(ra, src) + j - (__shfl(incl, src) - __shfl(len, src));
+		const float ts = __shfl(t, src), Hs = __shfl(H, src);
+		const uint64_t os = __shfl(obs, src);
+		const bool cs = __shfl((int)cells, src) != 0;
+		const float sl = fminf(ts, (float)(w + 1) * tick) - (float)w * tick;
+		const uint64_t q = (valid && cs) ? (uint64_t)rintf(sl * qscale) : 0ull;
+		emit(q != 0ull, w, os, q);
+		if (valid)
+		{
+			const uint64_t qe = (uint64_t)rintf((Hs * sl) * qscale);
+			if (qe)
+				fe(w, qe);
+		}
 	}
 }
 
