@@ -35,7 +35,14 @@
 #error "MBG_GRAD_NP must be defined by the generated code"
 #endif
 
-extern "C" __global__ void __launch_bounds__(256) simulate_grad(mbg_grad_args a)
+// Waves per SIMD the registers are budgeted for (set by the host from the
+// parameter count; the per-parameter accumulators take most registers).
+#ifndef MBG_GRAD_WAVES
+#define MBG_GRAD_WAVES 1
+#endif
+
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MBG_GRAD_WAVES)))
+simulate_grad(mbg_grad_args a)
 {
 	constexpr int NP = MBG_GRAD_NP;
 	constexpr int W = MBG_W;

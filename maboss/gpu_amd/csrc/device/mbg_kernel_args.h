@@ -65,7 +65,9 @@ struct mbg_sim_args
 	const float* istate; // [n_nodes] probability that node starts on
 
 	int win_mode, fin_mode, ent_lds, lds_bytes; // storage modes, LDS size in bytes
-	int win_lds_off, fin_lds_off, ent_lds_off, pad1; // u64 offsets into dynamic LDS
+	int win_lds_off, fin_lds_off, ent_lds_off; // u64 offsets into dynamic LDS
+	int own_lds_off; // u64 offset of 64 bytes per wave (the lane that owns each item of a
+	                 // cooperative pass), then the block's 3 u64 counters (lane kernel)
 
 	unsigned long long* win_vals;   // dense cells (MBG_HIST_DENSE)
 	unsigned long long* win_table;  // hash slots {key, value} (MBG_HIST_HASH), 16 B each

@@ -290,6 +290,27 @@ __device__ __forceinline__ void record(bool discrete, float tp, float t, uint64_
 	}
 }
 
+// Inclusive prefix sum over the wave's 64 lanes with DPP moves (all lanes
+// active): within rows of 16 by row_shr 1, 2, 4, 8, then rows 1 and 3 add
+// lane 15 of the row before (row_bcast:15) and rows 2 and 3 add lane 31
+// (row_bcast:31).  A move with no source lane adds 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int add_dpp_i(int v)
+{
+	return v + __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, false);
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+	v = add_dpp_i<0x111, 0xF>(v);
+	v = add_dpp_i<0x112, 0xF>(v);
+	v = add_dpp_i<0x114, 0xF>(v);
+	v = add_dpp_i<0x118, 0xF>(v);
+	v = add_dpp_i<0x142, 0xA>(v);
+	v = add_dpp_i<0x143, 0xC>(v);
+	return v;
+}
+
 // record() for the whole wave at once: every lane calls (on: the lane has a
 // record) and every window cell completed reaches emit(has, w, obs, q) at a
 // point where the whole wave is active (emit can then stage it with
@@ -303,7 +324,7 @@ __device__ __forceinline__ void record(bool discrete, float tp, float t, uint64_
 template <typename EMIT, typename FE, typename FT>
 __device__ __forceinline__ void record_wave(bool on, bool discrete, float tp, float t, uint64_t obs, float H, float tick,
 											float qscale, int windows, pending& p, EMIT&& emit, FE&& fe, FT&& ft,
-											bool final_tail)
+											bool final_tail, uint8_t* own)
 {
 	// the lane's pending cell when the record moves to another one
 	auto merge = [&](int w, uint64_t q, bool& has, int& ow, uint64_t& oo, uint64_t& oq) {
@@ -411,33 +432,29 @@ __device__ __forceinline__ void record_wave(bool on, bool discrete, float tp, fl
 		return;
 
 	// the windows in between, 64 at a time: item j of the wave's concatenated
-	// ranges goes to lane j % 64
+	// ranges goes to lane j % 64; own[] (64 bytes of LDS for the wave) maps
+	// the items of a pass to the lanes whose ranges hold them
 	const int lane = __lane_id();
-	int incl = len;
-#pragma unroll
-	for (int d = 1; d < 64; d <<= 1)
-	{
-		const int v = __shfl_up(incl, (unsigned int)d);
-		if (lane >= d)
-			incl += v;
-	}
-	const int total = __shfl(incl, 63);
+	const int incl = wave_incl_scan(len);
+	const int excl = incl - len;
+	const int total = __builtin_amdgcn_readlane(incl, 63);
+	const int rx = ((ra - excl) << 1) | (cells ? 1 : 0); // window of item j: (rx >> 1) + j
 	for (int base = 0; base < total; base += 64)
 	{
+		const int hi = min(incl, base + 64);
+		for (int i = max(excl, base); i < hi; ++i)
+			own[i - base] = (uint8_t)lane;
+		MBG_WAVE_SYNC();
 		const int j = base + lane;
-		// owner: the number of lanes whose range ends at or before j
-		int src = 0;
-#pragma unroll
-		for (int step = 32; step > 0; step >>= 1)
-			src += (__shfl(incl, src + step - 1) <= j) ? step : 0;
 		const bool valid = j < total;
-		src = valid ? src : 0;
-		const int w = __shfl(ra, src) + j - (__shfl(incl, src) - __shfl(len, src));
+		const int src = valid ? (int)own[lane] : 0;
+		MBG_WAVE_SYNC();
+		const int rxs = __shfl(rx, src);
 		const float ts = __shfl(t, src), Hs = __shfl(H, src);
 		const uint64_t os = __shfl(obs, src);
-		const bool cs = __shfl((int)cells, src) != 0;
+		const int w = (rxs >> 1) + j;
 		const float sl = fminf(ts, (float)(w + 1) * tick) - (float)w * tick;
-		const uint64_t q = (valid && cs) ? (uint64_t)rintf(sl * qscale) : 0ull;
+		const uint64_t q = (valid && (rxs & 1)) ? (uint64_t)rintf(sl * qscale) : 0ull;
 		emit(q != 0ull, w, os, q);
 		if (valid)
 		{

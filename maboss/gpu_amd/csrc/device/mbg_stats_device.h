@@ -358,21 +358,25 @@ __device__ bool fp_lds_insert(unsigned int* t, int nslots, const uint32_t (&key)
 	return false;
 }
 
-struct lane_hist;
-
-struct lane_hist
+// WM / FM: window and final-state storage (MBG_HIST_*) fixed at compile
+// time, or -1 to follow the run's arguments.
+template <int WM = -1, int FM = -1>
+struct lane_hist_t
 {
 	unsigned long long* lds;
 	const mbg_sim_args& a;
 
+	__device__ __forceinline__ int win_mode() const { return WM >= 0 ? WM : a.win_mode; }
+	__device__ __forceinline__ int fin_mode() const { return FM >= 0 ? FM : a.fin_mode; }
+
 	__device__ void win(int w, uint64_t obs, uint64_t q) const
 	{
 		const uint64_t cell = ((uint64_t)w << a.n_obs) | obs;
-		if (a.win_mode == MBG_HIST_LDS)
+		if (win_mode() == MBG_HIST_LDS)
 			atomicAdd(lds + a.win_lds_off + cell, (unsigned long long)q);
-		else if (a.win_mode == MBG_HIST_DENSE)
+		else if (win_mode() == MBG_HIST_DENSE)
 			atomicAdd(a.win_vals + cell, (unsigned long long)q);
-		else if (a.win_mode == MBG_HIST_SORT)
+		else if (win_mode() == MBG_HIST_SORT)
 			append_pair_chunked(lds + a.pair_lds_off + 4 * (threadIdx.x >> 6), a.pair_keys, a.pair_vals, a.pair_cap,
 								a.counters + MBG_CTR_PAIRS, (uint64_t)a.pair_chunk, cell, q);
 		else
@@ -407,14 +411,16 @@ struct lane_hist
 
 	__device__ void fin(uint64_t obs) const
 	{
-		if (a.fin_mode == MBG_HIST_LDS)
+		if (fin_mode() == MBG_HIST_LDS)
 			atomicAdd(lds + a.fin_lds_off + obs, 1ull);
-		else if (a.fin_mode == MBG_HIST_DENSE)
+		else if (fin_mode() == MBG_HIST_DENSE)
 			atomicAdd(a.fin_vals + obs, 1ull);
 		else if (!a.fin_lds_cache_slots || !lds_count_add(lds + a.fin_lds_cache_off, a.fin_lds_cache_slots, obs, 1ull))
 			hash_add(a.fin_table, a.fin_mask, obs, 1ull, a.counters + MBG_CTR_FIN_OVERFLOW);
 	}
 };
+
+using lane_hist = lane_hist_t<>;
 
 // End of a block: add its LDS histograms and caches to the global storage
 // (after a barrier).

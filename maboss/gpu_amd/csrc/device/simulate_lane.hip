@@ -84,6 +84,12 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long* d_trans, u
 
 #if MBG_PARAM != 1
 
+// The plain kernel is compiled for the run's window and final-state storage
+// (MBG_HIST_*): only that storage's code is in it.
+#if !defined(MBG_WIN_MODE) || !defined(MBG_FIN_MODE)
+#error "MBG_WIN_MODE and MBG_FIN_MODE select the storage compiled into the plain kernel"
+#endif
+
 #if MBG_PARAM == 2
 #define MBG_EVAL_STEP(s, u2, e) mbg_eval_p(s, u2, a.pv, e, nullptr)
 #else
@@ -102,7 +108,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 #endif
 	__syncthreads();
 
-	const lane_hist hist { lds, a };
+	const lane_hist_t<MBG_WIN_MODE, MBG_FIN_MODE> hist { lds, a };
 	const int lane = __lane_id();
 	const int wave = threadIdx.x >> 6;
 	const uint32_t B0 = ((uint32_t)a.n_nodes + 3u) >> 2;
@@ -114,8 +120,9 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 	// pipelined table insert per lane, the claim of each batch settled when the
 	// next batch is issued), or straight into the histogram
 	unsigned long long* cur = lds + a.pair_lds_off + 4 * wave;
-	const bool hashed = a.win_mode == MBG_HIST_HASH;
-	const bool staged = (a.win_mode == MBG_HIST_SORT || hashed) && a.pair_stage;
+	uint8_t* const own = (uint8_t*)(lds + a.own_lds_off) + 64 * wave;
+	const bool hashed = MBG_WIN_MODE == MBG_HIST_HASH;
+	const bool staged = (MBG_WIN_MODE == MBG_HIST_SORT || hashed) && a.pair_stage;
 	pair_stage_reg stage { lds + a.pair_stage_off + 2 * a.pair_stage * wave, a.pair_stage };
 	hash_inflight ins;
 	unsigned long long* const win_over = a.counters + MBG_CTR_WIN_OVERFLOW;
@@ -125,7 +132,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 			ins.finish(a.win_table, a.win_mask, win_over);
 			ins.issue(a.win_table, a.win_mask, key, q);
 		}
-		else
+		else if (MBG_WIN_MODE == MBG_HIST_SORT)
 			append_pair_chunked(cur, a.pair_keys, a.pair_vals, a.pair_cap, a.counters + MBG_CTR_PAIRS,
 								(uint64_t)a.pair_chunk, key, q);
 	};
@@ -145,7 +152,9 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 	uint64_t id = 0, obs = 0; // obs: observed index of s, kept up to date at each flip
 	bool active = false, exhausted = false;
 	pending pend;
-	uint64_t ntrans = 0, nfinished = 0, ntrunc = 0;
+	// the block's {transitions, finished, truncated} counters, after the
+	// waves' item-owner bytes; a trajectory adds its steps when it ends
+	unsigned long long* const bctr = (unsigned long long*)((uint8_t*)(lds + a.own_lds_off) + 64 * (blockDim.x >> 6));
 
 	id_chunks queue;
 	for (;;)
@@ -187,22 +196,17 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 			tp = t;
 			t = advance_time(a.discrete, t, e.total, u1, a.tick, a.max_time, fixed);
 			H = fixed ? 0.0f : e.H;
-			ntrans++;
 		}
 		record_wave(active, a.discrete, tp, t, obs, H, a.tick, a.qscale, a.windows, pend, emit, fe, ft,
-					use_tails && t >= a.max_time);
+					use_tails && t >= a.max_time, own);
 		if (active)
 		{
 			k++;
-			if (t >= a.max_time)
+			if (t >= a.max_time || k >= a.max_steps)
 			{
 				ended = true;
-				nfinished++;
-			}
-			else if (k >= a.max_steps)
-			{
-				ended = true;
-				ntrunc++;
+				atomicAdd(bctr, (unsigned long long)k);
+				atomicAdd(bctr + (t >= a.max_time ? 1 : 2), 1ull);
 			}
 			else
 			{
@@ -239,14 +243,15 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 		stage.drain(sink);
 	if (hashed)
 		ins.finish(a.win_table, a.win_mask, win_over);
-	wave_add_counters(a.counters + MBG_CTR_TRANSITIONS, a.counters + MBG_CTR_FINISHED, a.counters + MBG_CTR_TRUNCATED,
-					  ntrans, nfinished, ntrunc);
-
-	if (a.win_mode == MBG_HIST_SORT)
+	if (MBG_WIN_MODE == MBG_HIST_SORT)
 		pad_pair_chunk(lds + a.pair_lds_off + 4 * (threadIdx.x >> 6), a.pair_keys, a.pair_vals, a.pair_cap);
 
-	// Flush the block's LDS histograms and caches.
+	// Flush the block's counters, LDS histograms and caches.
 	__syncthreads();
+	if (threadIdx.x < 3 && bctr[threadIdx.x])
+		atomicAdd(a.counters + (threadIdx.x == 0 ? MBG_CTR_TRANSITIONS
+											: threadIdx.x == 1 ? MBG_CTR_FINISHED : MBG_CTR_TRUNCATED),
+				  bctr[threadIdx.x]);
 	flush_block_hist(lds, a);
 	flush_block_fp<W>(lds, a);
 }

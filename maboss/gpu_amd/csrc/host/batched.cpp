@@ -81,7 +81,7 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 	std::vector<std::unique_ptr<result>> out;
 	if (vs.empty())
 		return out;
-	hipFunction_t fn = p.function(1);
+	hipFunction_t fn = p.function(1, 0, 0);
 	const int np = (int)p.gen.param_names.size();
 	const float tick = m.time_tick();
 	const uint64_t seed = bp.seed < 0 ? m.seed() : (uint64_t)bp.seed;

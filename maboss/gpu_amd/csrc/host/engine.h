@@ -60,8 +60,10 @@ struct program
 	gen_options opts;
 	int W = 1;
 	// lane kernels per flavour: 0 plain, 1 batched, 2 plain with run-time rates
-	hipModule_t mods[3] = { nullptr, nullptr, nullptr };
-	hipFunction_t fns[3] = { nullptr, nullptr, nullptr };
+	// lane kernels per flavour; the plain flavours (0, 2) also per window and
+	// final-state storage mode, which they take as compile-time constants
+	hipModule_t mods[3][4][3] = {};
+	hipFunction_t fns[3][4][3] = {};
 	double compile_seconds = 0;
 	bool no_cache = false;
 	int device = 0;
@@ -101,7 +103,8 @@ struct program
 	std::string wave_unsupported; // reason, when the model does not fit the wave kernel
 
 	~program();
-	hipFunction_t function(int flavour);
+	// win_mode / fin_mode: MBG_HIST_* of the run (ignored for flavour 1)
+	hipFunction_t function(int flavour, int win_mode, int fin_mode);
 	device_buffer b_rate_view; // run-time rates of the plain flavour 2
 	// Kernel for the wave path at the model's nodes-per-lane (loads on first use).
 	hipFunction_t wave_function();
@@ -163,6 +166,8 @@ struct result
 };
 
 std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t stream);
+// Window and final-state storage modes a default run of the model uses.
+void default_storage_modes(const model& m, int& win_mode, int& fin_mode);
 
 // ---- batched form
 struct variant_spec

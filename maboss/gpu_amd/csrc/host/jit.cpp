@@ -158,9 +158,11 @@ std::vector<char> compile_to_code_object(const std::string& source)
 
 program::~program()
 {
-	for (hipModule_t m : mods)
-		if (m)
-			(void)hipModuleUnload(m);
+	for (auto& by_win : mods)
+		for (auto& by_fin : by_win)
+			for (hipModule_t m : by_fin)
+				if (m)
+					(void)hipModuleUnload(m);
 	if (mod_wave)
 		(void)hipModuleUnload(mod_wave);
 	if (batch_graph)
@@ -185,7 +187,13 @@ hipFunction_t program::grad_function()
 	const grad_code& gc = grad();
 	scope_timer tm("compiler> gradient kernel");
 	const double t0 = wall_seconds();
-	const std::string src = gen.source + gc.source + "\n#include \"mbg_grad_kernel.h\"\n";
+	// Occupancy: 2 waves per SIMD up to 96 rate parameters (their T and BT
+	// accumulators, 2 registers each, then leave room for the rate code; on
+	// the 30-node model with 60 parameters 2 waves with a few spilled
+	// registers ran 1.6x faster than 1 wave without), 1 wave above.
+	const int waves = gc.n_params <= 96 ? 2 : 1;
+	const std::string src = "#define MBG_GRAD_WAVES " + std::to_string(waves) + "\n" + gen.source + gc.source
+							+ "\n#include \"mbg_grad_kernel.h\"\n";
 	uint64_t h = fnv1a(src.data(), src.size());
 	for (const char* f : k_compile_flags)
 		h = fnv1a(f, std::strlen(f), h);
@@ -240,25 +248,31 @@ hipFunction_t program::wave_function()
 	return fn;
 }
 
-hipFunction_t program::function(int flavour)
+hipFunction_t program::function(int flavour, int win_mode, int fin_mode)
 {
 	if (flavour < 0 || flavour > 2)
 		throw error(MBG_E_INTERNAL, "unknown kernel flavour");
-	hipFunction_t& fn = fns[flavour];
+	const bool param = flavour == 1;
+	if (param)
+		win_mode = fin_mode = 0;
+	if (win_mode < 0 || win_mode > MBG_HIST_SORT || fin_mode < 0 || fin_mode > MBG_HIST_LDS)
+		throw error(MBG_E_INTERNAL, "unknown storage mode");
+	hipFunction_t& fn = fns[flavour][win_mode][fin_mode];
 	if (fn)
 		return fn;
-	hipModule_t& mod = mods[flavour];
-	const bool param = flavour == 1;
+	hipModule_t& mod = mods[flavour][win_mode][fin_mode];
 
 	scope_timer tm("compiler> whole_compilation");
 	const double t0 = wall_seconds();
 	// Occupancy (waves per SIMD; the compiler alone picks fewer waves with
-	// more registers each).  The kernel waits on LDS, shuffles and the queue
-	// more than it issues, so resident waves pay even with a few spilled
-	// registers.  Measured on MI355X (A/B in one run each): 3 waves beat 1 by
-	// 1.5x on the 100-node model and 2 or 4 to 6 waves at 50-100 nodes (4 was
-	// within noise of 3 on the continuous 100-node model); at 30 nodes 4 waves
-	// beat the compiler's choice by 15 %; above 160 nodes 2.
+	// more registers each): the most waves whose registers fit without
+	// spilling, starting from 4 up to 40 nodes, 3 up to 160 and 2 above.  A
+	// spilled register is reloaded from scratch memory, and that load waits
+	// for every atomic the wave still has in flight (window-cell inserts), so
+	// a kernel that spills goes down one wave while it spills.  Measured on
+	// MI355X: at 30 nodes 4 waves beat the compiler's choice by 15 %; on the
+	// 100-node model 2 waves without spills beat 3 with 11 spilled registers
+	// by 5 %.
 	const int n_nodes = mdl->n_nodes();
 	int waves = 4;
 	if (opts.lane_waves > 0)
@@ -267,38 +281,52 @@ hipFunction_t program::function(int flavour)
 		waves = 2;
 	else if (n_nodes > 40)
 		waves = 3;
-	const std::string src = "#define MBG_KW " + std::to_string(W) + "\n#define MBG_PARAM " + std::to_string(flavour)
-							+ "\n#define MBG_LANE_WAVES " + std::to_string(waves) + "\n" + gen.source + "\n"
-							+ mbg_src_lane;
-	uint64_t h = fnv1a(src.data(), src.size());
-	for (const char* f : k_compile_flags)
-		h = fnv1a(f, std::strlen(f), h);
-	for (const char* hd :
-		 { mbg_hdr_numeric, mbg_hdr_abi, mbg_hdr_kernel_args, mbg_hdr_stats_device, mbg_hdr_lane_core })
-		h = fnv1a(hd, std::strlen(hd), h);
-	char name[64];
-	std::snprintf(name, sizeof(name), "%016llx_%d.co", (unsigned long long)h, flavour);
-	const std::string dir = cache_dir();
-	const std::string path = dir + "/" + name;
+	for (;;)
+	{
+		std::string src = "#define MBG_KW " + std::to_string(W) + "\n#define MBG_PARAM " + std::to_string(flavour)
+						  + "\n#define MBG_LANE_WAVES " + std::to_string(waves) + "\n";
+		if (!param)
+			src += "#define MBG_WIN_MODE " + std::to_string(win_mode) + "\n#define MBG_FIN_MODE "
+				   + std::to_string(fin_mode) + "\n";
+		src += gen.source + "\n" + mbg_src_lane;
+		uint64_t h = fnv1a(src.data(), src.size());
+		for (const char* f : k_compile_flags)
+			h = fnv1a(f, std::strlen(f), h);
+		for (const char* hd :
+			 { mbg_hdr_numeric, mbg_hdr_abi, mbg_hdr_kernel_args, mbg_hdr_stats_device, mbg_hdr_lane_core })
+			h = fnv1a(hd, std::strlen(hd), h);
+		char name[64];
+		std::snprintf(name, sizeof(name), "%016llx_%d.co", (unsigned long long)h, flavour);
+		const std::string dir = cache_dir();
+		const std::string path = dir + "/" + name;
 
-	std::vector<char> co;
-	if (no_cache || !read_all(path, co))
-	{
-		co = compile_to_code_object(src);
-		if (!no_cache)
+		std::vector<char> co;
+		if (no_cache || !read_all(path, co))
 		{
-			mkdirs(dir);
-			write_atomic(path, co.data(), co.size());
+			co = compile_to_code_object(src);
+			if (!no_cache)
+			{
+				mkdirs(dir);
+				write_atomic(path, co.data(), co.size());
+			}
 		}
+		hip_check(hipModuleLoadData(&mod, co.data()), "hipModuleLoadData");
+		if (param)
+		{
+			hip_check(hipModuleGetFunction(&fn, mod, "simulate_batched"), "hipModuleGetFunction(simulate_batched)");
+			hip_check(hipModuleGetFunction(&fn_fill, mod, "mbg_fill_u64"), "hipModuleGetFunction(mbg_fill_u64)");
+		}
+		else
+			hip_check(hipModuleGetFunction(&fn, mod, "simulate"), "hipModuleGetFunction(simulate)");
+		int scratch = 0;
+		hip_check(hipFuncGetAttribute(&scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fn), "hipFuncGetAttribute");
+		if (scratch == 0 || waves <= 2 || opts.lane_waves > 0)
+			break;
+		(void)hipModuleUnload(mod);
+		mod = nullptr;
+		fn = nullptr;
+		--waves;
 	}
-	hip_check(hipModuleLoadData(&mod, co.data()), "hipModuleLoadData");
-	if (param)
-	{
-		hip_check(hipModuleGetFunction(&fn, mod, "simulate_batched"), "hipModuleGetFunction(simulate_batched)");
-		hip_check(hipModuleGetFunction(&fn_fill, mod, "mbg_fill_u64"), "hipModuleGetFunction(mbg_fill_u64)");
-	}
-	else
-		hip_check(hipModuleGetFunction(&fn, mod, "simulate"), "hipModuleGetFunction(simulate)");
 	compile_seconds += wall_seconds() - t0;
 	return fn;
 }
@@ -332,7 +360,11 @@ std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const g
 		throw error(MBG_E_HIP, std::string("this build targets gfx950; current device is ") + prop.gcnArchName);
 	p->num_cus = prop.multiProcessorCount;
 	if (p->W <= max_lane_words)
-		p->function(0);
+	{
+		int wm = 0, fm = 0;
+		default_storage_modes(*p->mdl, wm, fm);
+		p->function(0, wm, fm);
+	}
 	else
 		p->wave_function();
 	return p;

@@ -78,6 +78,7 @@ struct plan
 	int win_off = 0, fin_off = 0, ent_off = 0, lds_bytes = 0;
 	int tail_off = 0, tail_slots = 0, fp_off = 0, fp_slots = 0; // LDS caches
 	int fin_cache_off = 0, fin_cache_slots = 0;
+	int own_off = 0;                                             // per-wave item owners (lane kernel)
 	int pair_off = 0;                                            // per-wave pair chunk cursors
 	int stage_off = 0, stage = 0;                                // per-wave pair staging (lane kernel)
 	uint64_t win_cells = 0, fin_cells = 0, win_cap = 0, fin_cap = 0, fp_cap = 0;
@@ -145,6 +146,12 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 		p.pair_off = (int)(lds / 8);
 		lds += 16 * 32; // per-wave chunk cursor {next, left} (32 bytes) for up to 16 waves
 	}
+	if (!wave_kernel)
+	{
+		// 64 bytes per wave (blocks of at most 4 waves), then the block's 3 counters
+		p.own_off = (int)(lds / 8);
+		lds += 4 * 64 + 3 * 8;
+	}
 	if ((p.win_mode == MBG_HIST_SORT || p.win_mode == MBG_HIST_HASH) && !wave_kernel)
 	{
 		// 128 staged pairs per wave, blocks of at most 4 waves (the lane kernel's bound)
@@ -204,6 +211,13 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 
 } // namespace
 
+void default_storage_modes(const model& m, int& win_mode, int& fin_mode)
+{
+	const plan pl = make_plan(m, sim_params {}, (uint64_t)m.sample_count(), m.windows(), m.n_observed(), false);
+	win_mode = pl.win_mode;
+	fin_mode = pl.fin_mode;
+}
+
 std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t stream)
 {
 	scope_timer tm("runner> simulate");
@@ -239,7 +253,6 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		throw error(MBG_E_LIMIT, "run-time rates are served by the lane kernel (up to 256 nodes)");
 	if (!sp.rates.empty() && sp.rates.size() != p.gen.param_names.size())
 		throw error(MBG_E_ARG, "expected " + std::to_string(p.gen.param_names.size()) + " rates");
-	hipFunction_t fn = use_wave ? p.wave_function() : p.function(sp.rates.empty() ? 0 : 2);
 	// run-time rates: a parameter view {rate, rate*log2(rate), enable masks} in device memory
 	const mbg_param_view* d_view = nullptr;
 	if (!sp.rates.empty())
@@ -267,6 +280,7 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 	}
 	res->kernel = use_wave ? 1 : 0;
 	plan pl = make_plan(m, sp, n_traj, windows, n_obs, use_wave);
+	hipFunction_t fn = use_wave ? p.wave_function() : p.function(sp.rates.empty() ? 0 : 2, pl.win_mode, pl.fin_mode);
 	// tail records for window storage in HBM (continuous time)
 	const bool tails = !m.discrete_time() && pl.win_mode != MBG_HIST_LDS;
 	uint64_t tail_cap = tails ? next_pow2(std::min<uint64_t>(std::max<uint64_t>(n_traj / 4, 1 << 12), 1 << 20)) : 0;
@@ -372,6 +386,7 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		a.tail_lds_slots = tails ? pl.tail_slots : 0;
 		a.fp_lds_off = pl.fp_off;
 		a.fp_lds_slots = pl.fp_slots;
+		a.own_lds_off = pl.own_off;
 		a.fin_lds_cache_off = pl.fin_cache_off;
 		a.fin_lds_cache_slots = pl.fin_cache_slots;
 		a.pair_keys = pk;

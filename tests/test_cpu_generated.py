@@ -207,8 +207,10 @@ def test_single_unit_kernels_compile_with_hiprtc(native_bin, tmp_path):
     lane = open(os.path.join(DEVICE_INC, "simulate_lane.hip")).read()
     headers = [os.path.join(DEVICE_INC, h) for h in ("mbg_numeric.h", "mbg_abi.h", "mbg_kernel_args.h",
                                                      "mbg_stats_device.h", "mbg_lane_core.h", "mbg_grad_kernel.h")]
-    units = {f"lane_p{p}": f"#define MBG_KW {m.info.n_words}\n#define MBG_PARAM {p}\n" + m.generate_code() + "\n" + lane
-             for p in (0, 1)}
+    # the plain flavour with hashed window cells and dense final states (MBG_HIST_HASH, MBG_HIST_DENSE)
+    modes = {0: "#define MBG_WIN_MODE 1\n#define MBG_FIN_MODE 0\n", 1: ""}
+    units = {f"lane_p{p}": f"#define MBG_KW {m.info.n_words}\n#define MBG_PARAM {p}\n" + modes[p] + m.generate_code()
+             + "\n" + lane for p in (0, 1)}
     units["grad"] = m.generate_code(with_gradient=True)
     for name, text in units.items():
         src = tmp_path / f"{name}.hip"
