@@ -266,6 +266,9 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 {
 	const int lane = __lane_id();
 	const uint32_t B0 = ((uint32_t)a.n_nodes + 3u) >> 2;
+	// per wave: the lane that owns each item of a cooperative window pass
+	__shared__ uint8_t own_tab[4][64];
+	uint8_t* const own = own_tab[(threadIdx.x >> 6) & 3];
 	uint32_t s[W];
 #pragma unroll
 	for (int w = 0; w < W; ++w)
@@ -294,6 +297,10 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 			atomicAdd(a.win_vals + win_off + (((uint64_t)w << a.n_obs) | o), (unsigned long long)q);
 		};
 		auto fe = [&](int w, uint64_t q) { atomicAdd(a.win_ent + ent_off + w, (unsigned long long)q); };
+		auto emit = [&](bool has, int w, uint64_t o, uint64_t q) {
+			if (has)
+				fw(w, o, q);
+		};
 
 		uint64_t next = ch.begin;
 		const uint64_t end = ch.end;
@@ -328,17 +335,22 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 				break;
 
 			bool ended = false, fixed = false;
+			mbg_eval_t e;
+			float tp = 0.0f, H = 0.0f;
 			if (active)
 			{
 				float u1, u2;
 				step_uniforms(a.discrete, B0, k, id, v, a.seed_lo, a.seed_hi, u1, u2);
-				mbg_eval_t e;
 				mbg_eval_p(s, u2, &pv, &e, nullptr);
-				const float tp = t;
+				tp = t;
 				t = advance_time(a.discrete, t, e.total, u1, a.tick, max_time, fixed);
-				const float H = fixed ? 0.0f : e.H;
+				H = fixed ? 0.0f : e.H;
 				ntrans++;
-				record(a.discrete, tp, t, obs, H, a.tick, a.qscale, windows, pend, fw, fe, [](int, uint64_t) {}, false);
+			}
+			record_wave(active, a.discrete, tp, t, obs, H, a.tick, a.qscale, windows, pend, emit, fe,
+						[](int, uint64_t) {}, false, own);
+			if (active)
+			{
 				k++;
 				if (t >= max_time)
 				{
