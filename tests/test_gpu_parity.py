@@ -134,6 +134,35 @@ def test_many_distinct_fixed_points(tmp_path, kernel, order):
     assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=3000, extra=order))
 
 
+def write_slow_model(tmp_path):
+    """Six slow togglers: a dwell lasts ~5 time units, ~100 windows of 0.05, so
+    the windows between a record's first and last ones fill many 64-lane
+    passes of the wave."""
+    nodes = "".join(f"Node S{i} {{\n  logic = !S{i};\n  rate_up = @logic ? {0.02 + 0.01 * i} : 0.0;\n"
+                    f"  rate_down = @logic ? 0.0 : {0.05 - 0.005 * i};\n}}\n" for i in range(6))
+    bnd, cfg = tmp_path / "slow.bnd", tmp_path / "slow.cfg"
+    bnd.write_text(nodes)
+    cfg.write_text("sample_count = 2000;\nmax_time = 100;\ntime_tick = 0.05;\ndiscrete_time = 0;\n"
+                   "seed_pseudorandom = 5;\n")
+    return str(bnd), str(cfg)
+
+
+@pytest.mark.parametrize("win", ["dense", "hash", "sort"])
+def test_long_dwells_bit_exact(tmp_path, win):
+    """Records spanning hundreds of windows (the wave-wide window pass), in
+    each HBM window storage, against the oracle; the batched kernel's run of
+    the same model equals the plain run."""
+    bnd, cfg = write_slow_model(tmp_path)
+    g, r = gpu_raw(tmp_path, bnd, cfg, n_trajectories=2000, window_mode=win)
+    assert r.info.window_mode == {"dense": 0, "hash": 1, "sort": 3}[win]
+    assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=2000))
+    if win == "dense":
+        results, _ = program(bnd, cfg).simulate_batched([{"n_trajectories": 2000}])
+        b = tmp_path / "batched.raw"
+        results[0].write_raw(str(b))
+        assert_same(b, g)
+
+
 def test_wave_kernel_reference_model(tmp_path):
     bnd, cfg = model_paths("cellcycle")
     g, _ = gpu_raw(tmp_path, bnd, cfg, kernel="wave", n_trajectories=20000)
