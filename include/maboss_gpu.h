@@ -90,7 +90,9 @@ typedef struct mbg_compile_opts
 	int32_t with_gradient;   /* mbg_generate_code only: append the gradient kernel's
 	                            model code and include (MBG_ERR_LIMIT if the model has none) */
 	int32_t lane_occupancy;  /* waves per SIMD the lane kernels are compiled for
-	                            (register budget 512 / waves per lane); 0 = automatic */
+	                            (register budget 512 / waves per lane); 0 = automatic:
+	                            4, 3 or 2 by node count, one fewer while the kernel
+	                            would spill registers (down to 2) */
 	int32_t reserved[4];
 } mbg_compile_opts;
 
