@@ -438,7 +438,8 @@ __device__ __forceinline__ void record_wave(bool on, bool discrete, float tp, fl
 	const int incl = wave_incl_scan(len);
 	const int excl = incl - len;
 	const int total = __builtin_amdgcn_readlane(incl, 63);
-	const int rx = ((ra - excl) << 1) | (cells ? 1 : 0); // window of item j: (rx >> 1) + j
+	// window of item j: base + j with base = ra - excl (may be negative); bit 0: cells
+	const int rx = (ra - excl) * 2 + (cells ? 1 : 0);
 	for (int base = 0; base < total; base += 64)
 	{
 		const int hi = min(incl, base + 64);
@@ -452,7 +453,7 @@ __device__ __forceinline__ void record_wave(bool on, bool discrete, float tp, fl
 		const int rxs = __shfl(rx, src);
 		const float ts = __shfl(t, src), Hs = __shfl(H, src);
 		const uint64_t os = __shfl(obs, src);
-		const int w = (rxs >> 1) + j;
+		const int w = (rxs - (rxs & 1)) / 2 + j;
 		const float sl = fminf(ts, (float)(w + 1) * tick) - (float)w * tick;
 		const uint64_t q = (valid && (rxs & 1)) ? (uint64_t)rintf(sl * qscale) : 0ull;
 		emit(q != 0ull, w, os, q);
