@@ -1,7 +1,7 @@
 """Model front-end: the four reference models, precedence, errors.
 
 Pins the parser against facts fixed by the model files themselves (node and
-observed-node counts, constants; SURVEY-independent: read off the .bnd/.cfg)
+observed-node counts and constants, read off the .bnd/.cfg themselves)
 and the expression semantics against Python's own evaluation of the same
 formulas with C precedence."""
 import itertools
