@@ -79,7 +79,23 @@ def cpu_baseline(bnd, cfg, target_s, threads):
     res = run(n)
     return {"value": res["transitions_per_s"], "unit": "transitions/s", "cores": threads, "kind": "port",
             "sample": f"first {n} trajectory ids of the same workload ({res['transitions']} transitions, "
-                      f"{res['seconds']:.1f} s, formulas compiled with g++ -O3 -march=native)"}
+                      f"{res['seconds']:.1f} s, formulas compiled with g++ -O3 -march=native)",
+            "host": host_cpu()}
+
+
+def host_cpu():
+    """CPU model and logical CPU count of the machine the baseline ran on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
 
 
 def bench_batched(mg, netgen, wdir, steps):
