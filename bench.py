@@ -98,6 +98,28 @@ def host_cpu():
             "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
 
 
+def bench_discrete(mg, netgen, wdir, steps):
+    """The same 100-node network in discrete time (one record per step at window round(t / tick))."""
+    import torch
+    bnd, cfg = netgen.write_workload("plain_100_discrete", wdir)
+    model = mg.Model.load(bnd, cfg)
+    prog = model.compile()
+    n = model.info.sample_count
+    prog.simulate(n_trajectories=n, device_only=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    trans, sim_ms = 0, 0.0
+    for _ in range(steps):
+        ri = prog.simulate(n_trajectories=n, device_only=True).info
+        trans += ri.transitions
+        sim_ms += ri.simulate_ms
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"workload": "plain_100_discrete: the plain_100 network with discrete_time = 1, 1e6 trajectories",
+            "value": trans / wall, "unit": "transitions/s", "ms_per_step": wall / steps * 1e3,
+            "kernel_ms": sim_ms / steps, "window_mode": ri.window_mode, "transitions_per_step": trans / steps}
+
+
 def bench_batched(mg, netgen, wdir, steps):
     """Batched form: the 256 variants of the batched workload (rates x lognormal(0, 0.5), 0-3 mutants,
     1e3-1e5 trajectories, max_time 10-200) in one graph-captured launch, against the same variants
@@ -296,6 +318,8 @@ def main():
                            "window_cells": fi.n_window_cells},
         "compile_seconds": prog.compile_seconds,
     }
+    if not a.no_forms and a.workload == "plain_100":
+        out["discrete"] = bench_discrete(mg, netgen, wdir, max(1, min(a.steps, 3)))
     if not a.no_forms:
         out["batched"] = bench_batched(mg, netgen, wdir, max(1, min(a.steps, 3)))
         out["grad"] = bench_grad(mg, netgen, wdir, max(1, min(a.steps, 3)))
