@@ -51,16 +51,22 @@ __device__ __forceinline__ void hash_add(unsigned long long* table, uint64_t mas
 	atomicAdd(overflow, 1ull);
 }
 
-// One lane's window-cell insert into the hashed table, split so the wait for
-// the key claim overlaps the lane's next step: issue() sends the atomicCAS of
-// the first probe; settle() (one step later) reads its result and either adds
-// the value (fire and forget) or sends the next probe.  finish() settles the
-// insert completely.  Same slots and sums as hash_add.
+// One lane's window-cell insert into the hashed table as a sequence of
+// non-blocking steps, so each memory round trip overlaps the lane's next
+// steps: the slot's key is read first (a key already in the table, about half
+// the inserts, then costs one read and the value add instead of a claim and
+// the add; a read of a slot is also much cheaper than a failed claim), an
+// empty slot is claimed with atomicCAS, a slot held by another key moves the
+// probe on.  step() reads the result of the operation in flight and sends
+// the next one, or the value add (fire and forget) when the slot is found.
+// Same slots and sums as hash_add.
 struct hash_inflight
 {
 	uint64_t slot = 0, key = 0, val = 0, ret = 0;
 	uint32_t probes = 0;
-	bool busy = false;
+	uint32_t state = 0; // 0 free, 1 key read in flight, 2 claim in flight
+
+	__device__ __forceinline__ bool busy() const { return state != 0u; }
 
 	__device__ __forceinline__ void issue(unsigned long long* table, uint64_t mask, uint64_t k, uint64_t q)
 	{
@@ -68,35 +74,42 @@ struct hash_inflight
 		val = q;
 		slot = mix64(k) & mask;
 		probes = 0;
-		ret = atomicCAS(table + 2 * slot, MBG_EMPTY_KEY, (unsigned long long)k);
-		busy = true;
+		ret = __hip_atomic_load(table + 2 * slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		state = 1;
 	}
 
-	__device__ __forceinline__ void settle(unsigned long long* table, uint64_t mask, unsigned long long* overflow)
+	__device__ __forceinline__ void step(unsigned long long* table, uint64_t mask, unsigned long long* overflow)
 	{
-		if (!busy)
+		if (state == 0u)
 			return;
-		if (ret == MBG_EMPTY_KEY || ret == key)
+		if (ret == key || (state == 2u && ret == MBG_EMPTY_KEY))
 		{
 			atomicAdd(table + 2 * slot + 1, (unsigned long long)val);
-			busy = false;
+			state = 0;
+		}
+		else if (state == 1u && ret == MBG_EMPTY_KEY)
+		{
+			// read as empty (or not yet seen filled): claim it
+			ret = atomicCAS(table + 2 * slot, MBG_EMPTY_KEY, (unsigned long long)key);
+			state = 2;
 		}
 		else if (++probes > mask)
 		{
 			atomicAdd(overflow, 1ull);
-			busy = false;
+			state = 0;
 		}
 		else
 		{
 			slot = (slot + 1) & mask;
-			ret = atomicCAS(table + 2 * slot, MBG_EMPTY_KEY, (unsigned long long)key);
+			ret = __hip_atomic_load(table + 2 * slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			state = 1;
 		}
 	}
 
 	__device__ __forceinline__ void finish(unsigned long long* table, uint64_t mask, unsigned long long* overflow)
 	{
-		while (busy)
-			settle(table, mask, overflow);
+		while (state != 0u)
+			step(table, mask, overflow);
 	}
 };
 
@@ -165,37 +178,50 @@ __device__ __forceinline__ void append_pair_chunked(unsigned long long* cur, uns
 	}
 }
 
-// Window pairs of one wave staged in LDS with the fill count in a register:
-// add() is called by the whole wave (lanes with has set contribute a pair);
-// when 64 or more pairs are staged, 64 of them go to the sink, one per lane
-// (sorted mode: HBM as whole lines; hashed mode: one table insert per lane),
-// and the rest move down, so up to 64 new pairs always fit (capacity 128).
+// Window pairs of one wave staged in LDS with the fill count in a register.
+// add() is called by the whole wave (lanes with has set contribute a pair).
+// Once 64 pairs are staged they go to the sink, one per lane that is ready
+// for one (sorted mode: every lane, HBM as whole lines; hashed mode: a lane
+// whose previous insert has finished after one more step), and the rest move
+// down; at most cap - 64 stay staged, so 64 new pairs always fit.
 struct pair_stage_reg
 {
 	unsigned long long* keys; // [cap] then values [cap]
 	int cap;
 	unsigned int n = 0;
 
+	// sink.step(): every lane advances its work in flight; sink.ready(): the
+	// lane can take a pair; sink.take(key, value)
 	template <typename SINK>
-	__device__ void out64(unsigned int take, SINK&& sink)
+	__device__ void out(SINK&& sink)
 	{
 		const unsigned int lane = (unsigned int)__lane_id();
-		if (lane < take)
-			sink(keys[lane], keys[cap + lane]);
-		unsigned long long k = 0, v = 0;
-		const unsigned int src = take + lane;
-		if (src < n)
+		sink.step();
+		const bool ready = sink.ready();
+		const uint64_t rm = __ballot(ready);
+		const unsigned int nready = (unsigned int)__popcll(rm);
+		const unsigned int take = n < nready ? n : nready;
+		const unsigned int rank = (unsigned int)__popcll(rm & ((1ull << lane) - 1ull));
+		if (ready && rank < take)
+			sink.take(keys[rank], keys[cap + rank]);
+		// the pairs [take, n) move to [0, n - take), 64 at a time from the bottom
+		for (unsigned int base = 0; base + take < n; base += 64u)
 		{
-			k = keys[src];
-			v = keys[cap + src];
+			unsigned long long k = 0, v = 0;
+			const unsigned int src = take + base + lane;
+			if (src < n)
+			{
+				k = keys[src];
+				v = keys[cap + src];
+			}
+			__builtin_amdgcn_wave_barrier();
+			if (src < n)
+			{
+				keys[base + lane] = k;
+				keys[cap + base + lane] = v;
+			}
+			__builtin_amdgcn_wave_barrier();
 		}
-		__builtin_amdgcn_wave_barrier();
-		if (src < n)
-		{
-			keys[lane] = k;
-			keys[cap + lane] = v;
-		}
-		__builtin_amdgcn_wave_barrier();
 		n -= take;
 	}
 
@@ -206,7 +232,9 @@ struct pair_stage_reg
 		if (!m)
 			return;
 		if (n >= 64u)
-			out64(64u, sink);
+			out(sink);
+		while (n > (unsigned int)cap - 64u)
+			out(sink);
 		if (has)
 		{
 			const unsigned int i = n + (unsigned int)__popcll(m & ((1ull << __lane_id()) - 1ull));
@@ -220,7 +248,7 @@ struct pair_stage_reg
 	__device__ void drain(SINK&& sink)
 	{
 		while (n > 0u)
-			out64(n < 64u ? n : 64u, sink);
+			out(sink);
 	}
 };
 

@@ -154,8 +154,10 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 	}
 	if ((p.win_mode == MBG_HIST_SORT || p.win_mode == MBG_HIST_HASH) && !wave_kernel)
 	{
-		// 128 staged pairs per wave, blocks of at most 4 waves (the lane kernel's bound)
-		p.stage = 128;
+		// 192 staged pairs per wave (64 arrive at a time; in hashed mode a lane
+		// whose insert is still in flight takes no new pair, so a flush may
+		// leave more than 64), blocks of at most 4 waves (the lane kernel's bound)
+		p.stage = 192;
 		p.stage_off = (int)(lds / 8);
 		lds += 4 * (size_t)p.stage * 16;
 	}
