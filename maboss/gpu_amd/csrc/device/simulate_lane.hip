@@ -101,6 +101,11 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 	extern __shared__ unsigned long long lds[];
 	for (int i = threadIdx.x; i < a.lds_bytes / 8; i += blockDim.x)
 		lds[i] = 0ull;
+	// initial-state probabilities in LDS: a refill then reads no global
+	// memory, so it does not wait for the window-cell atomics in flight
+	__shared__ float istate[32 * W];
+	for (int i = threadIdx.x; i < a.n_nodes; i += blockDim.x)
+		istate[i] = a.istate[i];
 #if MBG_PARAM == 0
 	mbg_eval_setup();
 #else
@@ -189,7 +194,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 				else
 					exhausted = true;
 			}
-			init_states_wave<W>(s, fresh, id, a.variant, a.seed_lo, a.seed_hi, a.n_nodes, a.istate);
+			init_states_wave<W>(s, fresh, id, a.variant, a.seed_lo, a.seed_hi, a.n_nodes, istate);
 			if (fresh)
 				obs = mbg_obs_index(s);
 		}
