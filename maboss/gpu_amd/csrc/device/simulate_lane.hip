@@ -285,6 +285,11 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 	// per wave: the lane that owns each item of a cooperative window pass
 	__shared__ uint8_t own_tab[4][64];
 	uint8_t* const own = own_tab[(threadIdx.x >> 6) & 3];
+	// per wave: the initial-state probabilities of its current variant (a
+	// refill then reads no global memory, so it does not wait for the
+	// histogram atomics in flight)
+	__shared__ float ist_tab[4][32 * W];
+	float* const istate = ist_tab[(threadIdx.x >> 6) & 3];
 	uint32_t s[W];
 #pragma unroll
 	for (int w = 0; w < W; ++w)
@@ -305,7 +310,9 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 		const int windows = vd.windows;
 		const unsigned long long win_off = vd.win_off, fin_off = vd.fin_off;
 		const unsigned ent_off = vd.ent_off;
-		const float* istate = vd.istate;
+		for (int i = lane; i < a.n_nodes; i += 64)
+			istate[i] = vd.istate[i];
+		MBG_WAVE_SYNC();
 		const mbg_param_view pv = vd.pv;
 		mbg_eval_p_setup(&pv);
 
