@@ -121,9 +121,9 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 	auto ft = [&](int w, uint64_t o) { hist.tail(w, o); };
 	const bool use_tails = a.tail_table != nullptr;
 	// window cells completed by the wave's records: staged in LDS, 64 at a
-	// time to the sink (sorted mode: pair chunks in HBM; hashed mode: one
-	// pipelined table insert per lane, the claim of each batch settled when the
-	// next batch is issued), or straight into the histogram
+	// time to the sink (sorted mode: pair chunks in HBM; hashed mode: one table
+	// insert per lane, run as steps that each wait for the previous batch's
+	// memory operation only), or straight into the histogram
 	unsigned long long* cur = lds + a.pair_lds_off + 4 * wave;
 	uint8_t* const own = (uint8_t*)(lds + a.own_lds_off) + 64 * wave;
 	const bool hashed = MBG_WIN_MODE == MBG_HIST_HASH;
