@@ -59,9 +59,9 @@ struct program
 	generated gen;
 	gen_options opts;
 	int W = 1;
-	// lane kernels per flavour: 0 plain, 1 batched, 2 plain with run-time rates
-	// lane kernels per flavour; the plain flavours (0, 2) also per window and
-	// final-state storage mode, which they take as compile-time constants
+	// lane kernels per flavour (0 plain, 1 batched, 2 plain with run-time
+	// rates); the plain flavours also per window and final-state storage mode,
+	// which they take as compile-time constants
 	hipModule_t mods[3][4][3] = {};
 	hipFunction_t fns[3][4][3] = {};
 	double compile_seconds = 0;
