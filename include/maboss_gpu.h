@@ -92,7 +92,7 @@ typedef struct mbg_compile_opts
 	int32_t lane_occupancy;  /* waves per SIMD the lane kernels are compiled for
 	                            (register budget 512 / waves per lane); 0 = automatic:
 	                            4, 3 or 2 by node count, one fewer while the kernel
-	                            would spill registers (down to 2) */
+	                            would spill more than 4 registers (down to 2) */
 	int32_t reserved[4];
 } mbg_compile_opts;
 

@@ -266,10 +266,10 @@ hipFunction_t program::function(int flavour, int win_mode, int fin_mode)
 	const double t0 = wall_seconds();
 	// Occupancy (waves per SIMD; the compiler alone picks fewer waves with
 	// more registers each): the most waves whose registers fit without
-	// spilling, starting from 4 up to 40 nodes, 3 up to 160 and 2 above.  A
-	// spilled register is reloaded from scratch memory, and that load waits
-	// for every atomic the wave still has in flight (window-cell inserts), so
-	// a kernel that spills goes down one wave while it spills.  Measured on
+	// spilling more than 4 words, starting from 4 up to 40 nodes, 3 up to 160
+	// and 2 above.  A spilled register is reloaded from scratch memory, and
+	// that load waits for every atomic the wave still has in flight
+	// (window-cell inserts), so a kernel that spills more goes down one wave.  Measured on
 	// MI355X: at 30 nodes 4 waves beat the compiler's choice by 15 %; on the
 	// 100-node model 2 waves without spills beat 3 with 11 spilled registers
 	// by 5 %.
@@ -320,7 +320,8 @@ hipFunction_t program::function(int flavour, int win_mode, int fin_mode)
 			hip_check(hipModuleGetFunction(&fn, mod, "simulate"), "hipModuleGetFunction(simulate)");
 		int scratch = 0;
 		hip_check(hipFuncGetAttribute(&scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fn), "hipFuncGetAttribute");
-		if (scratch == 0 || waves <= 2 || opts.lane_waves > 0)
+		// a few spilled registers (up to 4 words) are kept: their reloads are rare
+		if (scratch <= 16 || waves <= 2 || opts.lane_waves > 0)
 			break;
 		(void)hipModuleUnload(mod);
 		mod = nullptr;
