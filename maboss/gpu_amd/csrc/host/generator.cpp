@@ -515,8 +515,9 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 		// run at 4 waves per SIMD with 128 registers) or every 4th: each
 		// group's terms are added before the next group is computed (no sum
 		// sunk to the end with every term kept live), while the scheduler may
-		// still interleave the group's nodes (on the 100-node model, at 2 waves,
-		// 2.5 % faster than a pin after every node)
+		// still interleave the group's nodes (with the selection pass below
+		// unpinned: the 50-node batched kernel 8 % faster, the 100-node plain
+		// kernel about 1 %)
 		if (two_pass && (N <= 40 || i % 4 == 3 || i == N - 1))
 			os << "\t\tMBG_PIN4(acc, S, T, nnz);\n";
 		os << "\t}\n";
@@ -621,8 +622,7 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 		os << "\t\tconst uint32_t xm = 0u - ((gx >> j) & 1u), msk = 0u - ((ge >> j) & 1u);\n";
 		os << "\t\tacc2 = acc2 + mbg_u2f((ub ^ ((ub ^ db) & xm)) & msk);\n";
 		// pinned per node up to 40 nodes (4 waves per SIMD, 128 registers);
-		// above, the compiler schedules the group freely (on the 100-node
-		// model, at 2 waves, 3 % faster)
+		// above, the compiler schedules the group freely
 		os << "\t\tcnt += (acc2 < rr) ? 1 : 0;\n" << (N <= 40 ? "\t\tMBG_PIN2(acc2, cnt);\n" : "") << "\t}\n";
 		os << "\tflip = gsel * " << k_group << " + cnt;\n";
 	}
