@@ -2,34 +2,38 @@
 
 One step = one pass of the simulate/statistics path over one batch: all
 trajectories of the workload simulated from their initial state to max_time,
-with window (hashed cells), final-state and fixed-point statistics accumulated in
-HBM.  The model tables and statistics storage are resident on the device when
-the timed region starts; the timed region covers K steps (statistics storage
-cleared, persistent simulate kernel, counters read back), bracketed by
-torch.cuda.synchronize().
+with window (hashed cells), final-state and fixed-point statistics
+accumulated in HBM, compacted and sorted on the device, and copied to pinned
+host memory.  The model tables are resident on the device when the timed
+region starts; the timed region covers K steps from the first kernel to the
+last statistic on the host, bracketed by torch.cuda.synchronize().  Steps are
+issued with mbg_simulate_async, so step k's copy to the host overlaps step
+k+1's kernels (at most two in flight).
 
 Default workload ``plain_100``: 100-node random network from seed 1 (4 state
 words, in-degree 1-4, AND/OR/NOT formulas), 50 internal nodes so the windows
 key on 50 observed bits (hashed window cells), rates lognormal(0, 1), 1e6
 trajectories, max_time 100, time_tick 1, continuous time.
 
-Prints one JSON line.  Extra fields: ``batched`` (256 variants in one launch
-vs launched one by one), ``grad`` (forward vs likelihood-ratio backward
-throughput), ``roofline`` (the simulate kernel against
-HBM with the reference's record traffic of 8W+16 bytes per transition),
-``roofline_valu`` (against the VALU issue rate with the per-transition op
-count 2*literals + 4N + 5*n_obs + 40), ``roofline_rate`` (the kernel's
-transitions/s against min(HBM/B, VALU/C)), ``cpu_baseline`` (the host oracle with
-compiled formulas on this machine's cores, bounded sample), and
-``host_inclusive`` (one step that also copies all statistics to the host), and
-``sorted_output`` (one step that leaves the window cells as a compact array sorted by
-key in HBM instead of the hash table).
+Prints one JSON line.  Besides ``value`` (the host-inclusive rate above):
+``device_only`` (the same steps with the statistics left in HBM),
+``latency`` (one synchronous step), ``roofline`` (the simulate kernel against
+the VALU issue rate with the per-transition op count 2*literals + 4N +
+5*n_obs + 40, which binds; ``hbm_effective`` inside it against HBM with the
+reference's record traffic of 8W+16 bytes per transition), ``scale_1000``
+(the 1000-node workload on the wave kernel, host-inclusive, with its own CPU
+baseline), ``discrete``, ``sorted_output``, ``batched`` (256 variants in one
+launch vs launched one by one), ``grad`` (forward vs likelihood-ratio
+backward throughput), ``cpu_baseline`` (the host oracle with compiled
+formulas on this machine's cores: median of 5 runs over bounded id prefixes)
+and ``cpu_config1`` (the oracle on the 10-node CPU workload in full).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import statistics
 import subprocess
 import sys
 import tempfile
@@ -62,62 +66,141 @@ def _ensure_built():
         __graft_entry__.build()
 
 
-def cpu_baseline(bnd, cfg, target_s, threads):
-    """The host oracle (compiled formulas, OpenMP) on a bounded prefix of the
-    same trajectories; transitions/s is intensive so the rates compare."""
+def cpu_threads():
+    """Threads the baseline uses: OMP_NUM_THREADS when set (the CPU share of
+    this job), else every CPU this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
+def host_cpu():
+    """CPU model, sockets, cores per socket, threads per core (lscpu) and the CPUs this process sees."""
+    info = {"logical_cpus": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(bnd, cfg, run_seconds, threads, min_ids=2000, runs=5):
+    """The host oracle (formulas compiled with g++, OpenMP over trajectory ids,
+    OMP_PROC_BIND=close, OMP_PLACES=cores) on prefixes of the same
+    trajectories: one warm-up run, then the median of `runs` runs.
+    Transitions/s is intensive, so the rates compare with the GPU's."""
     oracle = os.path.join(ROOT, "oracle", "build", "maboss_oracle")
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores", OMP_NUM_THREADS=str(threads))
 
     def run(n):
         p = subprocess.run([oracle, "--compiled", "--time", "--threads", str(threads), "--n", str(n), bnd, cfg],
-                           capture_output=True, text=True, timeout=600)
+                           capture_output=True, text=True, timeout=900, env=env)
         if p.returncode != 0:
             raise RuntimeError(p.stderr)
         return json.loads(p.stderr.strip().splitlines()[-1])
 
-    probe = run(2000)
-    n = int(min(1_000_000, max(2000, 2000 * target_s / max(probe["seconds"], 1e-3))))
-    res = run(n)
-    return {"value": res["transitions_per_s"], "unit": "transitions/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} trajectory ids of the same workload ({res['transitions']} transitions, "
-                      f"{res['seconds']:.1f} s, formulas compiled with g++ -O3 -march=native)",
+    probe = run(min(2000, min_ids))  # warm-up, also sizes the runs
+    n = int(min(1_000_000, max(min_ids, min(2000, min_ids) * run_seconds / max(probe["seconds"], 1e-3))))
+    res = [run(n) for _ in range(runs)]
+    rates = [r["transitions_per_s"] for r in res]
+    med = statistics.median(rates)
+    return {"value": med, "unit": "transitions/s", "cores": threads, "kind": "port",
+            "runs": rates,
+            "sample": f"median of {runs} runs over the first {n} trajectory ids of the same workload "
+                      f"({res[0]['transitions']} transitions, ~{res[0]['seconds']:.1f} s each) after one warm-up run; "
+                      f"formulas compiled with g++ -O3 -march=native; OMP_PROC_BIND=close OMP_PLACES=cores",
             "host": host_cpu()}
 
 
-def host_cpu():
-    """CPU model and logical CPU count of the machine the baseline ran on."""
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    return {"model": model, "logical_cpus": os.cpu_count(),
-            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+def pipelined_steps(prog, n, steps, **kw):
+    """K host-inclusive steps, at most two in flight; returns (wall s, transitions, sim ms per step, results)."""
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prev, trans, sims = None, 0, []
+    for _ in range(steps):
+        cur = prog.simulate(n_trajectories=n, wait=False, **kw)
+        if prev is not None:
+            prev.wait()
+            pi = prev.info
+            trans += pi.transitions
+            sims.append(pi.simulate_ms)
+        prev = cur
+    prev.wait()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    li = prev.info
+    trans += li.transitions
+    sims.append(li.simulate_ms)
+    return wall, trans, sims, li
+
+
+def device_only_steps(prog, n, steps, **kw):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    trans, sims, paths = 0, [], []
+    for _ in range(steps):
+        ri = prog.simulate(n_trajectories=n, device_only=True, **kw).info
+        trans += ri.transitions
+        sims.append(ri.simulate_ms)
+        paths.append(ri.kernel_ms)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, trans, sims, paths
 
 
 def bench_discrete(mg, netgen, wdir, steps):
     """The same 100-node network in discrete time (one record per step at window round(t / tick))."""
-    import torch
     bnd, cfg = netgen.write_workload("plain_100_discrete", wdir)
     model = mg.Model.load(bnd, cfg)
     prog = model.compile()
     n = model.info.sample_count
-    prog.simulate(n_trajectories=n, device_only=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    trans, sim_ms = 0, 0.0
-    for _ in range(steps):
-        ri = prog.simulate(n_trajectories=n, device_only=True).info
-        trans += ri.transitions
-        sim_ms += ri.simulate_ms
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    pipelined_steps(prog, n, 1)
+    wall, trans, sims, li = pipelined_steps(prog, n, steps)
     return {"workload": "plain_100_discrete: the plain_100 network with discrete_time = 1, 1e6 trajectories",
             "value": trans / wall, "unit": "transitions/s", "ms_per_step": wall / steps * 1e3,
-            "kernel_ms": sim_ms / steps, "window_mode": ri.window_mode, "transitions_per_step": trans / steps}
+            "kernel_ms": sum(sims) / len(sims), "window_mode": li.window_mode,
+            "window_cells": li.n_window_cells, "transitions_per_step": trans / steps}
+
+
+def bench_scale(mg, netgen, wdir, steps, cpu, cpu_seconds):
+    """The 1000-node workload (wave per trajectory), host-inclusive, against its own CPU baseline."""
+    bnd, cfg = netgen.write_workload("scale_1000", wdir)
+    model = mg.Model.load(bnd, cfg)
+    prog = model.compile()
+    info = model.info
+    n = info.sample_count
+    pipelined_steps(prog, n, 1)
+    wall, trans, sims, li = pipelined_steps(prog, n, steps)
+    rate = trans / wall
+    kernel_s = sum(sims) / len(sims) / 1e3
+    per = trans / steps
+    out = {"workload": f"scale_1000: {WORKLOAD_TEXT['scale_1000']}", "value": rate, "unit": "transitions/s",
+           "steps": steps, "ms_per_step": wall / steps * 1e3, "kernel": f"simulate_wave_n{_rows(info.n_nodes)}",
+           "kernel_ms": kernel_s * 1e3, "kernel_transitions_per_s": per / kernel_s,
+           "transitions_per_step": per, "window_cells": li.n_window_cells,
+           "hbm_effective_frac": (8 * info.n_words + 16) * per / kernel_s / 1e9 / HBM_PEAK_GBS}
+    if cpu:
+        cb = cpu_baseline(bnd, cfg, cpu_seconds, cpu_threads(), min_ids=20000)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = rate / cb["value"]
+        out["target_100x_met"] = rate / cb["value"] >= 100.0
+    return out
+
+
+def _rows(n_nodes):
+    rows = 1
+    while rows < (n_nodes + 63) // 64:
+        rows *= 2
+    return rows
 
 
 def bench_batched(mg, netgen, wdir, steps):
@@ -140,7 +223,10 @@ def bench_batched(mg, netgen, wdir, steps):
     one = {"transitions_per_s": trans / dev, "device_ms": dev / steps * 1e3, "wall_ms": wall / steps * 1e3,
            "block": info.block_size, "blocks_per_cu": info.blocks_per_cu, "traj_per_lane": info.traj_per_lane,
            "graph": bool(info.used_graph)}
-    sdev, swall, strans = 0.0, 0.0, 0
+    t0 = time.perf_counter()
+    _, hinfo = prog.simulate_batched(variants)
+    one["host_inclusive_transitions_per_s"] = hinfo.transitions / (time.perf_counter() - t0)
+    sdev, strans = 0.0, 0
     t0 = time.perf_counter()
     for v in variants:
         _, vi = prog.simulate_batched([v], device_only=True, use_graph=False, block_size=info.block_size,
@@ -182,7 +268,7 @@ def bench_grad(mg, netgen, wdir, steps):
     return {"workload": "grad_30: 30 nodes, 60 rate parameters, 5 observed, 1e6 trajectories, max_time 20",
             "forward_transitions_per_s": ft / fwd, "backward_transitions_per_s": bt / bwd,
             "forward_ms": fwd / steps * 1e3, "backward_ms": bwd / steps * 1e3,
-            "backward_over_forward_throughput": (bt / bwd) / (ft / fwd), "accumulator": "f64"}
+            "backward_over_forward_throughput": (bt / bwd) / (ft / fwd), "accumulator": mgrad.ACCUMULATOR}
 
 
 def main():
@@ -192,10 +278,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="plain_100", choices=sorted(WORKLOAD_TEXT))
     ap.add_argument("--trajectories", type=int, default=-1, help="override the workload's trajectory count")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=4.0, help="seconds of one CPU baseline run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-forms", action="store_true", help="skip the batched and gradient forms")
+    ap.add_argument("--no-forms", action="store_true", help="skip the batched, gradient, discrete and scale blocks")
     ap.add_argument("--no-sorted-output", action="store_true", help="skip the sorted-pairs step")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one plain device copy of this many MiB (calibrates HBM byte counters)")
@@ -229,52 +314,36 @@ def main():
     prog = model.compile()
     n = a.trajectories if a.trajectories > 0 else info.sample_count
 
-    for _ in range(a.warmup):
-        prog.simulate(n_trajectories=n, device_only=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    transitions, kernel_ms, sim_ms = 0, 0.0, 0.0
-    for _ in range(a.steps):
-        r = prog.simulate(n_trajectories=n, device_only=True)
-        ri = r.info
-        transitions += ri.transitions
-        kernel_ms += ri.kernel_ms
-        sim_ms += ri.simulate_ms
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    if a.warmup:
+        pipelined_steps(prog, n, a.warmup)
+    # the headline: K host-inclusive steps
+    elapsed, transitions, sims, li = pipelined_steps(prog, n, a.steps)
     per_step = transitions / a.steps
-    kernel_s = sim_ms / a.steps / 1e3  # the simulate kernel alone (HIP events on its stream)
-    path_s = kernel_ms / a.steps / 1e3  # simulate + tail expansion (+ window sort/reduce in the sorted mode)
-
-    # the same step with every statistic copied back to the host (PCIe, cells in key order)
+    kernel_s = sum(sims) / len(sims) / 1e3  # the simulate kernel alone (HIP events on its stream)
+    # the same steps with the statistics left in HBM
+    d_wall, d_trans, d_sims, d_paths = device_only_steps(prog, n, a.steps)
+    # one synchronous step: the latency of a single result
     t1 = time.perf_counter()
     full = prog.simulate(n_trajectories=n)
-    host_s = time.perf_counter() - t1
     fi = full.info
+    lat_s = time.perf_counter() - t1
+    del full
 
-    # the same step with the window cells left as a compact sorted array in HBM (sorted pairs)
     sorted_out = None
-    if ri.window_mode == 1 and not a.no_sorted_output:
-        prog.simulate(n_trajectories=n, device_only=True, window_mode="sort")
-        torch.cuda.synchronize()
-        ts = time.perf_counter()
-        so = prog.simulate(n_trajectories=n, device_only=True, window_mode="sort").info
-        torch.cuda.synchronize()
-        sorted_out = {"window_mode": "sort", "value": so.transitions / (time.perf_counter() - ts),
-                      "unit": "transitions/s", "device_path_ms": so.kernel_ms, "simulate_ms": so.simulate_ms,
-                      "window_pairs": so.window_pairs}
+    if li.window_mode == 1 and not a.no_sorted_output:
+        pipelined_steps(prog, n, 1, window_mode="sort")
+        s_wall, s_trans, s_sims, s_li = pipelined_steps(prog, n, max(1, min(a.steps, 3)), window_mode="sort")
+        sorted_out = {"window_mode": "sort", "value": s_trans / s_wall, "unit": "transitions/s",
+                      "simulate_ms": sum(s_sims) / len(s_sims), "window_pairs": s_li.window_pairs,
+                      "note": "window cells appended as (key, value) pairs and sort-reduced instead of hashed"}
 
     W, N, nobs = info.n_words, info.n_nodes, info.n_observed
     bytes_per = 8 * W + 16
     ops_per = 2 * info.total_literals + 4 * N + 5 * nobs + 40
-    achieved = bytes_per * per_step / kernel_s / 1e9
-    if ri.kernel == 1:  # wave per trajectory: rate rows = nodes per lane rounded up to a power of two
-        rows = 1
-        while rows < (N + 63) // 64:
-            rows *= 2
-        kname = f"simulate_wave_n{rows}"
-    else:
-        kname = "simulate"
+    kname = f"simulate_wave_n{_rows(N)}" if li.kernel == 1 else "simulate"
+    rate_k = per_step / kernel_s
+    bound_rate = min(HBM_PEAK_GBS * 1e9 / bytes_per, VALU_PEAK_TOPS * 1e12 / ops_per)
+    valu_bound = VALU_PEAK_TOPS * 1e12 / ops_per <= HBM_PEAK_GBS * 1e9 / bytes_per
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"{kname}_{a.workload}_traffic.md")
     if os.path.exists(tpath):
@@ -283,6 +352,8 @@ def main():
                 if line.startswith("bytes_per_launch:"):
                     traffic = float(line.split(":", 1)[1])
 
+    valu_ach = ops_per * rate_k / 1e12
+    hbm_ach = bytes_per * rate_k / 1e9
     out = {
         "metric": METRIC,
         "value": transitions / elapsed,
@@ -298,34 +369,44 @@ def main():
         "data": "synthetic (seeded network generator)",
         "config": {"workload": f"{a.workload}: {WORKLOAD_TEXT[a.workload]}", "trajectories": n,
                    "nodes": N, "state_words": W, "observed_nodes": nobs, "windows": info.windows,
-                   "form": "plain", "transitions_per_step": per_step},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kname, "kernel_ms": kernel_s * 1e3, "device_path_ms": path_s * 1e3,
-                     "bytes_per_transition": bytes_per,
-                     "note": "effective: the fused kernel writes no records; algorithmic bytes are the "
-                             "reference's record write + read, 8W+16 per transition"},
-        "roofline_valu": {"achieved": ops_per * per_step / kernel_s / 1e12, "peak": VALU_PEAK_TOPS,
-                          "unit": "T lane-ops/s", "ops_per_transition": ops_per,
-                          "frac": ops_per * per_step / kernel_s / 1e12 / VALU_PEAK_TOPS},
-        # the combined bound R* = min(HBM / B, VALU / C) transitions/s and the kernel's share of it
-        "roofline_rate": {"bound_transitions_per_s": min(HBM_PEAK_GBS * 1e9 / bytes_per, VALU_PEAK_TOPS * 1e12 / ops_per),
-                          "kernel_transitions_per_s": per_step / kernel_s,
-                          "frac": per_step / kernel_s / min(HBM_PEAK_GBS * 1e9 / bytes_per,
-                                                            VALU_PEAK_TOPS * 1e12 / ops_per)},
+                   "form": "plain", "transitions_per_step": per_step, "window_cells_per_step": li.n_window_cells,
+                   "timed": "first kernel to the last statistic in pinned host memory; steps pipelined "
+                            "(a step's copy overlaps the next step's kernels)"},
+        "roofline": {"bound": "valu" if valu_bound else "hbm",
+                     "achieved": valu_ach if valu_bound else hbm_ach,
+                     "peak": VALU_PEAK_TOPS if valu_bound else HBM_PEAK_GBS,
+                     "unit": "T lane-ops/s" if valu_bound else "GB/s",
+                     "frac": rate_k / bound_rate, "traffic": traffic,
+                     "kernel": kname, "kernel_ms": kernel_s * 1e3, "transitions_per_launch": per_step,
+                     "ops_per_transition": ops_per, "achieved_transitions_per_s": rate_k,
+                     "bound_transitions_per_s": bound_rate,
+                     "hbm_effective": {"achieved": hbm_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": hbm_ach / HBM_PEAK_GBS, "bytes_per_transition": bytes_per,
+                                       "note": "effective: the fused kernel writes no records; algorithmic "
+                                               "bytes are the reference's record write + read, 8W+16"}},
+        "device_only": {"value": d_trans / d_wall, "unit": "transitions/s", "ms_per_step": d_wall / a.steps * 1e3,
+                        "simulate_ms": sum(d_sims) / len(d_sims), "device_path_ms": sum(d_paths) / len(d_paths),
+                        "note": "statistics left in HBM (MBG_SIM_DEVICE_ONLY)"},
+        "latency": {"seconds": lat_s, "transitions_per_s": fi.transitions / lat_s, "device_path_ms": fi.kernel_ms,
+                    "window_cells": fi.n_window_cells,
+                    "note": "one synchronous mbg_simulate: kernel, compaction, sort, copy to pinned host memory"},
+        "host_over_device": (d_trans / d_wall) / (transitions / elapsed),
         "sorted_output": sorted_out,
-        "host_inclusive": {"value": fi.transitions / host_s, "unit": "transitions/s", "seconds": host_s,
-                           "window_cells": fi.n_window_cells},
         "compile_seconds": prog.compile_seconds,
     }
     if not a.no_forms and a.workload == "plain_100":
         out["discrete"] = bench_discrete(mg, netgen, wdir, max(1, min(a.steps, 3)))
     if not a.no_forms:
+        out["scale_1000"] = bench_scale(mg, netgen, wdir, 2, not a.no_cpu_baseline, max(a.cpu_seconds, 4.0))
         out["batched"] = bench_batched(mg, netgen, wdir, max(1, min(a.steps, 3)))
         out["grad"] = bench_grad(mg, netgen, wdir, max(1, min(a.steps, 3)))
     if not a.no_cpu_baseline:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(bnd, cfg, a.cpu_seconds, threads)
+        out["cpu_baseline"] = cpu_baseline(bnd, cfg, a.cpu_seconds, cpu_threads())
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        b1, c1 = netgen.write_workload("plain_small", wdir)
+        cb1 = cpu_baseline(b1, c1, 0.0, cpu_threads(), min_ids=10000)
+        cb1["sample"] = "config 1 in full: " + cb1["sample"]
+        out["cpu_config1"] = cb1
     print(json.dumps(out), flush=True)
 
 

@@ -141,9 +141,21 @@ typedef struct mbg_sim_params
 /* Fill p with the defaults above. */
 void mbg_sim_params_default(mbg_sim_params* p);
 /* Simulate and collect window, final-state and fixed-point statistics.
- * Inputs live in HBM for the whole run; the result is copied to the host
- * before returning (the stream is synchronised). */
+ * Inputs live in HBM for the whole run.  On the device the occupied cells of
+ * every table are compacted and sorted by key; they are then copied to
+ * pinned host memory, and the result is complete when this returns (the
+ * stream is synchronised). */
 int mbg_simulate(mbg_program* p, const mbg_sim_params* params, void* stream, mbg_result** out);
+/* The same run, returning once it is enqueued: the statistics reach the host
+ * on the program's own copy stream while the caller's stream moves on, so a
+ * following run's kernels overlap this run's copy.  Counters (transitions,
+ * finished, truncated) and the storage modes are final on return; the
+ * statistics after mbg_result_wait, which every mbg_result_* accessor calls
+ * first.  A program has two output slots: a third run in flight waits on the
+ * host for the first one's copy.  No counterpart in the reference, whose
+ * statistics copies are synchronous cudaMemcpy calls
+ * (src/statistics/window_average_small.cpp:91-107, final_states.cpp:45-51). */
+int mbg_simulate_async(mbg_program* p, const mbg_sim_params* params, void* stream, mbg_result** out);
 
 /* ---- batched variants.  No counterpart in the reference: its rates are
  *      baked into the generated code (src/generator.cpp:104-106), one model
@@ -228,6 +240,9 @@ int mbg_backward(mbg_program* p, const mbg_grad_params* params, const float* rat
 /* ---- results (replaces stats::finalize / visualize / write_csv,
  *      src/statistics/stats.h:15-29) ---- */
 void mbg_result_free(mbg_result* r);
+/* Block until the result's statistics are on the host (a no-op for results
+ * of mbg_simulate and after the first call). */
+int mbg_result_wait(mbg_result* r);
 
 typedef struct mbg_result_info
 {
@@ -245,9 +260,11 @@ typedef struct mbg_result_info
 	uint64_t n_fixed_points;         /* distinct fixed points */
 	uint64_t n_final_states;         /* distinct final observed states */
 	uint64_t n_window_cells;         /* nonzero (window, state) cells */
-	double kernel_ms;                /* device time of the whole device path (simulate kernel,
-	                                    tail expansion; sorted mode: window sort/reduce) */
-	double total_ms;                 /* host wall time of mbg_simulate */
+	double kernel_ms;                /* device time of the whole device path: simulate kernel,
+	                                    tail expansion, compaction and sort of the occupied cells
+	                                    (with MBG_SIM_DEVICE_ONLY: simulate and tail expansion) */
+	double total_ms;                 /* host wall time from the mbg_simulate(_async) call until the
+	                                    statistics were on the host */
 	int32_t window_mode, final_mode;
 	uint64_t window_capacity;
 	int32_t kernel;                  /* 0 lane per trajectory, 1 wave per trajectory */

@@ -142,9 +142,13 @@ class Program:
                  max_steps: int = 0, window_mode: str = "auto", final_mode: str = "auto",
                  window_capacity: int = 0, final_capacity: int = 0, fixed_capacity: int = 0,
                  block_size: int = 0, blocks_per_cu: int = 0, normalization: float = 0.0,
-                 device_only: bool = False, kernel: str = "auto", rates=None, stream=None) -> "Result":
+                 device_only: bool = False, kernel: str = "auto", rates=None, stream=None,
+                 wait: bool = True) -> "Result":
         """Simulate; ``rates`` (one float per ``Model.param_names``) replaces the model's
-        parameter values at run time (lane kernel)."""
+        parameter values at run time (lane kernel).  With ``wait=False`` the call returns
+        once the run is enqueued (``mbg_simulate_async``): the statistics reach the host on
+        the program's copy stream while the next run's kernels go ahead, and the result
+        completes in :meth:`Result.wait` (which every accessor calls)."""
         p = SimParams()
         lib().mbg_sim_params_default(ctypes.byref(p))
         p.n_trajectories = n_trajectories
@@ -170,7 +174,8 @@ class Program:
             p.rates = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         h = ctypes.c_void_p()
         s = ctypes.c_void_p(stream) if stream else None
-        check(lib().mbg_simulate(self._h, ctypes.byref(p), s, ctypes.byref(h)))
+        entry = lib().mbg_simulate if wait else lib().mbg_simulate_async
+        check(entry(self._h, ctypes.byref(p), s, ctypes.byref(h)))
         return Result(h, self.model)
 
     def simulate_batched(self, variants, seed: int = -1, max_steps: int = 0, block_size: int = 0,
@@ -246,6 +251,11 @@ class Result:
         out = ResultInfo()
         check(lib().mbg_result_get_info(self._h, ctypes.byref(out)))
         return out
+
+    def wait(self) -> "Result":
+        """Block until the statistics are on the host (runs started with ``wait=False``)."""
+        check(lib().mbg_result_wait(self._h))
+        return self
 
     def window(self, w: int) -> Window:
         n = lib().mbg_result_window_size(self._h, w)

@@ -30,10 +30,10 @@ EXPORTED = [
     "mbg_model_eval_rate",
     "mbg_generate_code", "mbg_free_string", "mbg_program_compile", "mbg_program_free",
     "mbg_program_compile_seconds",
-    "mbg_sim_params_default", "mbg_simulate",
+    "mbg_sim_params_default", "mbg_simulate", "mbg_simulate_async",
     "mbg_batch_params_default", "mbg_simulate_batched",
     "mbg_grad_params_default", "mbg_grad_check", "mbg_backward",
-    "mbg_result_free", "mbg_result_get_info", "mbg_result_window_size", "mbg_result_window",
+    "mbg_result_free", "mbg_result_wait", "mbg_result_get_info", "mbg_result_window_size", "mbg_result_window",
     "mbg_result_window_probs", "mbg_result_entropies", "mbg_result_final_size",
     "mbg_result_final", "mbg_result_fixed_size", "mbg_result_fixed",
     "mbg_result_write_csv", "mbg_result_print", "mbg_result_write_raw",
@@ -168,6 +168,7 @@ def _declare(lib):
         "mbg_program_compile_seconds": ([P], ctypes.c_double),
         "mbg_sim_params_default": ([ctypes.POINTER(SimParams)], None),
         "mbg_simulate": ([P, ctypes.POINTER(SimParams), P, pp], ctypes.c_int),
+        "mbg_simulate_async": ([P, ctypes.POINTER(SimParams), P, pp], ctypes.c_int),
         "mbg_batch_params_default": ([ctypes.POINTER(BatchParams)], None),
         "mbg_grad_params_default": ([ctypes.POINTER(GradParams)], None),
         "mbg_grad_check": ([P], ctypes.c_int),
@@ -175,6 +176,7 @@ def _declare(lib):
         "mbg_simulate_batched": ([P, ctypes.POINTER(BatchParams), i32, ctypes.POINTER(Variant), P, pp,
                                   ctypes.POINTER(BatchInfo)], ctypes.c_int),
         "mbg_result_free": ([P], None),
+        "mbg_result_wait": ([P], ctypes.c_int),
         "mbg_result_get_info": ([P, ctypes.POINTER(ResultInfo)], ctypes.c_int),
         "mbg_result_window_size": ([P, i32], i64),
         "mbg_result_window": ([P, i32, P, P], ctypes.c_int),

@@ -13,6 +13,8 @@
 //                    count those trajectories per (first full window, observed
 //                    state) and this kernel adds count * q(w) to each window w,
 //                    which is exactly the sum of the per-window additions.
+// mbg_table_compact -- occupied slots of a hash table as (key, value) pairs.
+// mbg_fp_compact   -- published slots of the fixed-point table as records.
 #include <hip/hip_runtime.h>
 
 #include "mbg_kernel_args.h"
@@ -74,24 +76,27 @@ extern "C" __global__ void __launch_bounds__(256) mbg_expand_tails(mbg_sim_args 
 	}
 }
 
-// The occupied cells of a hashed window table as (cell key, value) pairs in
-// the sorted mode's pair buffers, so the pair sort hands the host the cells
-// in key order.  Each block takes 16 x 256 consecutive slots, thread t every
-// 256th from its own: threads count their occupied slots, a block scan orders
-// them, and one atomic per block reserves the output range.
+// The occupied slots {key, value} of a hash table (window cells, final
+// states) as (key, value) pairs, so that a sort hands the host the cells in
+// key order.  Each block takes 16 x 256 consecutive slots, thread t every
+// 256th from its own (coalesced across the block): threads count their
+// occupied slots, a block scan orders them, and one atomic per block reserves
+// the output range.  Slots with a zero value are skipped.
 constexpr int k_compact_per_thread = 16;
-extern "C" __global__ void __launch_bounds__(256) mbg_table_pairs(mbg_sim_args a)
+extern "C" __global__ void __launch_bounds__(256) mbg_table_compact(const unsigned long long* table,
+																	unsigned long long cap, unsigned long long* keys,
+																	unsigned long long* vals,
+																	unsigned long long out_cap,
+																	unsigned long long* count)
 {
 	__shared__ unsigned int part[256];
 	__shared__ unsigned long long base;
-	const uint64_t cap = a.win_mask + 1;
-	// slot j of thread t: first + j * blockDim.x + t (coalesced across the block)
 	const uint64_t first = (uint64_t)blockIdx.x * blockDim.x * k_compact_per_thread + threadIdx.x;
 	unsigned int mine = 0;
 	for (int j = 0; j < k_compact_per_thread; ++j)
 	{
 		const uint64_t i = first + (uint64_t)j * blockDim.x;
-		if (i < cap && a.win_table[2 * i] != MBG_EMPTY_KEY && a.win_table[2 * i + 1] != 0ull)
+		if (i < cap && table[2 * i] != MBG_EMPTY_KEY && table[2 * i + 1] != 0ull)
 			mine++;
 	}
 	// inclusive scan of the per-thread counts over the block
@@ -105,7 +110,7 @@ extern "C" __global__ void __launch_bounds__(256) mbg_table_pairs(mbg_sim_args a
 		__syncthreads();
 	}
 	if (threadIdx.x == blockDim.x - 1)
-		base = atomicAdd(a.counters + MBG_CTR_PAIRS, (unsigned long long)part[threadIdx.x]);
+		base = atomicAdd(count, (unsigned long long)part[threadIdx.x]);
 	__syncthreads();
 	uint64_t o = base + part[threadIdx.x] - mine;
 	for (int j = 0; j < k_compact_per_thread; ++j)
@@ -113,16 +118,38 @@ extern "C" __global__ void __launch_bounds__(256) mbg_table_pairs(mbg_sim_args a
 		const uint64_t i = first + (uint64_t)j * blockDim.x;
 		if (i < cap)
 		{
-			const unsigned long long key = a.win_table[2 * i], val = a.win_table[2 * i + 1];
+			const unsigned long long key = table[2 * i], val = table[2 * i + 1];
 			if (key != MBG_EMPTY_KEY && val != 0ull)
 			{
-				if (o < a.pair_cap)
+				if (o < out_cap)
 				{
-					a.pair_keys[o] = key;
-					a.pair_vals[o] = val;
+					keys[o] = key;
+					vals[o] = val;
 				}
 				o++;
 			}
 		}
+	}
+}
+
+// The published slots of a fixed-point table (slots of `stride` words {tag,
+// count, key words}; tag 2 = complete) as records of stride - 1 words {count,
+// key words}, in no particular order (the host orders them).
+extern "C" __global__ void __launch_bounds__(256) mbg_fp_compact(const unsigned int* table, unsigned long long cap,
+																 int stride, unsigned int* out,
+																 unsigned long long out_cap,
+																 unsigned long long* count)
+{
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x)
+	{
+		const unsigned int* e = table + i * (uint64_t)stride;
+		if (e[0] != 2u || e[1] == 0u)
+			continue;
+		const unsigned long long o = atomicAdd(count, 1ull);
+		if (o >= out_cap)
+			continue;
+		unsigned int* r = out + o * (uint64_t)(stride - 1);
+		for (int w = 1; w < stride; ++w)
+			r[w - 1] = e[w];
 	}
 }

@@ -73,6 +73,7 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 	scope_timer tm("batched> simulate");
 	const double t0 = wall_seconds();
 	const model& m = *p.mdl;
+	p.order_after_last_run(stream);
 	const int W = m.n_words(), N = m.n_nodes(), n_obs = m.n_observed();
 	if (W > max_lane_words)
 		throw error(MBG_E_LIMIT, "the batched form serves models of up to 256 nodes");
@@ -332,9 +333,44 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 	else
 		ms = timed([&] { launch(block, grid); });
 
-	std::vector<unsigned long long> ctr(MBG_NCOUNTERS), vc((size_t)L.nv * 3);
-	hip_check(hipMemcpy(ctr.data(), counters, MBG_NCOUNTERS * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-	hip_check(hipMemcpy(vc.data(), vctr, vc.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+	// ---- occupied cells compacted on the device (in key order: the dense
+	// arrays are compacted in index order), then one copy into pinned memory
+	auto* cnt = (unsigned long long*)p.b_bcnt.ensure(4 * 8);
+	unsigned long long *ck = nullptr, *cv = nullptr, *fk = nullptr, *fv = nullptr;
+	unsigned int* fpc = nullptr;
+	const size_t rec = L.fp_stride - 1; // {count, state words, variant}
+	if (!bp.device_only)
+	{
+		hip_check(hipMemsetAsync(cnt, 0, 4 * 8, stream), "hipMemsetAsync");
+		ck = (unsigned long long*)p.b_bck.ensure(std::max<uint64_t>(L.win_cells, 1) * 8);
+		cv = (unsigned long long*)p.b_bcv.ensure(std::max<uint64_t>(L.win_cells, 1) * 8);
+		compact_dense(p.b_sort_tmp, win, L.win_cells, ck, cv, cnt + 0, stream);
+		fk = (unsigned long long*)p.b_bfk.ensure(std::max<uint64_t>(L.fin_cells, 1) * 8);
+		fv = (unsigned long long*)p.b_bfv.ensure(std::max<uint64_t>(L.fin_cells, 1) * 8);
+		compact_dense(p.b_sort_tmp, fin, L.fin_cells, fk, fv, cnt + 1, stream);
+		fpc = (unsigned int*)p.b_bfpc.ensure(L.fp_cap * rec * 4);
+		p.load_util();
+		{
+			int stride = (int)L.fp_stride;
+			unsigned long long* c2 = cnt + 2;
+			void* args[] = { &fp, &L.fp_cap, &stride, &fpc, &L.fp_cap, &c2 };
+			const unsigned g = (unsigned)std::min<uint64_t>(4096, (L.fp_cap + 255) / 256);
+			hip_check(hipModuleLaunchKernel(p.fn_fp_compact, g, 1, 1, 256, 1, 1, 0, stream, args, nullptr),
+					  "hipModuleLaunchKernel(mbg_fp_compact)");
+		}
+	}
+	if (!p.mailbox)
+		p.mailbox = p.pool->get(4096);
+	auto* mb = (unsigned long long*)p.mailbox.get();
+	std::vector<unsigned long long> vc((size_t)L.nv * 3);
+	std::shared_ptr<char> vblk = p.pool->get(vc.size() * 8 + 256);
+	hip_check(hipMemcpyAsync(mb, counters, MBG_NCOUNTERS * 8, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+	hip_check(hipMemcpyAsync(mb + 16, cnt, 4 * 8, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+	hip_check(hipMemcpyAsync(vblk.get(), vctr, vc.size() * 8, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+	hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+	std::memcpy(vc.data(), vblk.get(), vc.size() * 8);
+	unsigned long long ctr[MBG_NCOUNTERS];
+	std::memcpy(ctr, mb, sizeof(ctr));
 	if (ctr[MBG_CTR_FP_TIMEOUT])
 		throw error(MBG_E_INTERNAL, "fixed-point table: a slot was never published");
 	if (ctr[MBG_CTR_FP_OVERFLOW])
@@ -353,27 +389,61 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 		timing->graph = used_graph;
 	}
 
-	// ---- gather per variant
-	std::vector<unsigned long long> hwin, hent, hfin;
-	std::vector<unsigned int> hfp;
+	const uint64_t n_cells = bp.device_only ? 0 : mb[16], n_fin = bp.device_only ? 0 : mb[17],
+				   n_fp = bp.device_only ? 0 : mb[18];
+	const size_t o_cv = (n_cells * 8 + 255) & ~(size_t)255, o_ent = o_cv + ((n_cells * 8 + 255) & ~(size_t)255);
+	const size_t o_fk = o_ent + ((L.ent_cells * 8 + 255) & ~(size_t)255), o_fv = o_fk + ((n_fin * 8 + 255) & ~(size_t)255);
+	const size_t o_fp = o_fv + ((n_fin * 8 + 255) & ~(size_t)255), total_bytes = o_fp + n_fp * rec * 4;
+	std::shared_ptr<char> blk;
 	if (!bp.device_only)
 	{
-		hwin.resize(L.win_cells);
-		hent.resize(L.ent_cells);
-		hfin.resize(L.fin_cells);
-		hfp.resize(L.fp_cap * L.fp_stride);
-		hip_check(hipMemcpy(hwin.data(), win, L.win_cells * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-		hip_check(hipMemcpy(hent.data(), ent, L.ent_cells * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-		hip_check(hipMemcpy(hfin.data(), fin, L.fin_cells * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-		hip_check(hipMemcpy(hfp.data(), fp, hfp.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+		blk = p.pool->get(total_bytes);
+		char* h = blk.get();
+		auto d2h = [&](size_t off, const void* src, size_t bytes) {
+			if (bytes)
+				hip_check(hipMemcpyAsync(h + off, src, bytes, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+		};
+		d2h(0, ck, n_cells * 8);
+		d2h(o_cv, cv, n_cells * 8);
+		d2h(o_ent, ent, L.ent_cells * 8);
+		d2h(o_fk, fk, n_fin * 8);
+		d2h(o_fv, fv, n_fin * 8);
+		d2h(o_fp, fpc, n_fp * rec * 4);
+		hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 	}
-	std::vector<std::vector<std::pair<std::vector<uint32_t>, uint64_t>>> fixed(L.nv);
-	for (uint64_t i = 0; i < (bp.device_only ? 0 : L.fp_cap); i++)
+	p.mark_run_done(stream);
+
+	// ---- gather per variant: views into the shared block
+	cell_view all_win, all_fin;
+	std::vector<std::vector<uint64_t>> fp_of(L.nv);
+	if (!bp.device_only)
 	{
-		const unsigned int* e = &hfp[i * L.fp_stride];
-		if (e[0] == 2u && e[1] && e[2 + W] < (unsigned)L.nv)
-			fixed[e[2 + W]].emplace_back(std::vector<uint32_t>(e + 2, e + 2 + W), e[1]);
+		const char* h = blk.get();
+		all_win.hold = blk;
+		all_win.key = (const uint64_t*)h;
+		all_win.val = (const uint64_t*)(h + o_cv);
+		all_win.n = n_cells;
+		all_fin.hold = blk;
+		all_fin.key = (const uint64_t*)(h + o_fk);
+		all_fin.val = (const uint64_t*)(h + o_fv);
+		all_fin.n = n_fin;
+		const uint32_t* r = (const uint32_t*)(h + o_fp);
+		for (uint64_t i = 0; i < n_fp; i++)
+		{
+			const uint32_t v = r[i * rec + 1 + W];
+			if (v < (unsigned)L.nv)
+				fp_of[v].push_back(i);
+		}
 	}
+	auto sub_view = [](const cell_view& all, uint64_t first, uint64_t end) {
+		cell_view c = all;
+		const uint64_t lo = all.lower_bound(first), hi = all.lower_bound(end);
+		c.key = all.key + lo;
+		c.val = all.val + lo;
+		c.n = hi - lo;
+		c.sub = first;
+		return c;
+	};
 	const uint64_t per = 1ull << n_obs;
 	for (int v = 0; v < L.nv; v++)
 	{
@@ -396,28 +466,20 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 		r->window_mode = MBG_HIST_DENSE;
 		r->final_mode = MBG_HIST_DENSE;
 		r->ent.assign(L.windows[v], 0);
-		r->win.assign(L.windows[v], {});
 		if (!bp.device_only)
 		{
-			for (int w = 0; w < L.windows[v]; w++)
+			r->win = sub_view(all_win, L.win_off[v], L.win_off[v] + ((uint64_t)L.windows[v] << n_obs));
+			r->fin = sub_view(all_fin, L.fin_off[v], L.fin_off[v] + per);
+			std::memcpy(r->ent.data(), blk.get() + o_ent + 8 * (size_t)L.ent_off[v], 8 * (size_t)L.windows[v]);
+			const uint32_t* rr = (const uint32_t*)(blk.get() + o_fp);
+			for (uint64_t i : fp_of[v])
 			{
-				r->ent[w] = hent[L.ent_off[v] + w];
-				const unsigned long long* c = &hwin[L.win_off[v] + (uint64_t)w * per];
-				for (uint64_t s = 0; s < per; s++)
-					if (c[s])
-						r->win[w].emplace_back(s, c[s]);
+				r->fp_count.push_back(rr[i * rec]);
+				r->fp_words.insert(r->fp_words.end(), rr + i * rec + 1, rr + i * rec + 1 + W);
 			}
-			for (uint64_t s = 0; s < per; s++)
-				if (hfin[L.fin_off[v] + s])
-					r->fin.emplace_back(s, hfin[L.fin_off[v] + s]);
-			r->fixed = std::move(fixed[v]);
-			std::sort(r->fixed.begin(), r->fixed.end(), [](const auto& x, const auto& y) {
-				for (int w = (int)x.first.size() - 1; w >= 0; w--)
-					if (x.first[w] != y.first[w])
-						return x.first[w] < y.first[w];
-				return false;
-			});
+			r->sort_fixed();
 		}
+		r->index_cells();
 		out.push_back(std::move(r));
 	}
 	const double total = (wall_seconds() - t0) * 1e3;

@@ -274,7 +274,7 @@ void mbg_sim_params_default(mbg_sim_params* p)
 	p->kernel = -1;
 }
 
-int mbg_simulate(mbg_program* p, const mbg_sim_params* params, void* stream, mbg_result** out)
+static int simulate_entry(mbg_program* p, const mbg_sim_params* params, void* stream, mbg_result** out, bool async)
 {
 	return guarded([&] {
 		if (!p || !out)
@@ -303,9 +303,20 @@ int mbg_simulate(mbg_program* p, const mbg_sim_params* params, void* stream, mbg
 		if (sp.block_size < 0 || sp.block_size > 1024 || (sp.block_size % 64) != 0)
 			throw mbg::error(mbg::MBG_E_ARG, "block_size must be a multiple of 64 up to 1024");
 		auto rr = std::make_unique<mbg_result>();
-		rr->r = mbg::simulate(*p->p, sp, (hipStream_t)stream);
+		rr->r = async ? mbg::simulate_async(*p->p, sp, (hipStream_t)stream)
+					  : mbg::simulate(*p->p, sp, (hipStream_t)stream);
 		*out = rr.release();
 	});
+}
+
+int mbg_simulate(mbg_program* p, const mbg_sim_params* params, void* stream, mbg_result** out)
+{
+	return simulate_entry(p, params, stream, out, false);
+}
+
+int mbg_simulate_async(mbg_program* p, const mbg_sim_params* params, void* stream, mbg_result** out)
+{
+	return simulate_entry(p, params, stream, out, true);
 }
 
 void mbg_batch_params_default(mbg_batch_params* p)
@@ -417,12 +428,22 @@ int mbg_backward(mbg_program* p, const mbg_grad_params* params, const float* rat
 
 void mbg_result_free(mbg_result* r) { delete r; }
 
+int mbg_result_wait(mbg_result* rr)
+{
+	return guarded([&] {
+		if (!rr)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		rr->r->wait();
+	});
+}
+
 int mbg_result_get_info(const mbg_result* rr, mbg_result_info* out)
 {
 	return guarded([&] {
 		if (!rr || !out)
 			throw mbg::error(mbg::MBG_E_ARG, "null argument");
 		const mbg::result& r = *rr->r;
+		r.wait();
 		std::memset(out, 0, sizeof(*out));
 		out->windows = r.windows;
 		out->n_observed = r.n_obs;
@@ -435,12 +456,9 @@ int mbg_result_get_info(const mbg_result* rr, mbg_result_info* out)
 		out->transitions = r.transitions;
 		out->finished = r.finished;
 		out->truncated = r.truncated;
-		out->n_fixed_points = r.fixed.size();
-		out->n_final_states = r.fin.size();
-		uint64_t cells = 0;
-		for (auto& w : r.win)
-			cells += w.size();
-		out->n_window_cells = cells;
+		out->n_fixed_points = r.n_fixed();
+		out->n_final_states = r.fin.n;
+		out->n_window_cells = r.n_cells();
 		out->kernel_ms = r.kernel_ms;
 		out->total_ms = r.total_ms;
 		out->simulate_ms = r.simulate_ms;
@@ -456,7 +474,16 @@ int64_t mbg_result_window_size(const mbg_result* rr, int32_t w)
 {
 	if (!rr || w < 0 || w >= rr->r->windows)
 		return -1;
-	return (int64_t)rr->r->win[w].size();
+	try
+	{
+		rr->r->wait();
+	}
+	catch (const std::exception& e)
+	{
+		g_last_error = e.what();
+		return -1;
+	}
+	return (int64_t)(rr->r->woff[w + 1] - rr->r->woff[w]);
 }
 
 int mbg_result_window(const mbg_result* rr, int32_t w, uint64_t* states, uint64_t* raw)
@@ -464,14 +491,16 @@ int mbg_result_window(const mbg_result* rr, int32_t w, uint64_t* states, uint64_
 	return guarded([&] {
 		if (!rr || w < 0 || w >= rr->r->windows)
 			throw mbg::error(mbg::MBG_E_ARG, "bad window");
+		const mbg::result& r = *rr->r;
+		r.wait();
+		const uint64_t om = r.obs_mask();
 		size_t i = 0;
-		for (auto& c : rr->r->win[w])
+		for (uint64_t c = r.woff[w]; c < r.woff[w + 1]; c++, i++)
 		{
 			if (states)
-				states[i] = c.first;
+				states[i] = r.win.k(c) & om;
 			if (raw)
-				raw[i] = c.second;
-			i++;
+				raw[i] = r.win.v(c);
 		}
 	});
 }
@@ -481,14 +510,16 @@ int mbg_result_window_probs(const mbg_result* rr, int32_t w, uint64_t* states, d
 	return guarded([&] {
 		if (!rr || w < 0 || w >= rr->r->windows)
 			throw mbg::error(mbg::MBG_E_ARG, "bad window");
+		const mbg::result& r = *rr->r;
+		r.wait();
+		const uint64_t om = r.obs_mask();
 		size_t i = 0;
-		for (auto& c : rr->r->win[w])
+		for (uint64_t c = r.woff[w]; c < r.woff[w + 1]; c++, i++)
 		{
 			if (states)
-				states[i] = c.first;
+				states[i] = r.win.k(c) & om;
 			if (probs)
-				probs[i] = rr->r->prob(c.second);
-			i++;
+				probs[i] = r.prob(r.win.v(c));
 		}
 	});
 }
@@ -498,6 +529,7 @@ int mbg_result_entropies(const mbg_result* rr, double* th, double* h)
 	return guarded([&] {
 		if (!rr)
 			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		rr->r->wait();
 		for (int w = 0; w < rr->r->windows; w++)
 		{
 			if (th)
@@ -508,41 +540,69 @@ int mbg_result_entropies(const mbg_result* rr, double* th, double* h)
 	});
 }
 
-int64_t mbg_result_final_size(const mbg_result* rr) { return rr ? (int64_t)rr->r->fin.size() : -1; }
+int64_t mbg_result_final_size(const mbg_result* rr)
+{
+	if (!rr)
+		return -1;
+	try
+	{
+		rr->r->wait();
+	}
+	catch (const std::exception& e)
+	{
+		g_last_error = e.what();
+		return -1;
+	}
+	return (int64_t)rr->r->fin.n;
+}
 
 int mbg_result_final(const mbg_result* rr, uint64_t* states, uint64_t* counts)
 {
 	return guarded([&] {
 		if (!rr)
 			throw mbg::error(mbg::MBG_E_ARG, "null argument");
-		size_t i = 0;
-		for (auto& f : rr->r->fin)
+		const mbg::result& r = *rr->r;
+		r.wait();
+		for (uint64_t i = 0; i < r.fin.n; i++)
 		{
 			if (states)
-				states[i] = f.first;
+				states[i] = r.fin.k(i);
 			if (counts)
-				counts[i] = f.second;
-			i++;
+				counts[i] = r.fin.v(i);
 		}
 	});
 }
 
-int64_t mbg_result_fixed_size(const mbg_result* rr) { return rr ? (int64_t)rr->r->fixed.size() : -1; }
+int64_t mbg_result_fixed_size(const mbg_result* rr)
+{
+	if (!rr)
+		return -1;
+	try
+	{
+		rr->r->wait();
+	}
+	catch (const std::exception& e)
+	{
+		g_last_error = e.what();
+		return -1;
+	}
+	return (int64_t)rr->r->n_fixed();
+}
 
 int mbg_result_fixed(const mbg_result* rr, uint32_t* states, uint64_t* counts)
 {
 	return guarded([&] {
 		if (!rr)
 			throw mbg::error(mbg::MBG_E_ARG, "null argument");
-		const size_t W = (size_t)rr->r->n_words;
-		size_t i = 0;
-		for (auto& f : rr->r->fixed)
+		const mbg::result& r = *rr->r;
+		r.wait();
+		const size_t W = (size_t)r.n_words;
+		for (uint64_t i = 0; i < r.n_fixed(); i++)
 		{
 			if (states)
-				std::memcpy(states + i * W, f.first.data(), W * 4);
+				std::memcpy(states + i * W, r.fixed_state(i), W * 4);
 			if (counts)
-				counts[i] = f.second;
-			i++;
+				counts[i] = r.fp_count[i];
 		}
 	});
 }
@@ -552,6 +612,7 @@ int mbg_result_write_csv(const mbg_result* rr, const char* prefix, int with_fina
 	return guarded([&] {
 		if (!rr || !prefix)
 			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		rr->r->wait();
 		rr->r->write_csv(prefix, with_finals != 0);
 	});
 }
@@ -561,6 +622,7 @@ int mbg_result_print(const mbg_result* rr, const char* path)
 	return guarded([&] {
 		if (!rr)
 			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		rr->r->wait();
 		if (!path)
 		{
 			rr->r->print(std::cout);
@@ -581,6 +643,7 @@ int mbg_result_write_raw(const mbg_result* rr, const char* path)
 		std::ofstream f(path);
 		if (!f)
 			throw mbg::error(mbg::MBG_E_IO, std::string("cannot write ") + path);
+		rr->r->wait();
 		rr->r->write_raw(f);
 	});
 }

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -44,16 +46,83 @@ struct device_buffer
 	~device_buffer() { release(); }
 };
 
-// The reference's kernel_compiler (src/kernel_compiler.{h,cpp}): generated
-// source -> hipRTC bitcode -> link with the precompiled kernel bitcode ->
-// code object -> module and kernel handles.
+// Pinned (page-locked) host blocks for statistics copied back from the
+// device, kept for reuse: pinning hundreds of MB costs far more than the copy
+// itself, so a block released by a result goes back to the pool and the next
+// run of the same size takes it.  Thread safe.
+class pinned_pool : public std::enable_shared_from_this<pinned_pool>
+{
+public:
+	~pinned_pool();
+	// A block of at least `bytes` bytes; the block returns to the pool when
+	// the last reference goes.
+	std::shared_ptr<char> get(size_t bytes);
+
+private:
+	void put(char* p, size_t bytes);
+	std::mutex mu_;
+	std::vector<std::pair<size_t, char*>> free_;
+	size_t retained_ = 0;
+};
+
+// A sorted run of (key, value) cells in host memory: window cells keyed by
+// (window << n_obs) | observed state, or final states keyed by the observed
+// state.  `hold` keeps the memory alive (a pinned block shared by the results
+// of one launch, or a vector); `sub` is subtracted from every stored key (the
+// first cell of a variant in a batched launch).
+struct cell_view
+{
+	std::shared_ptr<const void> hold;
+	const uint64_t* key = nullptr;
+	const uint64_t* val = nullptr;
+	uint64_t n = 0;
+	uint64_t sub = 0;
+	uint64_t k(uint64_t i) const { return key[i] - sub; }
+	uint64_t v(uint64_t i) const { return val[i]; }
+	// index of the first cell whose key is >= kk
+	uint64_t lower_bound(uint64_t kk) const;
+	// cells from owned vectors
+	static cell_view from(std::vector<uint64_t> keys, std::vector<uint64_t> vals);
+};
+
+// Device storage a run's statistics leave the device from.  A program keeps
+// two, used in turn, so that one run's copy to the host (on the program's
+// copy stream) overlaps the next run's kernels; a slot is reused only after
+// the copy of its previous run has completed.
+struct out_slot
+{
+	// window cells: compacted (hashed / dense modes) or appended (sorted
+	// mode) pairs, the sort's alternate buffers
+	device_buffer pk, pv, pka, pva;
+	device_buffer ent;            // [windows] entropy sums
+	device_buffer fk, fv, fka, fva; // final states, compacted
+	device_buffer fpc;            // fixed points, compacted records {count, words}
+	device_buffer cnt;            // [4] counts of the compactions
+	hipEvent_t copied = nullptr;  // the last copy out of this slot
+	bool pending = false;         // copied has been recorded
+	~out_slot();
+};
+
+// The reference's kernel_compiler (src/kernel_compiler.{h,cpp}) plus the run
+// workspace: generated source -> hipRTC -> code object -> module and kernel
+// handles, and the device storage reused by successive runs.
 struct program
 {
 	// run workspace (kept between runs so repeated launches do not reallocate)
-	device_buffer b_counters, b_win_vals, b_win_keys, b_ent, b_fin_vals, b_fin_keys, b_fp, b_istate, b_pv, b_wave;
-	// sorted window mode: appended pairs, their sort scratch, unique count
-	device_buffer b_pk, b_pv2, b_pk_alt, b_pv_alt, b_sort_tmp, b_ucount;
+	device_buffer b_counters, b_win_vals, b_win_keys, b_fin_vals, b_fin_keys, b_fp, b_istate, b_pv, b_wave;
+	device_buffer b_sort_tmp;
 	uint64_t pairs_per_traj = 0; // learned from the last sorted run (capacity hint)
+	out_slot slots[2];
+	int slot_next = 0;
+	hipStream_t copy_stream = nullptr; // device-to-host copies of the statistics
+	std::shared_ptr<pinned_pool> pool = std::make_shared<pinned_pool>();
+	std::shared_ptr<char> mailbox;     // small pinned block: counters and counts read mid-run
+	// recorded after the last device work a run enqueued on its stream; the
+	// next run (which may come on another stream) waits for it, since both
+	// use the shared workspace above
+	hipEvent_t run_done = nullptr;
+	void order_after_last_run(hipStream_t stream);
+	void mark_run_done(hipStream_t stream);
 
 	std::shared_ptr<const model> mdl;
 	generated gen;
@@ -71,10 +140,13 @@ struct program
 
 	hipFunction_t fn_fill = nullptr; // mbg_fill_u64 of the parametric module
 
-	// utility kernels (slot fill, tail expansion), precompiled
+	// utility kernels (slot fill, tail expansion, table compaction), precompiled
 	hipModule_t mod_util = nullptr;
-	hipFunction_t fn_fill_slots = nullptr, fn_expand_tails = nullptr, fn_table_pairs = nullptr;
+	hipFunction_t fn_fill_slots = nullptr, fn_expand_tails = nullptr, fn_table_compact = nullptr,
+				  fn_fp_compact = nullptr;
 	void load_util();
+	// waits for every copy in flight (before buffers are released)
+	void drain_copies();
 	device_buffer b_tail;
 
 	// batched form: launch shape chosen by the autotuner, keyed by the batch
@@ -87,6 +159,7 @@ struct program
 	hipGraphExec_t batch_graph = nullptr;
 	uint64_t batch_graph_key = 0;
 	device_buffer b_bwin, b_bent, b_bfin, b_bfp, b_bctr, b_bvar, b_bchunk, b_btab;
+	device_buffer b_bck, b_bcv, b_bfk, b_bfv, b_bfpc, b_bcnt; // compacted cells of a batched launch
 
 	// likelihood-ratio gradient kernel (compiled per model on first use)
 	std::unique_ptr<grad_code> gcode;
@@ -141,11 +214,14 @@ struct result
 	std::vector<std::string> names;
 	std::vector<int> observed;
 
-	// statistics; raw values are fixed point (see simulate_lane.hip)
-	std::vector<std::vector<std::pair<uint64_t, uint64_t>>> win; // per window (state, raw), sorted
-	std::vector<uint64_t> ent;                                    // per window
-	std::vector<std::pair<uint64_t, uint64_t>> fin;               // (state, count), sorted
-	std::vector<std::pair<std::vector<uint32_t>, uint64_t>> fixed; // reference order
+	// statistics; raw values are fixed point (see simulate_lane.hip).  Valid
+	// after wait() (a run may still be copying them to the host).
+	cell_view win;                  // window cells sorted by (window << n_obs) | state
+	std::vector<uint64_t> woff;     // [windows + 1]: window w's cells are [woff[w], woff[w + 1])
+	std::vector<uint64_t> ent;      // per window
+	cell_view fin;                  // final states sorted by state
+	std::vector<uint32_t> fp_words; // fixed points, n_words words each, reference order
+	std::vector<uint64_t> fp_count;
 
 	uint64_t transitions = 0, finished = 0, truncated = 0;
 	double kernel_ms = 0, total_ms = 0, simulate_ms = 0;
@@ -158,14 +234,33 @@ struct result
 	double th(int w) const;              // transition entropy of window w
 	double h(int w) const;               // state entropy of window w (float, as the reference)
 	std::string state_name(uint64_t obs) const;
-	std::string full_state_name(const std::vector<uint32_t>& s) const;
+	std::string full_state_name(const uint32_t* s) const;
+	uint64_t n_cells() const { return woff.empty() ? 0 : woff.back() - woff.front(); }
+	uint64_t n_fixed() const { return fp_count.size(); }
+	const uint32_t* fixed_state(uint64_t i) const { return fp_words.data() + i * (size_t)n_words; }
+	uint64_t obs_mask() const { return n_obs >= 64 ? ~0ull : ((1ull << n_obs) - 1ull); }
+	// The cells, window offsets and fixed-point order from what the run left
+	// in the views (windows from the cell keys; fixed points sorted most
+	// significant word first, src/state.h:43-49).
+	void index_cells();
+	void sort_fixed();
 
 	void write_csv(const std::string& prefix, bool with_finals) const;
 	void print(std::ostream& os) const;
 	void write_raw(std::ostream& os) const;
+
+	// Completion of a run whose copy is still in flight: wait() blocks until
+	// it has landed and then indexes it.  Safe to call more than once.
+	void wait() const;
+	mutable std::function<void()> on_ready;
+	~result();
 };
 
 std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t stream);
+// As simulate(), but returns once the run is enqueued: the statistics are
+// copied to pinned host memory on the program's copy stream and the result
+// completes in wait().
+std::unique_ptr<result> simulate_async(program& p, const sim_params& sp, hipStream_t stream);
 // Window and final-state storage modes a default run of the model uses.
 void default_storage_modes(const model& m, int& win_mode, int& fin_mode);
 
@@ -208,6 +303,16 @@ void sort_reduce_pairs(device_buffer& temp, unsigned long long* keys, unsigned l
 					   unsigned long long* keys_alt, unsigned long long* vals_alt, uint64_t n, int key_bits,
 					   unsigned long long* unique_count, hipStream_t stream, unsigned long long** out_keys,
 					   unsigned long long** out_vals);
+// The same sort without the reduction (the keys are already distinct: cells
+// compacted from a hash table).  *out_keys / *out_vals: the buffer pair that
+// holds the sorted pairs.
+void sort_pairs(device_buffer& temp, unsigned long long* keys, unsigned long long* vals, unsigned long long* keys_alt,
+				unsigned long long* vals_alt, uint64_t n, int key_bits, hipStream_t stream,
+				unsigned long long** out_keys, unsigned long long** out_vals);
+// Nonzero entries of a dense array, in index order: out_keys receives their
+// indices, out_vals their values and *count (device) their number.
+void compact_dense(device_buffer& temp, const unsigned long long* dense, uint64_t n, unsigned long long* out_keys,
+				   unsigned long long* out_vals, unsigned long long* count, hipStream_t stream);
 
 // ---- gradient form
 struct grad_params

@@ -158,6 +158,12 @@ std::vector<char> compile_to_code_object(const std::string& source)
 
 program::~program()
 {
+	// the copies in flight read the slots' buffers
+	drain_copies();
+	if (copy_stream)
+		(void)hipStreamDestroy(copy_stream);
+	if (run_done)
+		(void)hipEventDestroy(run_done);
 	for (auto& by_win : mods)
 		for (auto& by_fin : by_win)
 			for (hipModule_t m : by_fin)
@@ -229,8 +235,9 @@ void program::load_util()
 	hip_check(hipModuleGetFunction(&fn_expand_tails, mod_util, "mbg_expand_tails"),
 			  "hipModuleGetFunction(mbg_expand_tails)");
 	hip_check(hipModuleGetFunction(&fn_fill_u64, mod_util, "mbg_fill_u64"), "hipModuleGetFunction(mbg_fill_u64)");
-	hip_check(hipModuleGetFunction(&fn_table_pairs, mod_util, "mbg_table_pairs"),
-			  "hipModuleGetFunction(mbg_table_pairs)");
+	hip_check(hipModuleGetFunction(&fn_table_compact, mod_util, "mbg_table_compact"),
+			  "hipModuleGetFunction(mbg_table_compact)");
+	hip_check(hipModuleGetFunction(&fn_fp_compact, mod_util, "mbg_fp_compact"), "hipModuleGetFunction(mbg_fp_compact)");
 }
 
 hipFunction_t program::wave_function()

@@ -72,6 +72,9 @@ int bits_for(uint64_t v)
 constexpr size_t k_lds_budget = 48 * 1024;
 constexpr uint64_t k_dense_limit = 8ull << 30;
 
+// LDS budget left after `used` bytes (0 when the budget is spent)
+size_t lds_left(size_t used) { return used < k_lds_budget ? k_lds_budget - used : 0; }
+
 struct plan
 {
 	int win_mode, fin_mode, ent_lds;
@@ -109,8 +112,9 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 	}
 
 	p.win_mode = sp.window_mode;
+	// the LDS histogram leaves room for the per-wave tables and caches below
 	if (p.win_mode < 0)
-		p.win_mode = win_dense_bytes <= 32 * 1024 ? MBG_HIST_LDS
+		p.win_mode = win_dense_bytes <= 32 * 1024 && win_dense_bytes + 2048 <= lds_left(lds) ? MBG_HIST_LDS
 					 : win_dense_bytes <= k_dense_limit ? MBG_HIST_DENSE
 														: MBG_HIST_HASH;
 	if (p.win_mode < 0 || p.win_mode > MBG_HIST_SORT)
@@ -119,7 +123,7 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 		throw error(MBG_E_LIMIT, "dense window histogram too large; use the hashed or sorted mode");
 	if (p.win_mode == MBG_HIST_LDS)
 	{
-		if (win_dense_bytes > k_lds_budget - lds)
+		if (win_dense_bytes > lds_left(lds))
 			throw error(MBG_E_LIMIT, "window histogram does not fit in LDS");
 		p.win_off = (int)(lds / 8);
 		lds += win_dense_bytes;
@@ -129,14 +133,14 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 	if (p.fin_mode > MBG_HIST_LDS)
 		throw error(MBG_E_ARG, "unknown final-state mode");
 	if (p.fin_mode < 0)
-		p.fin_mode = fin_dense_bytes <= k_lds_budget - lds ? MBG_HIST_LDS
+		p.fin_mode = fin_dense_bytes <= lds_left(lds) ? MBG_HIST_LDS
 					 : fin_dense_bytes <= (1ull << 30)	 ? MBG_HIST_DENSE
 														 : MBG_HIST_HASH;
 	if (p.fin_mode != MBG_HIST_HASH && fin_dense_bytes > k_dense_limit)
 		throw error(MBG_E_LIMIT, "dense final-state histogram too large; use the hashed mode");
 	if (p.fin_mode == MBG_HIST_LDS)
 	{
-		if (fin_dense_bytes > k_lds_budget - lds)
+		if (fin_dense_bytes > lds_left(lds))
 			throw error(MBG_E_LIMIT, "final-state histogram does not fit in LDS");
 		p.fin_off = (int)(lds / 8);
 		lds += fin_dense_bytes;
@@ -166,14 +170,14 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 	{
 		// the wave kernel ends one trajectory per wave every few thousand
 		// steps and needs its LDS for node tables: a smaller cache
-		p.tail_slots = cache_slots(k_lds_budget - lds, 16, wave_kernel ? 128 : 512, 64);
+		p.tail_slots = cache_slots(lds_left(lds), 16, wave_kernel ? 128 : 512, 64);
 		p.tail_off = (int)(lds / 8);
 		lds += (size_t)p.tail_slots * 16;
 	}
 	{
 		// the wave kernel keeps a smaller cache: its LDS holds the node tables
 		const size_t slot = 4 * (2 + (size_t)m.n_words());
-		p.fp_slots = wave_kernel ? cache_slots(k_lds_budget - lds, slot, 16, 8) : cache_slots(k_lds_budget - lds, slot, 128, 16);
+		p.fp_slots = wave_kernel ? cache_slots(lds_left(lds), slot, 16, 8) : cache_slots(lds_left(lds), slot, 128, 16);
 		p.fp_off = (int)(lds / 8);
 		lds += ((size_t)p.fp_slots * slot + 7) & ~(size_t)7;
 	}
@@ -181,10 +185,13 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 	// that end in the same state add to one LDS slot
 	if (p.fin_mode == MBG_HIST_HASH)
 	{
-		p.fin_cache_slots = cache_slots(k_lds_budget - lds, 16, wave_kernel ? 128 : 512, 64);
+		p.fin_cache_slots = cache_slots(lds_left(lds), 16, wave_kernel ? 128 : 512, 64);
 		p.fin_cache_off = (int)(lds / 8);
 		lds += (size_t)p.fin_cache_slots * 16;
 	}
+	if (lds > k_lds_budget)
+		throw error(MBG_E_LIMIT, "the run's LDS statistics (" + std::to_string(lds) + " bytes) exceed the "
+									 + std::to_string(k_lds_budget) + "-byte budget");
 	p.lds_bytes = (int)lds;
 
 	if (p.win_mode == MBG_HIST_HASH || p.win_mode == MBG_HIST_SORT)
@@ -213,6 +220,55 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 
 } // namespace
 
+namespace
+{
+
+void launch_util(hipFunction_t fn, unsigned grid, void** args, hipStream_t stream, const char* what)
+{
+	hip_check(hipModuleLaunchKernel(fn, grid, 1, 1, 256, 1, 1, 0, stream, args, nullptr), what);
+}
+
+// Occupied slots {key, value} of a hash table as (key, value) pairs; their
+// number is added to *count.
+void table_compact(program& p, const unsigned long long* table, uint64_t cap, unsigned long long* keys,
+				   unsigned long long* vals, uint64_t out_cap, unsigned long long* count, hipStream_t stream)
+{
+	void* args[] = { &table, &cap, &keys, &vals, &out_cap, &count };
+	const unsigned g = (unsigned)std::max<uint64_t>(1, (cap + 256 * 16 - 1) / (256 * 16)); // 16 slots per thread
+	launch_util(p.fn_table_compact, g, args, stream, "hipModuleLaunchKernel(mbg_table_compact)");
+}
+
+// Published fixed-point slots as records {count, state words}.
+void fp_compact(program& p, const unsigned int* table, uint64_t cap, int stride, unsigned int* out, uint64_t out_cap,
+				unsigned long long* count, hipStream_t stream)
+{
+	void* args[] = { &table, &cap, &stride, &out, &out_cap, &count };
+	const unsigned g = (unsigned)std::min<uint64_t>(4096, std::max<uint64_t>(1, (cap + 255) / 256));
+	launch_util(p.fn_fp_compact, g, args, stream, "hipModuleLaunchKernel(mbg_fp_compact)");
+}
+
+struct run_events
+{
+	hipEvent_t e0 = nullptr, es = nullptr, e1 = nullptr, copied = nullptr;
+	run_events()
+	{
+		hip_check(hipEventCreate(&e0), "hipEventCreate");
+		hip_check(hipEventCreate(&es), "hipEventCreate");
+		hip_check(hipEventCreate(&e1), "hipEventCreate");
+		hip_check(hipEventCreateWithFlags(&copied, hipEventDisableTiming), "hipEventCreateWithFlags");
+	}
+	~run_events()
+	{
+		for (hipEvent_t e : { e0, es, e1, copied })
+			if (e)
+				(void)hipEventDestroy(e);
+	}
+};
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+} // namespace
+
 void default_storage_modes(const model& m, int& win_mode, int& fin_mode)
 {
 	const plan pl = make_plan(m, sim_params {}, (uint64_t)m.sample_count(), m.windows(), m.n_observed(), false);
@@ -220,7 +276,7 @@ void default_storage_modes(const model& m, int& win_mode, int& fin_mode)
 	fin_mode = pl.fin_mode;
 }
 
-std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t stream)
+std::unique_ptr<result> simulate_async(program& p, const sim_params& sp, hipStream_t stream)
 {
 	scope_timer tm("runner> simulate");
 	const double t0 = wall_seconds();
@@ -245,6 +301,8 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 	for (auto& n : m.nodes)
 		res->names.push_back(n.name);
 	res->observed = m.observed_nodes();
+	res->ent.assign(windows, 0);
+	res->woff.assign((size_t)windows + 1, 0);
 
 	const bool use_wave = sp.kernel == 1 || (sp.kernel < 0 && W > max_lane_words);
 	if (!use_wave && W > max_lane_words)
@@ -255,6 +313,28 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		throw error(MBG_E_LIMIT, "run-time rates are served by the lane kernel (up to 256 nodes)");
 	if (!sp.rates.empty() && sp.rates.size() != p.gen.param_names.size())
 		throw error(MBG_E_ARG, "expected " + std::to_string(p.gen.param_names.size()) + " rates");
+	res->kernel = use_wave ? 1 : 0;
+	plan pl = make_plan(m, sp, n_traj, windows, n_obs, use_wave);
+	hipFunction_t fn = use_wave ? p.wave_function() : p.function(sp.rates.empty() ? 0 : 2, pl.win_mode, pl.fin_mode);
+	p.load_util();
+	if (!p.copy_stream)
+		hip_check(hipStreamCreateWithFlags(&p.copy_stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+	if (!p.mailbox)
+		p.mailbox = p.pool->get(4096);
+	auto* mb = (unsigned long long*)p.mailbox.get(); // [0, 16) counters, [16, 20) compaction counts
+
+	// The run's output slot: its previous copy to the host must be complete
+	// before this run writes into it or grows its buffers (a host-side wait:
+	// a buffer may be reallocated below).
+	out_slot& slot = p.slots[p.slot_next];
+	p.slot_next ^= 1;
+	if (slot.pending)
+	{
+		hip_check(hipEventSynchronize(slot.copied), "hipEventSynchronize");
+		slot.pending = false;
+	}
+	p.order_after_last_run(stream);
+
 	// run-time rates: a parameter view {rate, rate*log2(rate), enable masks} in device memory
 	const mbg_param_view* d_view = nullptr;
 	if (!sp.rates.empty())
@@ -278,11 +358,10 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		}
 		std::memset(img.data() + off_en, 0xff, 8 * (size_t)W);
 		hip_check(hipMemcpyAsync(base, img.data(), img.size(), hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
+		// the pageable source must stay valid until the copy has read it
+		hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 		d_view = (const mbg_param_view*)base;
 	}
-	res->kernel = use_wave ? 1 : 0;
-	plan pl = make_plan(m, sp, n_traj, windows, n_obs, use_wave);
-	hipFunction_t fn = use_wave ? p.wave_function() : p.function(sp.rates.empty() ? 0 : 2, pl.win_mode, pl.fin_mode);
 	// tail records for window storage in HBM (continuous time)
 	const bool tails = !m.discrete_time() && pl.win_mode != MBG_HIST_LDS;
 	uint64_t tail_cap = tails ? next_pow2(std::min<uint64_t>(std::max<uint64_t>(n_traj / 4, 1 << 12), 1 << 20)) : 0;
@@ -294,21 +373,25 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 											  : std::max<uint64_t>((uint64_t)windows, 16);
 		pair_cap = sp.window_capacity ? sp.window_capacity : std::max<uint64_t>(n_traj * per, 1 << 16);
 	}
-	p.load_util();
+	std::vector<float> istate;
+	for (auto& n : m.nodes)
+		istate.push_back(n.istate);
+	auto ev = std::make_shared<run_events>();
+	const size_t fp_stride = 2 + (size_t)W;
 
 	for (int attempt = 0;; attempt++)
 	{
 		// ---- storage
 		auto* counters = (unsigned long long*)p.b_counters.ensure(MBG_NCOUNTERS * 8);
 		hip_check(hipMemsetAsync(counters, 0, MBG_NCOUNTERS * 8, stream), "hipMemsetAsync");
-		auto* ent = (unsigned long long*)p.b_ent.ensure((size_t)windows * 8);
+		auto* ent = (unsigned long long*)slot.ent.ensure((size_t)windows * 8);
 		hip_check(hipMemsetAsync(ent, 0, (size_t)windows * 8, stream), "hipMemsetAsync");
 		unsigned long long *win_vals = nullptr, *win_keys = nullptr, *fin_vals = nullptr, *fin_keys = nullptr;
 		unsigned long long *pk = nullptr, *pv = nullptr;
 		if (pl.win_mode == MBG_HIST_SORT)
 		{
-			pk = (unsigned long long*)p.b_pk.ensure(pair_cap * 8);
-			pv = (unsigned long long*)p.b_pv2.ensure(pair_cap * 8);
+			pk = (unsigned long long*)slot.pk.ensure(pair_cap * 8);
+			pv = (unsigned long long*)slot.pv.ensure(pair_cap * 8);
 		}
 		else if (pl.win_mode == MBG_HIST_HASH)
 		{
@@ -336,13 +419,9 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 			tail_tab = (unsigned long long*)p.b_tail.ensure(tail_cap * 16);
 			hip_fill_slots(p, tail_tab, tail_cap, stream);
 		}
-		const size_t fp_stride = 2 + (size_t)W;
 		auto* fp = (unsigned int*)p.b_fp.ensure(pl.fp_cap * fp_stride * 4);
 		hip_check(hipMemsetAsync(fp, 0, pl.fp_cap * fp_stride * 4, stream), "hipMemsetAsync");
 
-		std::vector<float> istate;
-		for (auto& n : m.nodes)
-			istate.push_back(n.istate);
 		auto* d_istate = (float*)p.b_istate.ensure(istate.size() * 4);
 		hip_check(hipMemcpyAsync(d_istate, istate.data(), istate.size() * 4, hipMemcpyHostToDevice, stream),
 				  "hipMemcpyAsync");
@@ -453,6 +532,8 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 				throw error(MBG_E_LIMIT, "wave kernel tables do not fit in LDS; use a smaller block size");
 			work_per_block = (uint64_t)(block / 64);
 		}
+		else if (block > 256)
+			throw error(MBG_E_ARG, "the lane kernel takes blocks of at most 256 lanes");
 		int per_cu = sp.blocks_per_cu;
 		if (per_cu <= 0)
 		{
@@ -463,60 +544,66 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 		uint64_t blocks_needed = (n_traj + work_per_block - 1) / work_per_block;
 		uint64_t grid = std::min<uint64_t>((uint64_t)p.num_cus * per_cu, std::max<uint64_t>(blocks_needed, 1));
 
-		hipEvent_t e0, e1, es;
-		hip_check(hipEventCreate(&e0), "hipEventCreate");
-		hip_check(hipEventCreate(&e1), "hipEventCreate");
-		hip_check(hipEventCreate(&es), "hipEventCreate");
-		hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+		hip_check(hipEventRecord(ev->e0, stream), "hipEventRecord");
 		if (n_traj > 0)
 		{
 			void* args[] = { &a };
 			hip_check(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, block, 1, 1, lds_total, stream, args, nullptr),
 					  "hipModuleLaunchKernel(simulate)");
-			hip_check(hipEventRecord(es, stream), "hipEventRecord");
-			if (tails)
-			{
-				const unsigned eg = (unsigned)std::min<uint64_t>(1024, (tail_cap + 255) / 256);
-				hip_check(hipModuleLaunchKernel(p.fn_expand_tails, eg, 1, 1, 256, 1, 1, 0, stream, args, nullptr),
-						  "hipModuleLaunchKernel(mbg_expand_tails)");
-			}
 		}
-		std::vector<unsigned long long> ctr(MBG_NCOUNTERS);
-		unsigned long long *uk = nullptr, *uv = nullptr, *ucount = nullptr;
-		bool pair_over = false;
-		if (pl.win_mode == MBG_HIST_SORT)
+		hip_check(hipEventRecord(ev->es, stream), "hipEventRecord");
+		if (n_traj > 0 && tails)
 		{
-			// the pair count sizes the sort
-			hip_check(hipMemcpyAsync(ctr.data(), counters, MBG_NCOUNTERS * 8, hipMemcpyDeviceToHost, stream),
-					  "hipMemcpyAsync");
-			hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-			const uint64_t n_pairs = ctr[MBG_CTR_PAIRS];
-			pair_over = n_pairs > pair_cap;
-			if (!pair_over && n_pairs)
-			{
-				auto* ka = (unsigned long long*)p.b_pk_alt.ensure(pair_cap * 8);
-				auto* va = (unsigned long long*)p.b_pv_alt.ensure(pair_cap * 8);
-				ucount = (unsigned long long*)p.b_ucount.ensure(8);
-				sort_reduce_pairs(p.b_sort_tmp, pk, pv, ka, va, n_pairs,
-								  n_obs + bits_for((uint64_t)windows - 1) + 1, // + the padding bit
-								  ucount, stream, &uk, &uv);
-			}
-			if (n_traj)
-				p.pairs_per_traj = (n_pairs + n_traj - 1) / n_traj;
+			void* args[] = { &a };
+			const unsigned eg = (unsigned)std::min<uint64_t>(1024, (tail_cap + 255) / 256);
+			hip_check(hipModuleLaunchKernel(p.fn_expand_tails, eg, 1, 1, 256, 1, 1, 0, stream, args, nullptr),
+					  "hipModuleLaunchKernel(mbg_expand_tails)");
 		}
-		hip_check(hipEventRecord(e1, stream), "hipEventRecord");
 
-		// ---- gather
-		hip_check(hipMemcpyAsync(ctr.data(), counters, MBG_NCOUNTERS * 8, hipMemcpyDeviceToHost, stream),
-				  "hipMemcpyAsync");
+		// ---- the occupied cells of every table, compacted into the output slot
+		auto* cnt = (unsigned long long*)slot.cnt.ensure(4 * 8);
+		if (!sp.device_only)
+		{
+			hip_check(hipMemsetAsync(cnt, 0, 4 * 8, stream), "hipMemsetAsync");
+			if (pl.win_mode == MBG_HIST_HASH)
+			{
+				auto* tk = (unsigned long long*)slot.pk.ensure(pl.win_cap * 8);
+				auto* tv = (unsigned long long*)slot.pv.ensure(pl.win_cap * 8);
+				table_compact(p, win_keys, pl.win_cap, tk, tv, pl.win_cap, cnt + 0, stream);
+			}
+			else if (pl.win_mode != MBG_HIST_SORT)
+			{
+				auto* tk = (unsigned long long*)slot.pk.ensure(pl.win_cells * 8);
+				auto* tv = (unsigned long long*)slot.pv.ensure(pl.win_cells * 8);
+				compact_dense(p.b_sort_tmp, win_vals, pl.win_cells, tk, tv, cnt + 0, stream);
+			}
+			if (pl.fin_mode == MBG_HIST_HASH)
+			{
+				auto* k = (unsigned long long*)slot.fk.ensure(pl.fin_cap * 8);
+				auto* v = (unsigned long long*)slot.fv.ensure(pl.fin_cap * 8);
+				table_compact(p, fin_keys, pl.fin_cap, k, v, pl.fin_cap, cnt + 1, stream);
+			}
+			else
+			{
+				auto* k = (unsigned long long*)slot.fk.ensure(pl.fin_cells * 8);
+				auto* v = (unsigned long long*)slot.fv.ensure(pl.fin_cells * 8);
+				compact_dense(p.b_sort_tmp, fin_vals, pl.fin_cells, k, v, cnt + 1, stream);
+			}
+			auto* fpc = (unsigned int*)slot.fpc.ensure(pl.fp_cap * (fp_stride - 1) * 4);
+			fp_compact(p, fp, pl.fp_cap, (int)fp_stride, fpc, pl.fp_cap, cnt + 2, stream);
+		}
+		else
+			hip_check(hipEventRecord(ev->e1, stream), "hipEventRecord");
+		hip_check(hipMemcpyAsync(mb, counters, MBG_NCOUNTERS * 8, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+		if (!sp.device_only)
+			hip_check(hipMemcpyAsync(mb + 16, cnt, 4 * 8, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
 		hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-		float ms = 0, sim_ms = 0;
-		hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-		if (n_traj > 0)
-			hip_check(hipEventElapsedTime(&sim_ms, e0, es), "hipEventElapsedTime");
-		(void)hipEventDestroy(e0);
-		(void)hipEventDestroy(e1);
-		(void)hipEventDestroy(es);
+		unsigned long long ctr[MBG_NCOUNTERS];
+		std::memcpy(ctr, mb, sizeof(ctr));
+		const uint64_t n_pairs = ctr[MBG_CTR_PAIRS];
+		const bool pair_over = pl.win_mode == MBG_HIST_SORT && n_pairs > pair_cap;
+		if (pl.win_mode == MBG_HIST_SORT && n_traj)
+			p.pairs_per_traj = (n_pairs + n_traj - 1) / n_traj;
 
 		const bool win_over = ctr[MBG_CTR_WIN_OVERFLOW] != 0, fin_over = ctr[MBG_CTR_FIN_OVERFLOW] != 0,
 				   fp_over = ctr[MBG_CTR_FP_OVERFLOW] != 0, tail_over = ctr[MBG_CTR_TAIL_OVERFLOW] != 0;
@@ -535,7 +622,7 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 			{
 				if (sp.window_capacity)
 					throw error(MBG_E_OVERFLOW, "window pair buffer overflow; raise window_capacity");
-				pair_cap = ctr[MBG_CTR_PAIRS] + ctr[MBG_CTR_PAIRS] / 16 + 1;
+				pair_cap = n_pairs + n_pairs / 16 + 1;
 			}
 			if (win_over)
 			{
@@ -558,9 +645,7 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 			continue;
 		}
 
-		res->kernel_ms = ms;
-		res->simulate_ms = sim_ms;
-		res->window_pairs = pl.win_mode == MBG_HIST_SORT ? ctr[MBG_CTR_PAIRS] : 0;
+		res->window_pairs = pl.win_mode == MBG_HIST_SORT ? n_pairs : 0;
 		res->transitions = ctr[MBG_CTR_TRANSITIONS];
 		res->finished = ctr[MBG_CTR_FINISHED];
 		res->truncated = ctr[MBG_CTR_TRUNCATED];
@@ -570,121 +655,114 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 
 		if (sp.device_only)
 		{
-			res->ent.assign(windows, 0);
-			res->win.assign(windows, {});
-			break;
+			p.mark_run_done(stream);
+			float ms = 0, sim_ms = 0;
+			hip_check(hipEventElapsedTime(&ms, ev->e0, ev->e1), "hipEventElapsedTime");
+			hip_check(hipEventElapsedTime(&sim_ms, ev->e0, ev->es), "hipEventElapsedTime");
+			res->kernel_ms = ms;
+			res->simulate_ms = sim_ms;
+			res->total_ms = (wall_seconds() - t0) * 1e3;
+			return res;
 		}
 
-		res->ent.resize(windows);
-		hip_check(hipMemcpy(res->ent.data(), ent, (size_t)windows * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+		// ---- order the cells on the device
+		const int key_bits = n_obs + bits_for((uint64_t)windows - 1);
+		uint64_t n_cells = mb[16], n_fin = mb[17], n_fp = mb[18];
+		unsigned long long *wk = (unsigned long long*)slot.pk.ptr, *wv = (unsigned long long*)slot.pv.ptr;
+		if (pl.win_mode == MBG_HIST_HASH && n_cells)
+		{
+			auto* ka = (unsigned long long*)slot.pka.ensure(n_cells * 8);
+			auto* va = (unsigned long long*)slot.pva.ensure(n_cells * 8);
+			sort_pairs(p.b_sort_tmp, wk, wv, ka, va, n_cells, key_bits, stream, &wk, &wv);
+		}
+		else if (pl.win_mode == MBG_HIST_SORT)
+		{
+			n_cells = 0;
+			if (n_pairs)
+			{
+				auto* ka = (unsigned long long*)slot.pka.ensure(pair_cap * 8);
+				auto* va = (unsigned long long*)slot.pva.ensure(pair_cap * 8);
+				sort_reduce_pairs(p.b_sort_tmp, pk, pv, ka, va, n_pairs, key_bits + 1, // + the padding bit
+								  cnt + 3, stream, &wk, &wv);
+				hip_check(hipMemcpyAsync(mb + 19, cnt + 3, 8, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+				hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+				n_cells = mb[19];
+			}
+		}
+		unsigned long long *fk = (unsigned long long*)slot.fk.ptr, *fv = (unsigned long long*)slot.fv.ptr;
+		if (pl.fin_mode == MBG_HIST_HASH && n_fin)
+		{
+			auto* ka = (unsigned long long*)slot.fka.ensure(n_fin * 8);
+			auto* va = (unsigned long long*)slot.fva.ensure(n_fin * 8);
+			sort_pairs(p.b_sort_tmp, fk, fv, ka, va, n_fin, std::max(n_obs, 1), stream, &fk, &fv);
+		}
+		hip_check(hipEventRecord(ev->e1, stream), "hipEventRecord");
+		p.mark_run_done(stream);
 
-		res->win.assign(windows, {});
-		if (pl.win_mode == MBG_HIST_SORT)
-		{
-			unsigned long long nu = 0;
-			if (ucount)
-				hip_check(hipMemcpy(&nu, ucount, 8, hipMemcpyDeviceToHost), "hipMemcpy");
-			std::vector<unsigned long long> keys(nu), vals(nu);
-			if (nu)
-			{
-				hip_check(hipMemcpy(keys.data(), uk, nu * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-				hip_check(hipMemcpy(vals.data(), uv, nu * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-			}
-			// keys arrive sorted by (window, observed state)
-			const uint64_t omask = n_obs >= 64 ? ~0ull : ((1ull << n_obs) - 1);
-			for (uint64_t i = 0; i < nu; i++)
-				if (keys[i] != MBG_PAIR_PAD_KEY)
-					res->win[keys[i] >> n_obs].emplace_back(keys[i] & omask, vals[i]);
-		}
-		else if (pl.win_mode == MBG_HIST_HASH)
-		{
-			// the occupied cells, compacted and sorted by key on the device
-			auto* tk = (unsigned long long*)p.b_pk.ensure(pl.win_cap * 8);
-			auto* tv = (unsigned long long*)p.b_pv2.ensure(pl.win_cap * 8);
-			auto* ka = (unsigned long long*)p.b_pk_alt.ensure(pl.win_cap * 8);
-			auto* va = (unsigned long long*)p.b_pv_alt.ensure(pl.win_cap * 8);
-			a.pair_keys = tk;
-			a.pair_vals = tv;
-			a.pair_cap = pl.win_cap;
-			hip_check(hipMemsetAsync(counters + MBG_CTR_PAIRS, 0, 8, stream), "hipMemsetAsync");
-			{
-				void* args[] = { &a };
-				const unsigned g = (unsigned)((pl.win_cap + 256 * 16 - 1) / (256 * 16)); // 16 slots per thread
-				hip_check(hipModuleLaunchKernel(p.fn_table_pairs, g, 1, 1, 256, 1, 1, 0, stream, args, nullptr),
-						  "hipModuleLaunchKernel(mbg_table_pairs)");
-			}
-			unsigned long long n_cells = 0;
-			hip_check(hipMemcpyAsync(&n_cells, counters + MBG_CTR_PAIRS, 8, hipMemcpyDeviceToHost, stream),
-					  "hipMemcpyAsync");
-			hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-			unsigned long long nu = 0, *hk = nullptr, *hv = nullptr;
-			if (n_cells)
-			{
-				auto* hcount = (unsigned long long*)p.b_ucount.ensure(8);
-				sort_reduce_pairs(p.b_sort_tmp, tk, tv, ka, va, n_cells, n_obs + bits_for((uint64_t)windows - 1),
-								  hcount, stream, &hk, &hv);
-				hip_check(hipMemcpy(&nu, hcount, 8, hipMemcpyDeviceToHost), "hipMemcpy");
-			}
-			std::vector<unsigned long long> keys(nu), vals(nu);
-			if (nu)
-			{
-				hip_check(hipMemcpy(keys.data(), hk, nu * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-				hip_check(hipMemcpy(vals.data(), hv, nu * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-			}
-			const uint64_t omask = n_obs >= 64 ? ~0ull : ((1ull << n_obs) - 1);
-			for (uint64_t i = 0; i < nu; i++)
-				res->win[keys[i] >> n_obs].emplace_back(keys[i] & omask, vals[i]);
-		}
-		else
-		{
-			std::vector<unsigned long long> vals(pl.win_cells);
-			hip_check(hipMemcpy(vals.data(), win_vals, pl.win_cells * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-			const uint64_t per = 1ull << n_obs;
-			for (int w = 0; w < windows; w++)
-				for (uint64_t s = 0; s < per; s++)
-					if (uint64_t v = vals[(uint64_t)w * per + s])
-						res->win[w].emplace_back(s, v);
-		}
+		// ---- copy out on the copy stream, into one pinned block
+		const size_t o_wv = align256(n_cells * 8), o_ent = o_wv + align256(n_cells * 8);
+		const size_t o_fk = o_ent + align256((size_t)windows * 8), o_fv = o_fk + align256(n_fin * 8);
+		const size_t o_fp = o_fv + align256(n_fin * 8), total = o_fp + align256(n_fp * (fp_stride - 1) * 4);
+		std::shared_ptr<char> blk = p.pool->get(total);
+		char* h = blk.get();
+		hipStream_t cs = p.copy_stream;
+		hip_check(hipStreamWaitEvent(cs, ev->e1, 0), "hipStreamWaitEvent");
+		auto d2h = [&](size_t off, const void* src, size_t bytes) {
+			if (bytes)
+				hip_check(hipMemcpyAsync(h + off, src, bytes, hipMemcpyDeviceToHost, cs), "hipMemcpyAsync");
+		};
+		d2h(0, wk, n_cells * 8);
+		d2h(o_wv, wv, n_cells * 8);
+		d2h(o_ent, ent, (size_t)windows * 8);
+		d2h(o_fk, fk, n_fin * 8);
+		d2h(o_fv, fv, n_fin * 8);
+		d2h(o_fp, slot.fpc.ptr, n_fp * (fp_stride - 1) * 4);
+		// the slot keeps an event of its own (the result's goes with the result)
+		if (!slot.copied)
+			hip_check(hipEventCreateWithFlags(&slot.copied, hipEventDisableTiming), "hipEventCreateWithFlags");
+		hip_check(hipEventRecord(slot.copied, cs), "hipEventRecord");
+		hip_check(hipEventRecord(ev->copied, cs), "hipEventRecord");
+		slot.pending = true;
 
-		if (pl.fin_mode == MBG_HIST_HASH)
-		{
-			std::vector<unsigned long long> tab(pl.fin_cap * 2);
-			hip_check(hipMemcpy(tab.data(), fin_keys, pl.fin_cap * 16, hipMemcpyDeviceToHost), "hipMemcpy");
-			for (uint64_t i = 0; i < pl.fin_cap; i++)
-				if (tab[2 * i] != MBG_EMPTY_KEY && tab[2 * i + 1])
-					res->fin.emplace_back(tab[2 * i], tab[2 * i + 1]);
-			std::sort(res->fin.begin(), res->fin.end());
-		}
-		else
-		{
-			std::vector<unsigned long long> vals(pl.fin_cells);
-			hip_check(hipMemcpy(vals.data(), fin_vals, pl.fin_cells * 8, hipMemcpyDeviceToHost), "hipMemcpy");
-			for (uint64_t s = 0; s < pl.fin_cells; s++)
-				if (vals[s])
-					res->fin.emplace_back(s, vals[s]);
-		}
-
-		{
-			std::vector<unsigned int> tab(pl.fp_cap * fp_stride);
-			hip_check(hipMemcpy(tab.data(), fp, tab.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy");
-			for (uint64_t i = 0; i < pl.fp_cap; i++)
+		result* r = res.get();
+		r->on_ready = [r, ev, blk, h, n_cells, n_fin, n_fp, o_wv, o_ent, o_fk, o_fv, o_fp, fp_stride, t0, W]() {
+			hip_check(hipEventSynchronize(ev->copied), "hipEventSynchronize");
+			float ms = 0, sim_ms = 0;
+			hip_check(hipEventElapsedTime(&ms, ev->e0, ev->e1), "hipEventElapsedTime");
+			hip_check(hipEventElapsedTime(&sim_ms, ev->e0, ev->es), "hipEventElapsedTime");
+			r->kernel_ms = ms;
+			r->simulate_ms = sim_ms;
+			r->win.hold = blk;
+			r->win.key = (const uint64_t*)h;
+			r->win.val = (const uint64_t*)(h + o_wv);
+			r->win.n = n_cells;
+			std::memcpy(r->ent.data(), h + o_ent, r->ent.size() * 8);
+			r->fin.hold = blk;
+			r->fin.key = (const uint64_t*)(h + o_fk);
+			r->fin.val = (const uint64_t*)(h + o_fv);
+			r->fin.n = n_fin;
+			const uint32_t* rec = (const uint32_t*)(h + o_fp);
+			r->fp_words.resize(n_fp * (size_t)W);
+			r->fp_count.resize(n_fp);
+			for (uint64_t i = 0; i < n_fp; i++)
 			{
-				const unsigned int* e = &tab[i * fp_stride];
-				if (e[0] == 2u && e[1])
-					res->fixed.emplace_back(std::vector<uint32_t>(e + 2, e + 2 + W), e[1]);
+				const uint32_t* e = rec + i * (fp_stride - 1);
+				r->fp_count[i] = e[0];
+				std::memcpy(r->fp_words.data() + i * (size_t)W, e + 1, 4 * (size_t)W);
 			}
-			// reference order: most significant word first (src/state.h:43-49)
-			std::sort(res->fixed.begin(), res->fixed.end(), [](const auto& x, const auto& y) {
-				for (int w = (int)x.first.size() - 1; w >= 0; w--)
-					if (x.first[w] != y.first[w])
-						return x.first[w] < y.first[w];
-				return false;
-			});
-		}
-		break;
+			r->index_cells();
+			r->sort_fixed();
+			r->total_ms = (wall_seconds() - t0) * 1e3;
+		};
+		return res;
 	}
-	res->total_ms = (wall_seconds() - t0) * 1e3;
-	return res;
+}
+
+std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t stream)
+{
+	auto r = simulate_async(p, sp, stream);
+	r->wait();
+	return r;
 }
 
 } // namespace mbg

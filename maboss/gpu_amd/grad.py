@@ -18,7 +18,10 @@ import numpy as np
 
 from ._capi import GradInfo, GradParams, check, lib
 
-__all__ = ["window_probabilities", "WindowProbabilities", "observed_marginals", "backward"]
+__all__ = ["window_probabilities", "WindowProbabilities", "observed_marginals", "backward", "ACCUMULATOR"]
+
+# Precision of the score accumulation: per trajectory, then across trajectories.
+ACCUMULATOR = "f32 per trajectory, f64 across trajectories"
 
 
 def _torch():
