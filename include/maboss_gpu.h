@@ -237,6 +237,21 @@ int mbg_grad_check(mbg_program* p);
 int mbg_backward(mbg_program* p, const mbg_grad_params* params, const float* rates, const double* grad_probs,
 				 double* grad_params, void* stream, mbg_grad_info* info);
 
+/* mbg_backward that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and result as mbg_backward without `info`; the kernel's
+ * workspace (counters and model tables) comes from a static device array of
+ * MBG_NOALLOC_SCRATCH_BYTES in this library, of which each GPU holds one
+ * copy for the life of the process.  Call mbg_grad_check on the program
+ * first (it loads the gradient kernel); MBG_ERR_ARG otherwise, and
+ * MBG_ERR_LIMIT if the workspace would exceed the scratch.  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (on
+ * different streams or threads) race. */
+#define MBG_NOALLOC_SCRATCH_BYTES 16384
+int mbg_backward_noalloc(mbg_program* p, const mbg_grad_params* params, const float* rates,
+						 const double* grad_probs, double* grad_params, void* stream);
+
 /* ---- results (replaces stats::finalize / visualize / write_csv,
  *      src/statistics/stats.h:15-29) ---- */
 void mbg_result_free(mbg_result* r);

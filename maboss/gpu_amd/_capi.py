@@ -32,7 +32,7 @@ EXPORTED = [
     "mbg_program_compile_seconds",
     "mbg_sim_params_default", "mbg_simulate", "mbg_simulate_async",
     "mbg_batch_params_default", "mbg_simulate_batched",
-    "mbg_grad_params_default", "mbg_grad_check", "mbg_backward",
+    "mbg_grad_params_default", "mbg_grad_check", "mbg_backward", "mbg_backward_noalloc",
     "mbg_result_free", "mbg_result_wait", "mbg_result_get_info", "mbg_result_window_size", "mbg_result_window",
     "mbg_result_window_probs", "mbg_result_entropies", "mbg_result_final_size",
     "mbg_result_final", "mbg_result_fixed_size", "mbg_result_fixed",
@@ -173,6 +173,7 @@ def _declare(lib):
         "mbg_grad_params_default": ([ctypes.POINTER(GradParams)], None),
         "mbg_grad_check": ([P], ctypes.c_int),
         "mbg_backward": ([P, ctypes.POINTER(GradParams), P, P, P, P, ctypes.POINTER(GradInfo)], ctypes.c_int),
+        "mbg_backward_noalloc": ([P, ctypes.POINTER(GradParams), P, P, P, P], ctypes.c_int),
         "mbg_simulate_batched": ([P, ctypes.POINTER(BatchParams), i32, ctypes.POINTER(Variant), P, pp,
                                   ctypes.POINTER(BatchInfo)], ctypes.c_int),
         "mbg_result_free": ([P], None),

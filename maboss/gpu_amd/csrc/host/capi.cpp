@@ -11,6 +11,8 @@
 #include "../../../../include/maboss_gpu.h"
 #include "engine.h"
 
+static_assert(mbg::k_noalloc_scratch_bytes == MBG_NOALLOC_SCRATCH_BYTES, "scratch size in the header");
+
 struct mbg_model
 {
 	std::shared_ptr<mbg::model> m;
@@ -393,6 +395,10 @@ int mbg_grad_check(mbg_program* p)
 		if (!p)
 			throw mbg::error(mbg::MBG_E_ARG, "null program");
 		(void)p->p->grad();
+		// load what a gradient launch needs, so later launches allocate nothing
+		(void)p->p->grad_function();
+		p->p->load_util();
+		(void)p->p->grad_tables();
 	});
 }
 
@@ -423,6 +429,30 @@ int mbg_backward(mbg_program* p, const mbg_grad_params* params, const float* rat
 			info->block_size = gt.block;
 			info->grid = gt.grid;
 		}
+	});
+}
+
+int mbg_backward_noalloc(mbg_program* p, const mbg_grad_params* params, const float* rates,
+						 const double* grad_probs, double* grad_params, void* stream)
+{
+	return guarded([&] {
+		if (!p || !grad_probs || !grad_params)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		mbg::grad_params gp;
+		if (params)
+		{
+			gp.n_traj = params->n_trajectories;
+			gp.first_id = params->first_id;
+			gp.seed = params->seed;
+			gp.variant = params->variant;
+			gp.max_steps = params->max_steps;
+			gp.block = params->block_size;
+			gp.per_cu = params->blocks_per_cu;
+		}
+		void* s = mbg::noalloc_scratch(p->p->device);
+		if (!s)
+			throw mbg::error(mbg::MBG_E_ARG, "the no-allocation scratch is not set up on the program's device");
+		mbg::simulate_grad(*p->p, gp, rates, grad_probs, grad_params, (hipStream_t)stream, nullptr, s);
 	});
 }
 

@@ -85,6 +85,15 @@ struct cell_view
 	static cell_view from(std::vector<uint64_t> keys, std::vector<uint64_t> vals);
 };
 
+// Tables of the gradient kernel (enable masks, initial-state probabilities,
+// jump parameters, the model's rates) in pinned host memory, uploaded at the
+// start of each gradient launch.
+struct grad_host_tables
+{
+	std::shared_ptr<char> host;
+	size_t en_bytes = 0, ist_bytes = 0, jp_bytes = 0, bytes = 0;
+};
+
 // Device storage a run's statistics leave the device from.  A program keeps
 // two, used in turn, so that one run's copy to the host (on the program's
 // copy stream) overlaps the next run's kernels; a slot is reused only after
@@ -167,7 +176,10 @@ struct program
 	hipFunction_t fn_grad = nullptr;
 	const grad_code& grad();          // throws MBG_E_LIMIT when the model has no gradient form
 	hipFunction_t grad_function();
-	device_buffer b_grates, b_gen, b_gjp;
+	device_buffer b_gen;                 // gradient workspace: counters, tables
+	grad_host_tables gtab;                    // the gradient kernel's tables, built on first use
+	const grad_host_tables& grad_tables();
+	size_t lds_per_block = 0;            // device property, read once
 	hipFunction_t fn_fill_u64 = nullptr; // utility module's fill (graph safe)
 
 	// wave-per-trajectory kernel (precompiled, table driven)
@@ -336,7 +348,18 @@ struct grad_timing
 // the trajectories of the forward run with the same ids, seed, variant and
 // rates (device [n_params], or null for the model's values).  Stream
 // ordered; with timing the stream is synchronised and timing filled.
+// scratch: when non-null, the device workspace (counters and tables, at most
+// k_noalloc_scratch_bytes) is taken from there and nothing is allocated; the
+// gradient kernel must already be loaded (mbg_grad_check).
 void simulate_grad(program& p, const grad_params& gp, const float* rates, const double* gprob, double* grad,
-				   hipStream_t stream, grad_timing* timing);
+				   hipStream_t stream, grad_timing* timing, void* scratch = nullptr);
+
+// Static device scratch of the no-allocation entry point (noalloc.hip): its
+// size, the per-device set-up (resolves the array's address on the current
+// device; called when a program is compiled) and the address on a device
+// (null before set-up there).
+constexpr size_t k_noalloc_scratch_bytes = 16 * 1024;
+void noalloc_setup(int device);
+void* noalloc_scratch(int device);
 
 } // namespace mbg

@@ -36,15 +36,46 @@ size_t grad_lds_bytes(int np, uint64_t cells, bool g_lds, int n_nodes, int block
 
 } // namespace
 
+const grad_host_tables& program::grad_tables()
+{
+	if (gtab.host)
+		return gtab;
+	const grad_code& gc = grad();
+	const model& m = *mdl;
+	const int N = m.n_nodes(), W = m.n_words(), np = gc.n_params;
+	// enable masks (all on), istate, jump parameters, then the model's rates,
+	// in pinned memory: the upload of each call is a plain DMA read of it
+	gtab.en_bytes = ((size_t)W * 4 + 15) & ~(size_t)15;
+	gtab.ist_bytes = ((size_t)N * 4 + 15) & ~(size_t)15;
+	gtab.jp_bytes = (gc.jparam.size() * 2 + 15) & ~(size_t)15;
+	gtab.bytes = gtab.en_bytes + gtab.ist_bytes + gtab.jp_bytes + (size_t)np * 4;
+	std::shared_ptr<char> h = pool->get(gtab.bytes);
+	char* t = h.get();
+	std::memset(t, 0, gtab.bytes);
+	std::memset(t, 0xff, (size_t)W * 4);
+	for (int i = 0; i < N; i++)
+		std::memcpy(t + gtab.en_bytes + 4 * (size_t)i, &m.nodes[i].istate, 4);
+	std::memcpy(t + gtab.en_bytes + gtab.ist_bytes, gc.jparam.data(), gc.jparam.size() * 2);
+	for (int k = 0; k < np; k++)
+	{
+		const float v = m.variables.at(gen.param_names[k]);
+		std::memcpy(t + gtab.en_bytes + gtab.ist_bytes + gtab.jp_bytes + 4 * (size_t)k, &v, 4);
+	}
+	gtab.host = h;
+	return gtab;
+}
+
 void simulate_grad(program& p, const grad_params& gp, const float* rates, const double* gprob, double* grad,
-				   hipStream_t stream, grad_timing* timing)
+				   hipStream_t stream, grad_timing* timing, void* scratch)
 {
 	scope_timer tm("grad> simulate");
 	const model& m = *p.mdl;
+	if (scratch && !p.fn_grad)
+		throw error(MBG_E_ARG, "the gradient kernel is not loaded: call mbg_grad_check first");
 	const grad_code& gc = p.grad();
 	if (m.n_words() > max_lane_words)
 		throw error(MBG_E_LIMIT, "the gradient form serves models of up to 256 nodes");
-	const int n_obs = m.n_observed(), windows = m.windows(), N = m.n_nodes(), W = m.n_words();
+	const int n_obs = m.n_observed(), windows = m.windows(), N = m.n_nodes();
 	if (n_obs > 24)
 		throw error(MBG_E_LIMIT, "the gradient form takes a dense dL/dP: at most 24 observed nodes");
 	if (!gprob || !grad)
@@ -55,34 +86,38 @@ void simulate_grad(program& p, const grad_params& gp, const float* rates, const 
 	const uint64_t cells = (uint64_t)windows << n_obs;
 	hipFunction_t fn = p.grad_function();
 
-	// rates: the caller's device table or the model's values
-	if (!rates)
+	const grad_host_tables& gt = p.grad_tables();
+	const size_t en_bytes = gt.en_bytes, ist_bytes = gt.ist_bytes, jp_bytes = gt.jp_bytes, tab_bytes = gt.bytes;
+	// device workspace: the counters, then the tables; from the library's
+	// static scratch array (no allocation) or from the program's buffers
+	const size_t ctr_bytes = 256;
+	char* ws = nullptr;
+	if (scratch)
 	{
-		std::vector<float> base(np);
-		for (int k = 0; k < np; k++)
-			base[k] = m.variables.at(p.gen.param_names[k]);
-		auto* d = (float*)p.b_grates.ensure((size_t)np * 4);
-		hip_check(hipMemcpyAsync(d, base.data(), (size_t)np * 4, hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
-		rates = d;
+		if (ctr_bytes + tab_bytes > k_noalloc_scratch_bytes)
+			throw error(MBG_E_LIMIT, "the gradient workspace (" + std::to_string(ctr_bytes + tab_bytes)
+										 + " bytes) exceeds the " + std::to_string(k_noalloc_scratch_bytes)
+										 + "-byte scratch of the no-allocation entry point");
+		ws = (char*)scratch;
 	}
-	// tables: enable masks (all on), istate, jump parameters
-	const size_t en_bytes = ((size_t)W * 4 + 15) & ~(size_t)15;
-	const size_t ist_bytes = ((size_t)N * 4 + 15) & ~(size_t)15;
-	std::vector<char> tab(en_bytes + ist_bytes + gc.jparam.size() * 2, 0);
-	std::memset(tab.data(), 0xff, (size_t)W * 4);
-	for (int i = 0; i < N; i++)
-		std::memcpy(tab.data() + en_bytes + 4 * (size_t)i, &m.nodes[i].istate, 4);
-	std::memcpy(tab.data() + en_bytes + ist_bytes, gc.jparam.data(), gc.jparam.size() * 2);
-	char* dtab = (char*)p.b_gjp.ensure(tab.size());
-	hip_check(hipMemcpyAsync(dtab, tab.data(), tab.size(), hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
-	auto* counters = (unsigned long long*)p.b_gen.ensure(MBG_NCOUNTERS * 8);
+	else
+		ws = (char*)p.b_gen.ensure(ctr_bytes + tab_bytes);
+	auto* counters = (unsigned long long*)ws;
+	char* dtab = ws + ctr_bytes;
+	hip_check(hipMemcpyAsync(dtab, gt.host.get(), tab_bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
+	if (!rates)
+		rates = (const float*)(dtab + en_bytes + ist_bytes + jp_bytes);
 	fill_u64(p, stream, counters, MBG_NCOUNTERS, 0);
 	fill_u64(p, stream, grad, (uint64_t)np, 0);
 
 	// launch shape: the G table goes to LDS when small; block size from the LDS budget
-	hipDeviceProp_t prop;
-	hip_check(hipGetDeviceProperties(&prop, p.device), "hipGetDeviceProperties");
-	const size_t lds_max = std::min<size_t>(prop.sharedMemPerBlock, 64 * 1024);
+	if (!p.lds_per_block)
+	{
+		hipDeviceProp_t prop;
+		hip_check(hipGetDeviceProperties(&prop, p.device), "hipGetDeviceProperties");
+		p.lds_per_block = prop.sharedMemPerBlock;
+	}
+	const size_t lds_max = std::min<size_t>(p.lds_per_block, 64 * 1024);
 	const bool g_lds = cells * 8 <= 32 * 1024;
 	int block = gp.block > 0 ? gp.block : 256;
 	if (block % 64 != 0 || block > 256)

@@ -367,6 +367,8 @@ std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const g
 	if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
 		throw error(MBG_E_HIP, std::string("this build targets gfx950; current device is ") + prop.gcnArchName);
 	p->num_cus = prop.multiProcessorCount;
+	// the no-allocation entry point's scratch on this device (once per device)
+	noalloc_setup(p->device);
 	if (p->W <= max_lane_words)
 	{
 		int wm = 0, fm = 0;
