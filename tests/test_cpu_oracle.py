@@ -4,7 +4,10 @@ Each check is fixed by mathematics or by the reference's own captured
 results, not by the oracle's code: closed-form Markov chains, exact counts in
 discrete time, conservation invariants, fixed points verified by the engine's
 separate expression interpreter, and 3-sigma binomial agreement with the
-reference's .res captures (test/*.res of MaBoSS.GPU)."""
+reference's .res captures (test/*.res of MaBoSS.GPU).  The closed forms and
+the .res comparisons run in both summation orders the oracle has: the lane
+kernel's sequential order and the wave kernel's lane-slice + scan order
+(--order wave), so the wave branch is pinned by the same mathematics."""
 import math
 import os
 
@@ -16,6 +19,7 @@ from conftest import RES, model_paths, read_raw, run_oracle
 from resfile import parse_report, zscores
 
 TWO32 = 4294967296.0
+ORDERS = pytest.mark.parametrize("order", ["seq", "wave"])
 
 
 def _write(tmp_path, name, bnd, cfg):
@@ -30,11 +34,12 @@ def _window_probs(raw, n, w, discrete=False):
     return {s: v / scale for (ww, s), v in raw["W"].items() if ww == w}
 
 
-def test_toggle_closed_form(tmp_path):
+@ORDERS
+def test_toggle_closed_form(tmp_path, order):
     lam, p0, T, tick, n = 2.0, 0.2, 2.0, 0.25, 200000
     b, c = _write(tmp_path, "toggle", "node A { logic = !A; rate_up = @logic ? $l : 0; rate_down = @logic ? 0 : $l; }",
                   f"$l = {lam}; A.istate = {p0}; max_time = {T}; time_tick = {tick}; sample_count = {n};")
-    run_oracle(b, c, "--raw", tmp_path / "t.raw", "--threads", 4)
+    run_oracle(b, c, "--raw", tmp_path / "t.raw", "--threads", 4, "--order", order)
     raw = read_raw(tmp_path / "t.raw")
     for w in range(8):
         a_, b_ = w * tick, (w + 1) * tick
@@ -46,12 +51,13 @@ def test_toggle_closed_form(tmp_path):
     assert raw["T"][1] == n and sum(raw["F"].values()) == n
 
 
-def test_decay_closed_form_and_fixed_points(tmp_path):
+@ORDERS
+def test_decay_closed_form_and_fixed_points(tmp_path, order):
     lam, T, tick, n = 0.7, 3.0, 0.5, 200000
     b, c = _write(tmp_path, "decay", "node A { rate_up = 0; rate_down = $l; }\nnode B { rate_up = 0; rate_down = 0; }",
                   f"$l = {lam}; A.istate = 1; B.istate = 0; B.is_internal = 1; max_time = {T}; time_tick = {tick}; "
                   f"sample_count = {n};")
-    run_oracle(b, c, "--raw", tmp_path / "d.raw", "--threads", 4)
+    run_oracle(b, c, "--raw", tmp_path / "d.raw", "--threads", 4, "--order", order)
     raw = read_raw(tmp_path / "d.raw")
     for w in range(6):
         a_, b_ = w * tick, (w + 1) * tick
@@ -65,11 +71,12 @@ def test_decay_closed_form_and_fixed_points(tmp_path):
     assert raw["F"].get(0, 0) + raw["F"].get(1, 0) == n
 
 
-def test_discrete_exact_counts(tmp_path):
+@ORDERS
+def test_discrete_exact_counts(tmp_path, order):
     n = 1000
     b, c = _write(tmp_path, "disc", "node A { logic = !A; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }",
                   f"A.istate = 0; max_time = 5; time_tick = 1; discrete_time = 1; sample_count = {n};")
-    run_oracle(b, c, "--raw", tmp_path / "x.raw")
+    run_oracle(b, c, "--raw", tmp_path / "x.raw", "--order", order)
     raw = read_raw(tmp_path / "x.raw")
     expect = {(w, w % 2): n for w in range(5)}
     assert raw["W"] == expect
@@ -77,12 +84,13 @@ def test_discrete_exact_counts(tmp_path):
     assert raw["F"] == {0: n}  # state of the last window: no flip after the last record
 
 
-def test_two_equal_togglers_entropy_one_bit(tmp_path):
+@ORDERS
+def test_two_equal_togglers_entropy_one_bit(tmp_path, order):
     n, T = 20000, 4.0
     bnd = "".join(f"node {x} {{ logic = !{x}; rate_up = @logic ? 1 : 0; rate_down = @logic ? 0 : 1; }}\n"
                   for x in "AB")
     b, c = _write(tmp_path, "two", bnd, f"max_time = {T}; time_tick = 1; sample_count = {n};")
-    run_oracle(b, c, "--raw", tmp_path / "e.raw")
+    run_oracle(b, c, "--raw", tmp_path / "e.raw", "--order", order)
     raw = read_raw(tmp_path / "e.raw")
     for w in range(4):
         assert abs(raw["E"][w] / (n * TWO32) - 1.0) < 1e-7
@@ -130,10 +138,11 @@ RES_CASES = [
 ]
 
 
+@ORDERS
 @pytest.mark.parametrize("name,n,finals", RES_CASES)
-def test_reference_res_3sigma(tmp_path, name, n, finals):
+def test_reference_res_3sigma(tmp_path, name, n, finals, order):
     bnd, cfg = model_paths(name)
-    run_oracle(bnd, cfg, "--n", n, "--compiled", "--print", tmp_path / "o.txt", timeout=900)
+    run_oracle(bnd, cfg, "--n", n, "--compiled", "--order", order, "--print", tmp_path / "o.txt", timeout=900)
     ref = parse_report(os.path.join(RES, name + ".res.txt"))
     new = parse_report(str(tmp_path / "o.txt"))
     n_ref = mg.Model.load(bnd, cfg).info.sample_count
@@ -152,3 +161,78 @@ def test_reference_res_3sigma(tmp_path, name, n, finals):
     worst = max(zs)
     assert (z > 3).mean() <= 0.01, (name, (z > 3).mean(), worst)
     assert z.max() < 5, (name, worst)
+
+
+# ---- the oracle's likelihood-ratio gradient (--grad-in) against closed forms
+
+TOGGLE_UD = """Node A {
+	logic = !A;
+	rate_up = @logic ? $u : 0;
+	rate_down = @logic ? 0 : $d;
+}
+"""
+DECAY_D = """Node A {
+	logic = !A;
+	rate_up = @logic ? 0 : 0;
+	rate_down = @logic ? 0 : $d;
+}
+"""
+
+
+def _grad_batches(tmp_path, b, c, G, n_batch, k_batches):
+    gpath = tmp_path / "G.bin"
+    np.ascontiguousarray(G, dtype=np.float64).tofile(gpath)
+    vals = []
+    for i in range(k_batches):
+        out = tmp_path / f"g{i}.txt"
+        run_oracle(b, c, "--n", n_batch, "--first-id", i * n_batch, "--threads", 8, "--grad-in", gpath,
+                   "--grad-out", out)
+        d = dict(line.split() for line in open(out))
+        vals.append({k: float(v) for k, v in d.items() if k != "transitions"})
+    names = sorted(vals[0])
+    arr = np.array([[v[k] for k in names] for v in vals])
+    return names, arr.mean(0), arr.std(0, ddof=1) / math.sqrt(k_batches)
+
+
+def test_oracle_gradient_toggle_closed_form(tmp_path):
+    """Two-state chain, u (off->on), d (on->off): P(on, t) = pi + (p0 - pi) e^{-(u+d) t}, pi = u/(u+d);
+    L = sum_w P_w(on), differentiated analytically (central differences of the closed form)."""
+    u, d, p0, T, tick = 1.25, 0.5, 0.3, 4.0, 0.5
+    b, c = _write(tmp_path, "tog", TOGGLE_UD, f"$u = {u}; $d = {d}; A.istate = {p0}; sample_count = 10; "
+                  f"max_time = {T}; time_tick = {tick}; discrete_time = 0; seed_pseudorandom = 7;")
+    W = int(round(T / tick))
+    G = np.zeros((W, 2))
+    G[:, 1] = 1.0
+
+    def L(u, d):
+        lam = u + d
+        pi = u / lam
+        return sum(pi + (p0 - pi) * (math.exp(-lam * w * tick) - math.exp(-lam * (w + 1) * tick)) / (lam * tick)
+                   for w in range(W))
+
+    h = 1e-6
+    exact = {"$u": (L(u + h, d) - L(u - h, d)) / (2 * h), "$d": (L(u, d + h) - L(u, d - h)) / (2 * h)}
+    names, mean, se = _grad_batches(tmp_path, b, c, G, 250000, 8)
+    for k in ("$u", "$d"):
+        j = names.index(k)
+        assert abs(mean[j] - exact[k]) < 4.5 * se[j], (k, mean[j], exact[k], se[j])
+        assert se[j] < 0.02 * abs(exact[k])
+
+
+def test_oracle_gradient_decay_closed_form(tmp_path):
+    """Pure decay from on: P(on, t) = e^{-d t}, with window weights 1..2 on P_w(on)."""
+    d, T, tick = 0.8, 3.0, 0.25
+    b, c = _write(tmp_path, "dec", DECAY_D, f"$d = {d}; A.istate = 1; sample_count = 10; max_time = {T}; "
+                  f"time_tick = {tick}; discrete_time = 0; seed_pseudorandom = 7;")
+    W = int(round(T / tick))
+    G = np.zeros((W, 2))
+    G[:, 1] = np.linspace(1.0, 2.0, W)
+
+    def L(d):
+        return sum(G[w, 1] * (math.exp(-d * w * tick) - math.exp(-d * (w + 1) * tick)) / (d * tick) for w in range(W))
+
+    exact = (L(d + 1e-6) - L(d - 1e-6)) / 2e-6
+    names, mean, se = _grad_batches(tmp_path, b, c, G, 250000, 8)
+    assert names == ["$d"]
+    assert abs(mean[0] - exact) < 4.5 * se[0], (mean[0], exact, se[0])
+    assert se[0] < 0.02 * abs(exact)
