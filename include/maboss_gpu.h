@@ -157,6 +157,18 @@ int mbg_simulate(mbg_program* p, const mbg_sim_params* params, void* stream, mbg
  * (src/statistics/window_average_small.cpp:91-107, final_states.cpp:45-51). */
 int mbg_simulate_async(mbg_program* p, const mbg_sim_params* params, void* stream, mbg_result** out);
 
+/* The window probabilities of a run as a dense device array, for the
+ * differentiable form: probs[w << n_observed | x] (device, doubles,
+ * windows << n_observed entries) = P_w(x) normalised as by
+ * mbg_result_window_probs; rates: the rate parameters in device memory
+ * (float[n_params], mbg_model_param_name order; NULL: the model's values);
+ * params->rates must be NULL.  Stream ordered: nothing goes through the host
+ * and the host does not wait.  Lane kernel with dense window cells (at most
+ * 24 observed nodes), else MBG_ERR_LIMIT.  No counterpart in the reference,
+ * whose rates are compiled in (src/generator.cpp:104-106). */
+int mbg_window_probs_device(mbg_program* p, const mbg_sim_params* params, const float* rates, double* probs,
+							void* stream);
+
 /* ---- batched variants.  No counterpart in the reference: its rates are
  *      baked into the generated code (src/generator.cpp:104-106), one model
  *      runs per process (src/main.cpp:154-164) and mutants are a TODO

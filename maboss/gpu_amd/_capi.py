@@ -30,7 +30,7 @@ EXPORTED = [
     "mbg_model_eval_rate",
     "mbg_generate_code", "mbg_free_string", "mbg_program_compile", "mbg_program_free",
     "mbg_program_compile_seconds",
-    "mbg_sim_params_default", "mbg_simulate", "mbg_simulate_async",
+    "mbg_sim_params_default", "mbg_simulate", "mbg_simulate_async", "mbg_window_probs_device",
     "mbg_batch_params_default", "mbg_simulate_batched",
     "mbg_grad_params_default", "mbg_grad_check", "mbg_backward", "mbg_backward_noalloc",
     "mbg_result_free", "mbg_result_wait", "mbg_result_get_info", "mbg_result_window_size", "mbg_result_window",
@@ -169,6 +169,7 @@ def _declare(lib):
         "mbg_sim_params_default": ([ctypes.POINTER(SimParams)], None),
         "mbg_simulate": ([P, ctypes.POINTER(SimParams), P, pp], ctypes.c_int),
         "mbg_simulate_async": ([P, ctypes.POINTER(SimParams), P, pp], ctypes.c_int),
+        "mbg_window_probs_device": ([P, ctypes.POINTER(SimParams), P, P, P], ctypes.c_int),
         "mbg_batch_params_default": ([ctypes.POINTER(BatchParams)], None),
         "mbg_grad_params_default": ([ctypes.POINTER(GradParams)], None),
         "mbg_grad_check": ([P], ctypes.c_int),

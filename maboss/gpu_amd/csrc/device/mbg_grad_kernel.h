@@ -18,10 +18,15 @@
 //   sum_w Gw S_j(e_w) = Gtot * (n_j / theta_j - T_j)
 //                       - sum_{jumps of j} C(t_k) / theta_j + BT_j,
 //   T_j = integral c_j du,  BT_j = integral c_j C du.
-// Per lane and parameter T_j and BT_j live in registers (float, one
-// trajectory), n_j in LDS (u16, lane private); the C(t_k) term and the
-// trajectory totals go to per-block double accumulators in LDS, and the
-// block totals to the output with one atomic per parameter.
+// Per lane and parameter T_j and BT_j live in registers in double precision
+// (the per-trajectory score accumulator), fed with each record's dwell and
+// C-weighted dwell summed in double over its window slices; n_j lives in LDS
+// (u16, lane private).  A trajectory's total is folded into the BT registers
+// when it ends (they then carry the lane's running sum over its
+// trajectories, and the next trajectory's BT adds to it), so ending a
+// trajectory is register work; the C(t_k) term goes to per-block double
+// accumulators in LDS at each jump, the lanes' sums at the end, and the block
+// totals to the output with one atomic per parameter.
 //
 // The trajectory is the forward kernel's: same Philox stream (seed, id,
 // variant), same rates (runtime values through mbg_eval_p), same time and
@@ -41,14 +46,24 @@
 #define MBG_GRAD_WAVES 1
 #endif
 
+// Trajectories of a wave that have ended wait, without stepping, until this
+// many of the wave's lanes have ended (or no lane is still running), and are
+// then folded into the lanes' running totals together: the fold touches
+// every parameter, so doing it for one lane at a time would run it on most
+// steps with one or two lanes active.
+#define MBG_GRAD_FOLD_LANES 16
+
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MBG_GRAD_WAVES)))
 simulate_grad(mbg_grad_args a)
 {
 	constexpr int NP = MBG_GRAD_NP;
 	constexpr int W = MBG_W;
+	// u16 jump counts per lane, lane-major: an odd number of 32-bit words per
+	// lane, so the lanes' counts of one parameter fall in distinct banks
+	constexpr int CW = ((NP + 1) / 2) | 1;
 	extern __shared__ double glds[];
 	// LDS: [NP] block accumulators, [NP] 1/theta, [cells] G (optional),
-	// [2N] jump parameters (u16), [NP][blockDim] jump counts (u16)
+	// [2N] jump parameters (u16), [blockDim][2 CW] jump counts (u16)
 	double* acc = glds;
 	double* invt = glds + NP;
 	const uint64_t cells = (uint64_t)a.windows << a.n_obs;
@@ -66,12 +81,12 @@ simulate_grad(mbg_grad_args a)
 			gtab[i] = a.gprob[i];
 	for (int i = threadIdx.x; i < 2 * a.n_nodes; i += blockDim.x)
 		jp[i] = a.jparam[i];
-	for (int i = threadIdx.x; i < NP * (int)blockDim.x; i += blockDim.x)
+	for (int i = threadIdx.x; i < 2 * CW * (int)blockDim.x; i += blockDim.x)
 		cnt[i] = 0;
 	mbg_eval_p_setup(&a.pv);
 	__syncthreads();
 	const double* G = a.g_lds ? gtab : a.gprob;
-	unsigned short* mycnt = cnt + threadIdx.x; // count of parameter j at mycnt[j * blockDim.x]
+	unsigned short* const mycnt = cnt + 2 * CW * threadIdx.x; // count of parameter j at mycnt[j]
 
 	const int lane = __lane_id();
 	const uint32_t B0 = ((uint32_t)a.n_nodes + 3u) >> 2;
@@ -79,29 +94,49 @@ simulate_grad(mbg_grad_args a)
 #pragma unroll
 	for (int w = 0; w < W; ++w)
 		s[w] = 0u;
-	float T[NP], BT[NP];
+	// T[j]: the current trajectory's integral of c_j; R[j]: the contributions
+	// of the lane's folded trajectories plus the current one's BT_j
+	double T[NP], R[NP];
 #pragma unroll
 	for (int j = 0; j < NP; ++j)
 	{
-		T[j] = 0.0f;
-		BT[j] = 0.0f;
+		T[j] = 0.0;
+		R[j] = 0.0;
 	}
 	float t = 0.0f;
 	uint32_t k = 0;
 	uint64_t id = 0;
-	bool active = false, exhausted = false;
+	bool active = false, waiting = false, exhausted = false;
 	double C = 0.0, gw = 0.0; // completed windows' G; G of the window in progress
 	uint64_t ntrans = 0;
 
 	mbg_dev::id_chunks queue;
 	for (;;)
 	{
-		const uint64_t need = __ballot(!active && !exhausted);
+		// fold the ended trajectories: Gtot * (n_j / theta_j - T_j) joins R_j
+		const uint64_t wm = __ballot(waiting);
+		if (wm && (__popcll(wm) >= MBG_GRAD_FOLD_LANES || __ballot(active) == 0ull))
+		{
+			if (waiting)
+			{
+				const double gt = C + gw;
+#pragma unroll
+				for (int j = 0; j < NP; ++j)
+				{
+					R[j] = fma(gt, fma((double)mycnt[j], invt[j], -T[j]), R[j]);
+					mycnt[j] = 0;
+					T[j] = 0.0;
+				}
+				waiting = false;
+			}
+		}
+
+		const uint64_t need = __ballot(!active && !waiting && !exhausted);
 		if (need)
 		{
 			const uint64_t mine = queue.take(need, a.counters + MBG_CTR_QUEUE, a.n_traj, 64);
 			bool fresh = false;
-			if (!active && !exhausted)
+			if (!active && !waiting && !exhausted)
 			{
 				if (mine < a.n_traj)
 				{
@@ -118,10 +153,9 @@ simulate_grad(mbg_grad_args a)
 			}
 			mbg_dev::init_states_wave<W>(s, fresh, id, a.variant, a.seed_lo, a.seed_hi, a.n_nodes, a.istate);
 		}
-		if (__ballot(active) == 0)
+		if (__ballot(active || waiting) == 0)
 			break;
 
-		bool ended = false;
 		if (active)
 		{
 			float u1, u2;
@@ -137,8 +171,8 @@ simulate_grad(mbg_grad_args a)
 			// dwell slices of [tp, t): G weights, window completions, C-weighted dwell
 			const uint64_t obs = mbg_obs_index(s);
 			int w = (int)floorf(tp / a.tick);
-			float b = tp, dsum = 0.0f;
-			double cdt = 0.0;
+			float b = tp;
+			double dsum = 0.0, cdt = 0.0;
 			while (t > b)
 			{
 				const float wend = (float)(w + 1) * a.tick;
@@ -147,7 +181,7 @@ simulate_grad(mbg_grad_args a)
 				cdt += C * (double)sl;
 				if (w < a.windows)
 					gw += G[((uint64_t)w << a.n_obs) | obs] * (double)sl;
-				dsum += sl;
+				dsum += (double)sl;
 				if (en == wend)
 				{
 					C += gw;
@@ -156,11 +190,14 @@ simulate_grad(mbg_grad_args a)
 				++w;
 				b = en;
 			}
-			mbg_grad_dwell(s, en, dsum, (float)cdt, T, BT);
+			mbg_grad_dwell(s, en, dsum, cdt, T, R);
 
 			k++;
 			if (t >= a.max_time || k >= a.max_steps)
-				ended = true;
+			{
+				active = false;
+				waiting = true;
+			}
 			else
 			{
 				const int f = e.flip;
@@ -168,7 +205,7 @@ simulate_grad(mbg_grad_args a)
 				const unsigned p = jp[2 * f + (int)xf];
 				if (p != 0xffffu)
 				{
-					mycnt[p * blockDim.x] += 1;
+					mycnt[p] += 1;
 					atomicAdd(acc + p, -invt[p] * C);
 				}
 #pragma unroll
@@ -180,31 +217,25 @@ simulate_grad(mbg_grad_args a)
 #pragma unroll
 					for (int j = 0; j < NP; ++j)
 					{
-						T[j] -= (float)((double)mycnt[j * blockDim.x] * invt[j]);
-						mycnt[j * blockDim.x] = 0;
+						T[j] -= (double)mycnt[j] * invt[j];
+						mycnt[j] = 0;
 					}
 				}
 			}
 		}
-
-		if (ended)
-		{
-			// trajectory total: Gtot * (n_j / theta_j - T_j) + BT_j
-			const double gt = C + gw;
-#pragma unroll
-			for (int j = 0; j < NP; ++j)
-			{
-				const double c = gt * ((double)mycnt[j * blockDim.x] * invt[j] - (double)T[j]) + (double)BT[j];
-				mycnt[j * blockDim.x] = 0;
-				if (c != 0.0)
-					atomicAdd(acc + j, c);
-				T[j] = 0.0f;
-				BT[j] = 0.0f;
-			}
-			active = false;
-		}
 	}
 
+	// the lanes' running totals to the block accumulators, one atomic per
+	// parameter and wave
+#pragma unroll
+	for (int j = 0; j < NP; ++j)
+	{
+		double v = R[j];
+		for (int off = 32; off > 0; off >>= 1)
+			v += __shfl_xor(v, off);
+		if (lane == 0 && v != 0.0)
+			atomicAdd(acc + j, v);
+	}
 	for (int off = 32; off > 0; off >>= 1)
 		ntrans += __shfl_xor(ntrans, off);
 	if (lane == 0 && ntrans)

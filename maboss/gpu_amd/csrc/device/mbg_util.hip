@@ -15,6 +15,8 @@
 //                    which is exactly the sum of the per-window additions.
 // mbg_table_compact -- occupied slots of a hash table as (key, value) pairs.
 // mbg_fp_compact   -- published slots of the fixed-point table as records.
+// mbg_rate_view    -- parameter view of run-time rates held in device memory.
+// mbg_cells_to_probs -- dense window cells to probabilities (double).
 #include <hip/hip_runtime.h>
 
 #include "mbg_kernel_args.h"
@@ -152,4 +154,37 @@ extern "C" __global__ void __launch_bounds__(256) mbg_fp_compact(const unsigned 
 		for (int w = 1; w < stride; ++w)
 			r[w - 1] = e[w];
 	}
+}
+
+// Parameter view of run-time rates held in device memory: the view header
+// (pointers into the same block), rate * log2(rate) of each rate (the same
+// float code as the host's table), and enable masks with every bit set.
+extern "C" __global__ void __launch_bounds__(256) mbg_rate_view(mbg_param_view* view, const float* rate, float* rate_lr,
+															   uint32_t* en, int n_params, int n_words)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i == 0)
+	{
+		mbg_param_view v;
+		v.rate = rate;
+		v.rate_lr = rate_lr;
+		v.up_en = en;
+		v.dn_en = en + n_words;
+		*view = v;
+	}
+	if (i < n_params)
+		rate_lr[i] = mbg_rlog2r(rate[i]);
+	if (i < 2 * n_words)
+		en[i] = 0xffffffffu;
+}
+
+// Window probabilities from dense fixed-point cells: raw / denom in double,
+// the division the host result applies (denom = n * 2^32 in continuous
+// time, n in discrete time).
+extern "C" __global__ void __launch_bounds__(256) mbg_cells_to_probs(const unsigned long long* cells,
+																	unsigned long long n, double denom, double* out)
+{
+	for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+		 i += (unsigned long long)gridDim.x * blockDim.x)
+		out[i] = (double)cells[i] / denom;
 }

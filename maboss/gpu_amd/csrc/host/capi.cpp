@@ -276,34 +276,40 @@ void mbg_sim_params_default(mbg_sim_params* p)
 	p->kernel = -1;
 }
 
+static mbg::sim_params to_sim_params(mbg_program* p, const mbg_sim_params* params)
+{
+	mbg::sim_params sp;
+	if (params)
+	{
+		sp.n_traj = params->n_trajectories;
+		sp.first_id = params->first_id;
+		sp.seed = params->seed;
+		sp.variant = params->variant;
+		sp.max_steps = params->max_steps;
+		sp.window_mode = params->window_mode;
+		sp.final_mode = params->final_mode;
+		sp.window_capacity = params->window_capacity;
+		sp.final_capacity = params->final_capacity;
+		sp.fixed_capacity = params->fixed_capacity;
+		sp.block_size = params->block_size;
+		sp.blocks_per_cu = params->blocks_per_cu;
+		sp.normalization = params->normalization;
+		sp.device_only = (params->flags & MBG_SIM_DEVICE_ONLY) != 0;
+		sp.kernel = params->kernel;
+		if (params->rates)
+			sp.rates.assign(params->rates, params->rates + p->p->gen.param_names.size());
+	}
+	if (sp.block_size < 0 || sp.block_size > 1024 || (sp.block_size % 64) != 0)
+		throw mbg::error(mbg::MBG_E_ARG, "block_size must be a multiple of 64 up to 1024");
+	return sp;
+}
+
 static int simulate_entry(mbg_program* p, const mbg_sim_params* params, void* stream, mbg_result** out, bool async)
 {
 	return guarded([&] {
 		if (!p || !out)
 			throw mbg::error(mbg::MBG_E_ARG, "null argument");
-		mbg::sim_params sp;
-		if (params)
-		{
-			sp.n_traj = params->n_trajectories;
-			sp.first_id = params->first_id;
-			sp.seed = params->seed;
-			sp.variant = params->variant;
-			sp.max_steps = params->max_steps;
-			sp.window_mode = params->window_mode;
-			sp.final_mode = params->final_mode;
-			sp.window_capacity = params->window_capacity;
-			sp.final_capacity = params->final_capacity;
-			sp.fixed_capacity = params->fixed_capacity;
-			sp.block_size = params->block_size;
-			sp.blocks_per_cu = params->blocks_per_cu;
-			sp.normalization = params->normalization;
-			sp.device_only = (params->flags & MBG_SIM_DEVICE_ONLY) != 0;
-			sp.kernel = params->kernel;
-			if (params->rates)
-				sp.rates.assign(params->rates, params->rates + p->p->gen.param_names.size());
-		}
-		if (sp.block_size < 0 || sp.block_size > 1024 || (sp.block_size % 64) != 0)
-			throw mbg::error(mbg::MBG_E_ARG, "block_size must be a multiple of 64 up to 1024");
+		const mbg::sim_params sp = to_sim_params(p, params);
 		auto rr = std::make_unique<mbg_result>();
 		rr->r = async ? mbg::simulate_async(*p->p, sp, (hipStream_t)stream)
 					  : mbg::simulate(*p->p, sp, (hipStream_t)stream);
@@ -319,6 +325,19 @@ int mbg_simulate(mbg_program* p, const mbg_sim_params* params, void* stream, mbg
 int mbg_simulate_async(mbg_program* p, const mbg_sim_params* params, void* stream, mbg_result** out)
 {
 	return simulate_entry(p, params, stream, out, true);
+}
+
+int mbg_window_probs_device(mbg_program* p, const mbg_sim_params* params, const float* rates, double* probs,
+							void* stream)
+{
+	return guarded([&] {
+		if (!p || !probs)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		if (params && params->rates)
+			throw mbg::error(mbg::MBG_E_ARG, "pass the rates as the device pointer argument");
+		const mbg::sim_params sp = to_sim_params(p, params);
+		mbg::window_probs_device(*p->p, sp, rates, probs, (hipStream_t)stream);
+	});
 }
 
 void mbg_batch_params_default(mbg_batch_params* p)

@@ -152,7 +152,7 @@ struct program
 	// utility kernels (slot fill, tail expansion, table compaction), precompiled
 	hipModule_t mod_util = nullptr;
 	hipFunction_t fn_fill_slots = nullptr, fn_expand_tails = nullptr, fn_table_compact = nullptr,
-				  fn_fp_compact = nullptr;
+				  fn_fp_compact = nullptr, fn_rate_view = nullptr, fn_cells_to_probs = nullptr;
 	void load_util();
 	// waits for every copy in flight (before buffers are released)
 	void drain_copies();
@@ -191,6 +191,10 @@ struct program
 	// win_mode / fin_mode: MBG_HIST_* of the run (ignored for flavour 1)
 	hipFunction_t function(int flavour, int win_mode, int fin_mode);
 	device_buffer b_rate_view; // run-time rates of the plain flavour 2
+	device_buffer b_pent;      // entropy sums of a device-probability run
+	// initial-state probabilities in pinned memory (their upload is stream ordered)
+	std::shared_ptr<char> istate_pinned;
+	std::vector<char> istate_host;
 	// Kernel for the wave path at the model's nodes-per-lane (loads on first use).
 	hipFunction_t wave_function();
 };
@@ -273,6 +277,12 @@ std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t s
 // copied to pinned host memory on the program's copy stream and the result
 // completes in wait().
 std::unique_ptr<result> simulate_async(program& p, const sim_params& sp, hipStream_t stream);
+// The window probabilities P[w][x] of a run as dense doubles in device memory
+// (probs: [windows << n_obs]), with the rate parameters read from device
+// memory (rates: [n_params]; null: the model's).  Stream ordered: nothing is
+// copied to or from the host and the host does not wait.  Lane kernel,
+// dense window storage (at most 24 observed nodes).
+void window_probs_device(program& p, const sim_params& sp, const float* rates, double* probs, hipStream_t stream);
 // Window and final-state storage modes a default run of the model uses.
 void default_storage_modes(const model& m, int& win_mode, int& fin_mode);
 

@@ -756,8 +756,9 @@ generated generate(const model& m, const gen_options& opt)
 // d total / d theta_j in a state is the number of nodes whose current rate
 // is theta_j: node i in the fast form has rate theta_u (up_param) when off
 // with logic on, theta_d (down_param) when on with logic off, else 0.  The
-// emitted mbg_grad_dwell adds that indicator times the record's dwell (T) and
-// times its G-weighted dwell (BT) to per-parameter register accumulators.
+// emitted mbg_grad_dwell adds the record's dwell (T) and its C-weighted dwell
+// (BT) to the per-parameter double accumulators of the nodes where that
+// indicator is set.
 grad_code generate_grad(const model& m, const generated& g)
 {
 	const int N = m.n_nodes();
@@ -777,7 +778,7 @@ grad_code generate_grad(const model& m, const generated& g)
 	std::ostringstream os;
 	os << "\n#define MBG_GRAD_NP " << gc.n_params << "\n\n";
 	os << "__device__ __forceinline__ void mbg_grad_dwell(const uint32_t* __restrict__ s, const uint32_t* __restrict__ en, "
-		  "float dt, float cdt, float (&T)[MBG_GRAD_NP], float (&BT)[MBG_GRAD_NP])\n{\n";
+		  "double dt, double cdt, double (&T)[MBG_GRAD_NP], double (&BT)[MBG_GRAD_NP])\n{\n";
 	for (int w = 0; w < m.n_words(); w++)
 		os << "\tconst uint32_t sw" << w << " = s[" << w << "], ew" << w << " = en[" << w << "];\n";
 	for (int i = 0; i < N; i++)
@@ -799,20 +800,21 @@ grad_code generate_grad(const model& m, const generated& g)
 		// the node's rate is its up parameter when off and enabled (logic on),
 		// its down parameter when on and enabled (logic off)
 		const int w = i / 32, b = i % 32;
+		// c is 0 or 1, so fma(c, dt, T) is T + dt or T exactly
 		os << "\t{ // node " << i << ": " << n.name << "\n";
-		os << "\t\tconst bool x = ((sw" << w << " >> " << b << "u) & 1u) != 0u;\n";
-		os << "\t\tconst bool e = ((ew" << w << " >> " << b << "u) & 1u) != 0u;\n";
+		os << "\t\tconst uint32_t x = (sw" << w << " >> " << b << "u) & 1u, e = (ew" << w << " >> " << b
+		   << "u) & 1u;\n";
 		if (ni.up_param >= 0)
 		{
 			gc.jparam[2 * (size_t)i] = (unsigned short)ni.up_param;
-			os << "\t\t{ const float c = (!x && e) ? 1.0f : 0.0f; T[" << ni.up_param << "] = fmaf(c, dt, T["
-			   << ni.up_param << "]); BT[" << ni.up_param << "] = fmaf(c, cdt, BT[" << ni.up_param << "]); }\n";
+			os << "\t\t{ const double c = (double)(e & (x ^ 1u)); T[" << ni.up_param << "] = fma(c, dt, T["
+			   << ni.up_param << "]); BT[" << ni.up_param << "] = fma(c, cdt, BT[" << ni.up_param << "]); }\n";
 		}
 		if (ni.down_param >= 0)
 		{
 			gc.jparam[2 * (size_t)i + 1] = (unsigned short)ni.down_param;
-			os << "\t\t{ const float c = (x && e) ? 1.0f : 0.0f; T[" << ni.down_param << "] = fmaf(c, dt, T["
-			   << ni.down_param << "]); BT[" << ni.down_param << "] = fmaf(c, cdt, BT[" << ni.down_param
+			os << "\t\t{ const double c = (double)(e & x); T[" << ni.down_param << "] = fma(c, dt, T["
+			   << ni.down_param << "]); BT[" << ni.down_param << "] = fma(c, cdt, BT[" << ni.down_param
 			   << "]); }\n";
 		}
 		os << "\t}\n";

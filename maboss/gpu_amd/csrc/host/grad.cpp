@@ -30,8 +30,9 @@ void fill_u64(program& p, hipStream_t stream, void* ptr, uint64_t words, uint64_
 // staged, the jump table and the per-lane 16-bit jump counts.
 size_t grad_lds_bytes(int np, uint64_t cells, bool g_lds, int n_nodes, int block)
 {
+	const size_t count_words = (size_t)(((np + 1) / 2) | 1); // per lane, odd (the kernel's layout)
 	return 8 * (size_t)(2 * np) + (g_lds ? 8 * (size_t)cells : 0) + 2 * (size_t)((2 * n_nodes + 1) & ~1)
-		   + 2 * (size_t)np * (size_t)block;
+		   + 4 * count_words * (size_t)block;
 }
 
 } // namespace

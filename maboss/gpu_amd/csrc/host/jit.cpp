@@ -193,11 +193,10 @@ hipFunction_t program::grad_function()
 	const grad_code& gc = grad();
 	scope_timer tm("compiler> gradient kernel");
 	const double t0 = wall_seconds();
-	// Occupancy: 2 waves per SIMD up to 96 rate parameters (their T and BT
-	// accumulators, 2 registers each, then leave room for the rate code; on
-	// the 30-node model with 60 parameters 2 waves with a few spilled
-	// registers ran 1.6x faster than 1 wave without), 1 wave above.
-	const int waves = gc.n_params <= 96 ? 2 : 1;
+	// Occupancy: the T and BT accumulators are doubles, 4 registers per
+	// parameter; 2 waves per SIMD (256 registers) leave room for the rate
+	// code up to 32 parameters, 1 wave (512 registers) above.
+	const int waves = gc.n_params <= 32 ? 2 : 1;
 	const std::string src = "#define MBG_GRAD_WAVES " + std::to_string(waves) + "\n" + gen.source + gc.source
 							+ "\n#include \"mbg_grad_kernel.h\"\n";
 	uint64_t h = fnv1a(src.data(), src.size());
@@ -238,6 +237,9 @@ void program::load_util()
 	hip_check(hipModuleGetFunction(&fn_table_compact, mod_util, "mbg_table_compact"),
 			  "hipModuleGetFunction(mbg_table_compact)");
 	hip_check(hipModuleGetFunction(&fn_fp_compact, mod_util, "mbg_fp_compact"), "hipModuleGetFunction(mbg_fp_compact)");
+	hip_check(hipModuleGetFunction(&fn_rate_view, mod_util, "mbg_rate_view"), "hipModuleGetFunction(mbg_rate_view)");
+	hip_check(hipModuleGetFunction(&fn_cells_to_probs, mod_util, "mbg_cells_to_probs"),
+			  "hipModuleGetFunction(mbg_cells_to_probs)");
 }
 
 hipFunction_t program::wave_function()

@@ -276,7 +276,15 @@ void default_storage_modes(const model& m, int& win_mode, int& fin_mode)
 	fin_mode = pl.fin_mode;
 }
 
-std::unique_ptr<result> simulate_async(program& p, const sim_params& sp, hipStream_t stream)
+namespace
+{
+
+// One run.  With probs set (window_probs_device) the run is stream ordered
+// end to end: rates from device memory (d_rates, or the model's baked ones
+// when null), dense window cells converted to probabilities into probs, no
+// host copy and no host wait; the function then returns null.
+std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t stream, const float* d_rates,
+								 double* probs)
 {
 	scope_timer tm("runner> simulate");
 	const double t0 = wall_seconds();
@@ -315,7 +323,11 @@ std::unique_ptr<result> simulate_async(program& p, const sim_params& sp, hipStre
 		throw error(MBG_E_ARG, "expected " + std::to_string(p.gen.param_names.size()) + " rates");
 	res->kernel = use_wave ? 1 : 0;
 	plan pl = make_plan(m, sp, n_traj, windows, n_obs, use_wave);
-	hipFunction_t fn = use_wave ? p.wave_function() : p.function(sp.rates.empty() ? 0 : 2, pl.win_mode, pl.fin_mode);
+	if (probs && (use_wave || (pl.win_mode != MBG_HIST_DENSE && pl.win_mode != MBG_HIST_LDS) || n_obs > 24))
+		throw error(MBG_E_LIMIT, "device window probabilities need the lane kernel and dense window cells "
+								 "(at most 24 observed nodes)");
+	const bool rt_rates = !sp.rates.empty() || (probs && d_rates);
+	hipFunction_t fn = use_wave ? p.wave_function() : p.function(rt_rates ? 2 : 0, pl.win_mode, pl.fin_mode);
 	p.load_util();
 	if (!p.copy_stream)
 		hip_check(hipStreamCreateWithFlags(&p.copy_stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
@@ -327,8 +339,9 @@ std::unique_ptr<result> simulate_async(program& p, const sim_params& sp, hipStre
 	// before this run writes into it or grows its buffers (a host-side wait:
 	// a buffer may be reallocated below).
 	out_slot& slot = p.slots[p.slot_next];
-	p.slot_next ^= 1;
-	if (slot.pending)
+	if (!probs)
+		p.slot_next ^= 1;
+	if (slot.pending && !probs)
 	{
 		hip_check(hipEventSynchronize(slot.copied), "hipEventSynchronize");
 		slot.pending = false;
@@ -337,7 +350,23 @@ std::unique_ptr<result> simulate_async(program& p, const sim_params& sp, hipStre
 
 	// run-time rates: a parameter view {rate, rate*log2(rate), enable masks} in device memory
 	const mbg_param_view* d_view = nullptr;
-	if (!sp.rates.empty())
+	if (probs && d_rates)
+	{
+		// the view over the caller's device rates, built on the device
+		const int np = (int)p.gen.param_names.size();
+		const size_t off_lr = (sizeof(mbg_param_view) + 15) & ~(size_t)15, off_en = off_lr + ((size_t)np * 4 + 15) / 16 * 16;
+		char* base = (char*)p.b_rate_view.ensure(off_en + 8 * (size_t)W);
+		auto* view = (mbg_param_view*)base;
+		auto* lr = (float*)(base + off_lr);
+		auto* en = (uint32_t*)(base + off_en);
+		int nw = W;
+		void* args[] = { &view, &d_rates, &lr, &en, (void*)&np, &nw };
+		const unsigned g = (unsigned)std::max(1, (std::max(np, 2 * W) + 255) / 256);
+		hip_check(hipModuleLaunchKernel(p.fn_rate_view, g, 1, 1, 256, 1, 1, 0, stream, args, nullptr),
+				  "hipModuleLaunchKernel(mbg_rate_view)");
+		d_view = view;
+	}
+	else if (!sp.rates.empty())
 	{
 		const size_t np = sp.rates.size();
 		const size_t off_rate = (sizeof(mbg_param_view) + 15) & ~(size_t)15;
@@ -362,8 +391,9 @@ std::unique_ptr<result> simulate_async(program& p, const sim_params& sp, hipStre
 		hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 		d_view = (const mbg_param_view*)base;
 	}
-	// tail records for window storage in HBM (continuous time)
-	const bool tails = !m.discrete_time() && pl.win_mode != MBG_HIST_LDS;
+	// tail records for window storage in HBM (continuous time); not in the
+	// stream-ordered probability run, which reads no overflow counter
+	const bool tails = !m.discrete_time() && pl.win_mode != MBG_HIST_LDS && !probs;
 	uint64_t tail_cap = tails ? next_pow2(std::min<uint64_t>(std::max<uint64_t>(n_traj / 4, 1 << 12), 1 << 20)) : 0;
 	// sorted window mode: pair capacity from the caller, the last run, or windows per trajectory
 	uint64_t pair_cap = 0;
@@ -384,7 +414,7 @@ std::unique_ptr<result> simulate_async(program& p, const sim_params& sp, hipStre
 		// ---- storage
 		auto* counters = (unsigned long long*)p.b_counters.ensure(MBG_NCOUNTERS * 8);
 		hip_check(hipMemsetAsync(counters, 0, MBG_NCOUNTERS * 8, stream), "hipMemsetAsync");
-		auto* ent = (unsigned long long*)slot.ent.ensure((size_t)windows * 8);
+		auto* ent = (unsigned long long*)(probs ? p.b_pent : slot.ent).ensure((size_t)windows * 8);
 		hip_check(hipMemsetAsync(ent, 0, (size_t)windows * 8, stream), "hipMemsetAsync");
 		unsigned long long *win_vals = nullptr, *win_keys = nullptr, *fin_vals = nullptr, *fin_keys = nullptr;
 		unsigned long long *pk = nullptr, *pv = nullptr;
@@ -423,7 +453,14 @@ std::unique_ptr<result> simulate_async(program& p, const sim_params& sp, hipStre
 		hip_check(hipMemsetAsync(fp, 0, pl.fp_cap * fp_stride * 4, stream), "hipMemsetAsync");
 
 		auto* d_istate = (float*)p.b_istate.ensure(istate.size() * 4);
-		hip_check(hipMemcpyAsync(d_istate, istate.data(), istate.size() * 4, hipMemcpyHostToDevice, stream),
+		if (p.istate_host.size() != istate.size() * 4)
+		{
+			// pinned, owned by the program: the copy may run after this call returns
+			p.istate_pinned = p.pool->get(istate.size() * 4);
+			std::memcpy(p.istate_pinned.get(), istate.data(), istate.size() * 4);
+			p.istate_host.assign((const char*)istate.data(), (const char*)istate.data() + istate.size() * 4);
+		}
+		hip_check(hipMemcpyAsync(d_istate, p.istate_pinned.get(), istate.size() * 4, hipMemcpyHostToDevice, stream),
 				  "hipMemcpyAsync");
 
 		// ---- kernel arguments
@@ -558,6 +595,19 @@ std::unique_ptr<result> simulate_async(program& p, const sim_params& sp, hipStre
 			const unsigned eg = (unsigned)std::min<uint64_t>(1024, (tail_cap + 255) / 256);
 			hip_check(hipModuleLaunchKernel(p.fn_expand_tails, eg, 1, 1, 256, 1, 1, 0, stream, args, nullptr),
 					  "hipModuleLaunchKernel(mbg_expand_tails)");
+		}
+
+		if (probs)
+		{
+			// dense cells to probabilities, stream ordered, and done
+			void* cargs[] = { &win_vals, (void*)&pl.win_cells, nullptr, &probs };
+			double denom = res->discrete ? res->norm : res->norm * 4294967296.0;
+			cargs[2] = &denom;
+			const unsigned g = (unsigned)std::min<uint64_t>(4096, std::max<uint64_t>(1, (pl.win_cells + 255) / 256));
+			hip_check(hipModuleLaunchKernel(p.fn_cells_to_probs, g, 1, 1, 256, 1, 1, 0, stream, cargs, nullptr),
+					  "hipModuleLaunchKernel(mbg_cells_to_probs)");
+			p.mark_run_done(stream);
+			return nullptr;
 		}
 
 		// ---- the occupied cells of every table, compacted into the output slot
@@ -758,11 +808,27 @@ std::unique_ptr<result> simulate_async(program& p, const sim_params& sp, hipStre
 	}
 }
 
+} // namespace
+
+std::unique_ptr<result> simulate_async(program& p, const sim_params& sp, hipStream_t stream)
+{
+	return run_impl(p, sp, stream, nullptr, nullptr);
+}
+
 std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t stream)
 {
-	auto r = simulate_async(p, sp, stream);
+	auto r = run_impl(p, sp, stream, nullptr, nullptr);
 	r->wait();
 	return r;
+}
+
+void window_probs_device(program& p, const sim_params& sp, const float* rates, double* probs, hipStream_t stream)
+{
+	if (!probs)
+		throw error(MBG_E_ARG, "null probability buffer");
+	if (!sp.rates.empty())
+		throw error(MBG_E_ARG, "device window probabilities take their rates from device memory");
+	(void)run_impl(p, sp, stream, rates, probs);
 }
 
 } // namespace mbg

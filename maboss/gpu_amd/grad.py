@@ -3,25 +3,25 @@ engine's C-ABI.
 
 No counterpart in the reference (its rates are compile-time constants and it
 differentiates nothing).  ``forward`` runs the simulate kernel with the rate
-vector ``theta`` given at run time (stream tag 0) and returns the window
-probabilities ``P[w, x]`` (observed state ``x``) as a float64 tensor;
-``backward`` hands ``dL/dP`` to ``mbg_backward``, which replays the same
-trajectories and returns the likelihood-ratio estimate of ``dL/dtheta``.
-Both run on the GPU: the backward reads and writes device tensors through
-their ``data_ptr`` on torch's current stream.
+vector ``theta`` read from device memory (stream tag 0) and returns the window
+probabilities ``P[w, x]`` (observed state ``x``) as a float64 device tensor
+normalised on the device (``mbg_window_probs_device``); ``backward`` hands
+``dL/dP`` to ``mbg_backward``, which replays the same trajectories and
+returns the likelihood-ratio estimate of ``dL/dtheta``.  Both run on torch's
+current stream and read and write device tensors through their
+``data_ptr``; nothing goes through the host.
 """
 from __future__ import annotations
 
 import ctypes
 
-import numpy as np
+from ._capi import GradInfo, GradParams, SimParams, check, lib
 
-from ._capi import GradInfo, GradParams, check, lib
-
-__all__ = ["window_probabilities", "WindowProbabilities", "observed_marginals", "backward", "ACCUMULATOR"]
+__all__ = ["window_probabilities", "WindowProbabilities", "observed_marginals", "backward", "window_probs_device",
+           "ACCUMULATOR"]
 
 # Precision of the score accumulation: per trajectory, then across trajectories.
-ACCUMULATOR = "f32 per trajectory, f64 across trajectories"
+ACCUMULATOR = "f64 per trajectory, f64 across trajectories"
 
 
 def _torch():
@@ -65,11 +65,32 @@ def backward(program, grad_probs, rates=None, n_trajectories: int = -1, seed: in
     return (out, gi) if info else out
 
 
-def _dense_probs(result, windows: int, n_obs: int) -> np.ndarray:
-    P = np.zeros((windows, 1 << n_obs), dtype=np.float64)
-    for w in range(windows):
-        win = result.window(w)
-        P[w, win.states.astype(np.int64)] = win.probs
+def window_probs_device(program, rates=None, n_trajectories: int = -1, seed: int = -1, max_steps: int = 0,
+                        first_id: int = 0):
+    """P[w, x] of a run as a device float64 tensor [windows, 2^n_obs], computed and
+    normalised on the device (``mbg_window_probs_device``) on torch's current stream;
+    ``rates`` is a device float32 tensor of the parameters (None: the model's).  The
+    tensors are allocated on that stream, so the caching allocator orders their reuse
+    after the kernels that read and write them."""
+    torch = _torch()
+    m = program.model.info
+    dev = rates.device if rates is not None and rates.is_cuda else torch.device("cuda")
+    r = None
+    if rates is not None:
+        r = rates.detach().to(device=dev, dtype=torch.float32).contiguous()
+        if r.numel() != m.n_params:
+            raise ValueError(f"rates must have {m.n_params} entries")
+    P = torch.empty((m.windows, 1 << m.n_observed), dtype=torch.float64, device=dev)
+    p = SimParams()
+    lib().mbg_sim_params_default(ctypes.byref(p))
+    p.n_trajectories = n_trajectories
+    p.seed = seed
+    p.max_steps = max_steps
+    p.first_id = first_id
+    s = torch.cuda.current_stream(dev).cuda_stream
+    check(lib().mbg_window_probs_device(program._h, ctypes.byref(p),
+                                        ctypes.c_void_p(r.data_ptr()) if r is not None else None,
+                                        ctypes.c_void_p(P.data_ptr()), ctypes.c_void_p(s)))
     return P
 
 
@@ -86,17 +107,16 @@ class WindowProbabilities:
             class _F(torch.autograd.Function):
                 @staticmethod
                 def forward(ctx, theta, program, n_trajectories, seed, max_steps):
-                    m = program.model.info
-                    th = theta.detach().to(torch.float32).cpu().numpy()
-                    res = program.simulate(n_trajectories=n_trajectories, seed=seed, max_steps=max_steps, rates=th)
-                    P = _dense_probs(res, m.windows, m.n_observed)
+                    dev = theta.device if theta.is_cuda else torch.device("cuda")
+                    th = theta.detach().to(device=dev, dtype=torch.float32)
+                    P = window_probs_device(program, th, n_trajectories=n_trajectories, seed=seed,
+                                            max_steps=max_steps)
                     ctx.program = program
                     ctx.n = n_trajectories
                     ctx.seed = seed
                     ctx.max_steps = max_steps
                     ctx.save_for_backward(theta)
-                    dev = theta.device if theta.is_cuda else torch.device("cuda")
-                    return torch.from_numpy(P).to(dev)
+                    return P
 
                 @staticmethod
                 def backward(ctx, gP):

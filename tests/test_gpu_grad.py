@@ -2,10 +2,12 @@
 
 * mbg_backward against the oracle's double-precision estimator on the same
   trajectories (the oracle evaluates sum_w Gw * S(e_w) directly, the kernel
-  its suffix form, so agreement also checks that algebra);
+  its suffix form, so agreement also checks that algebra); both accumulate
+  in double, so every component agrees to 1e-6 relative;
 * against closed forms (a two-state toggle and pure decay), statistically;
 * the torch.autograd.Function end to end, against mbg_backward and against
-  central finite differences with common random numbers.
+  central finite differences with common random numbers (step 1e-3 of each
+  rate).
 """
 import math
 import re
@@ -87,10 +89,11 @@ def test_backward_matches_oracle(tmp_path, g30):
     scale = np.abs(o).max()
     assert scale > 0
     err = np.abs(got - o)
-    # float32 per-trajectory accumulators in the kernel, double in the oracle
-    assert err.max() <= 1e-4 * scale, (err.max(), scale)
-    big = np.abs(o) > 1e-2 * scale
-    assert (err[big] <= 1e-3 * np.abs(o[big])).all()
+    # Both accumulate every score in double on the same trajectories; they
+    # differ only by rounding (suffix form vs direct form, atomic order).
+    # Every component within 1e-6 relative, with an absolute floor of
+    # 1e-10 of the largest component for components that average to ~0.
+    assert np.all(err <= 1e-6 * np.abs(o) + 1e-10 * scale), (err / np.maximum(np.abs(o), 1e-300)).max()
 
 
 def test_backward_runtime_rates_and_shards(tmp_path, g30):
@@ -111,7 +114,7 @@ def test_backward_runtime_rates_and_shards(tmp_path, g30):
     got = gpu_grad(prog, G, n, rates=rates)
     want = oracle_grad(tmp_path, bnd, str(cfg2), G, n, tag="s")
     o = np.array([want[k] for k in names])
-    assert np.abs(got - o).max() <= 1e-4 * np.abs(o).max()
+    assert np.all(np.abs(got - o) <= 1e-6 * np.abs(o) + 1e-10 * np.abs(o).max())
     # ids [0, 5000) + [5000, 12000), each normalised by its own count
     a = gpu_grad(prog, G, 5000, rates=rates)
     b = gpu_grad(prog, G, 7000, first_id=5000, rates=rates)
@@ -199,7 +202,7 @@ def test_autograd_function_and_crn_finite_differences(g30):
         lrs.append(th.grad[k].numpy())
         row = []
         for j in k.tolist():
-            h = 1e-2 * float(base[j])
+            h = 1e-3 * float(base[j])
             with torch.no_grad():
                 tp = base.clone()
                 tp[j] += h
@@ -211,6 +214,34 @@ def test_autograd_function_and_crn_finite_differences(g30):
     diff = lrs - fds
     se = diff.std(0, ddof=1) / math.sqrt(len(diff))
     assert np.all(np.abs(diff.mean(0)) < 4.5 * se + 0.02 * np.abs(lrs.mean(0))), (lrs.mean(0), fds.mean(0), se)
+
+
+def _dense(res, windows, n_obs):
+    Q = np.zeros((windows, 1 << n_obs))
+    for w in range(windows):
+        win = res.window(w)
+        Q[w, win.states.astype(np.int64)] = win.probs
+    return Q
+
+
+def test_window_probs_device_equal_result_probs(g30):
+    """The stream-ordered device probabilities (the autograd forward) equal the
+    host result's probabilities bit for bit: model rates, device rates, and a
+    discrete-time model."""
+    bnd, cfg, prog = g30
+    m = prog.model.info
+    base = prog.model.param_values
+    for rates in (None, (base * np.float32(1.2)).astype(np.float32)):
+        print(f"[gpu] device probabilities grad_30 rates={'model' if rates is None else 'scaled'}", flush=True)
+        r = None if rates is None else torch.as_tensor(rates).cuda()
+        P = mgrad.window_probs_device(prog, r, n_trajectories=20000, seed=5).cpu().numpy()
+        res = prog.simulate(n_trajectories=20000, seed=5, rates=rates)
+        assert np.array_equal(P, _dense(res, m.windows, m.n_observed))
+    cc = mg.Model.load(*model_paths("cellcycle")).compile()
+    ci = cc.model.info
+    assert ci.discrete_time == 1
+    P = mgrad.window_probs_device(cc, None, n_trajectories=5000).cpu().numpy()
+    assert np.array_equal(P, _dense(cc.simulate(n_trajectories=5000), ci.windows, ci.n_observed))
 
 
 def test_gradient_form_refusals(g30):
