@@ -282,6 +282,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the batched, gradient, discrete and scale blocks")
     ap.add_argument("--no-sorted-output", action="store_true", help="skip the sorted-pairs step")
+    ap.add_argument("--only", choices=["discrete", "scale_1000", "batched", "grad"], default=None,
+                    help="run only this form's block and print it (for profiling its kernels)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one plain device copy of this many MiB (calibrates HBM byte counters)")
     a = ap.parse_args()
@@ -308,6 +310,14 @@ def main():
 
     torch.cuda.set_device(0)
     wdir = tempfile.mkdtemp(prefix="mbg_bench_")
+    if a.only:
+        steps = max(1, min(a.steps, 3))
+        block = {"discrete": lambda: bench_discrete(mg, netgen, wdir, steps),
+                 "scale_1000": lambda: bench_scale(mg, netgen, wdir, steps, not a.no_cpu_baseline, a.cpu_seconds),
+                 "batched": lambda: bench_batched(mg, netgen, wdir, steps),
+                 "grad": lambda: bench_grad(mg, netgen, wdir, steps)}[a.only]()
+        print(json.dumps({"only": a.only, a.only: block}), flush=True)
+        return
     bnd, cfg = netgen.write_workload(a.workload, wdir)
     model = mg.Model.load(bnd, cfg)
     info = model.info

@@ -80,10 +80,11 @@ extern "C" __global__ void __launch_bounds__(256) mbg_expand_tails(mbg_sim_args 
 
 // The occupied slots {key, value} of a hash table (window cells, final
 // states) as (key, value) pairs, so that a sort hands the host the cells in
-// key order.  Each block takes 16 x 256 consecutive slots, thread t every
-// 256th from its own (coalesced across the block): threads count their
-// occupied slots, a block scan orders them, and one atomic per block reserves
-// the output range.  Slots with a zero value are skipped.
+// key order.  A block takes 16 x 256 consecutive slots, thread t every 256th
+// from its own (coalesced across the block), reads each slot once into
+// registers, and the block's threads order their occupied slots with a wave
+// scan and the waves' totals; one atomic per block reserves the output
+// range.  Slots with a zero value are skipped.
 constexpr int k_compact_per_thread = 16;
 extern "C" __global__ void __launch_bounds__(256) mbg_table_compact(const unsigned long long* table,
 																	unsigned long long cap, unsigned long long* keys,
@@ -91,47 +92,56 @@ extern "C" __global__ void __launch_bounds__(256) mbg_table_compact(const unsign
 																	unsigned long long out_cap,
 																	unsigned long long* count)
 {
-	__shared__ unsigned int part[256];
+	__shared__ unsigned int wtot[4];
 	__shared__ unsigned long long base;
+	const ulonglong2* slots = (const ulonglong2*)table;
 	const uint64_t first = (uint64_t)blockIdx.x * blockDim.x * k_compact_per_thread + threadIdx.x;
+	ulonglong2 e[k_compact_per_thread];
 	unsigned int mine = 0;
+#pragma unroll
 	for (int j = 0; j < k_compact_per_thread; ++j)
 	{
 		const uint64_t i = first + (uint64_t)j * blockDim.x;
-		if (i < cap && table[2 * i] != MBG_EMPTY_KEY && table[2 * i + 1] != 0ull)
-			mine++;
+		e[j] = i < cap ? slots[i] : make_ulonglong2(MBG_EMPTY_KEY, 0ull);
+		mine += (e[j].x != MBG_EMPTY_KEY && e[j].y != 0ull) ? 1u : 0u;
 	}
-	// inclusive scan of the per-thread counts over the block
-	part[threadIdx.x] = mine;
-	__syncthreads();
-	for (unsigned int off = 1; off < blockDim.x; off <<= 1)
+	// inclusive scan over the wave, then the waves' totals in order
+	const int lane = __lane_id(), wave = threadIdx.x >> 6;
+	unsigned int incl = mine;
+#pragma unroll
+	for (int off = 1; off < 64; off <<= 1)
 	{
-		const unsigned int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-		__syncthreads();
-		part[threadIdx.x] += v;
-		__syncthreads();
+		const unsigned int v = __shfl_up(incl, off);
+		if (lane >= off)
+			incl += v;
 	}
-	if (threadIdx.x == blockDim.x - 1)
-		base = atomicAdd(count, (unsigned long long)part[threadIdx.x]);
+	if (lane == 63)
+		wtot[wave] = incl;
 	__syncthreads();
-	uint64_t o = base + part[threadIdx.x] - mine;
-	for (int j = 0; j < k_compact_per_thread; ++j)
+	if (threadIdx.x == 0)
 	{
-		const uint64_t i = first + (uint64_t)j * blockDim.x;
-		if (i < cap)
+		unsigned int tot = 0;
+		for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
 		{
-			const unsigned long long key = table[2 * i], val = table[2 * i + 1];
-			if (key != MBG_EMPTY_KEY && val != 0ull)
-			{
-				if (o < out_cap)
-				{
-					keys[o] = key;
-					vals[o] = val;
-				}
-				o++;
-			}
+			const unsigned int t = wtot[w];
+			wtot[w] = tot;
+			tot += t;
 		}
+		base = tot ? atomicAdd(count, (unsigned long long)tot) : 0ull;
 	}
+	__syncthreads();
+	uint64_t o = base + wtot[wave] + incl - mine;
+#pragma unroll
+	for (int j = 0; j < k_compact_per_thread; ++j)
+		if (e[j].x != MBG_EMPTY_KEY && e[j].y != 0ull)
+		{
+			if (o < out_cap)
+			{
+				keys[o] = e[j].x;
+				vals[o] = e[j].y;
+			}
+			o++;
+		}
 }
 
 // The published slots of a fixed-point table (slots of `stride` words {tag,

@@ -94,6 +94,16 @@ struct grad_host_tables
 	size_t en_bytes = 0, ist_bytes = 0, jp_bytes = 0, bytes = 0;
 };
 
+// A buffer a run left cleared (its tables are cleared right after they have
+// been read, before the run's copy to the host starts, so the next run finds
+// them ready and does not clear them while that copy is in flight).
+struct cleared_buf
+{
+	const void* ptr = nullptr;
+	size_t bytes = 0;
+	bool holds(const void* p, size_t b) const { return p && ptr == p && bytes >= b; }
+};
+
 // Device storage a run's statistics leave the device from.  A program keeps
 // two, used in turn, so that one run's copy to the host (on the program's
 // copy stream) overlaps the next run's kernels; a slot is reused only after
@@ -107,6 +117,7 @@ struct out_slot
 	device_buffer fk, fv, fka, fva; // final states, compacted
 	device_buffer fpc;            // fixed points, compacted records {count, words}
 	device_buffer cnt;            // [4] counts of the compactions
+	cleared_buf cl_ent, cl_cnt;
 	hipEvent_t copied = nullptr;  // the last copy out of this slot
 	bool pending = false;         // copied has been recorded
 	~out_slot();
@@ -120,6 +131,9 @@ struct program
 	// run workspace (kept between runs so repeated launches do not reallocate)
 	device_buffer b_counters, b_win_vals, b_win_keys, b_fin_vals, b_fin_keys, b_fp, b_istate, b_pv, b_wave;
 	device_buffer b_sort_tmp;
+	cleared_buf cl_counters, cl_win_keys, cl_win_vals, cl_fin_keys, cl_fin_vals, cl_tail, cl_fp, cl_pent;
+	const void* istate_dev = nullptr; // b_istate holds the model's initial-state probabilities
+	const void* wave_dev = nullptr;   // b_wave holds the wave kernel's tables
 	uint64_t pairs_per_traj = 0; // learned from the last sorted run (capacity hint)
 	out_slot slots[2];
 	int slot_next = 0;
@@ -194,7 +208,6 @@ struct program
 	device_buffer b_pent;      // entropy sums of a device-probability run
 	// initial-state probabilities in pinned memory (their upload is stream ordered)
 	std::shared_ptr<char> istate_pinned;
-	std::vector<char> istate_host;
 	// Kernel for the wave path at the model's nodes-per-lane (loads on first use).
 	hipFunction_t wave_function();
 };

@@ -8,7 +8,8 @@
 // loop and needs no bitcode from this build, so any hipRTC / code-object
 // manager the process has loaded (the system ROCm's, or one a framework
 // bundles) can compile it.  Code objects are cached on disk, keyed by a hash
-// of the complete source and the compile options.
+// of the complete source, the compile options and the compiler and runtime
+// versions; a cached file that does not load is compiled again.
 #include <hip/hiprtc.h>
 
 
@@ -154,6 +155,49 @@ std::vector<char> compile_to_code_object(const std::string& source)
 	return co;
 }
 
+// Hash of a translation unit for the code-object cache: the full source, the
+// compile flags, the embedded headers, and the versions of hipRTC and of the
+// HIP runtime (a code object from another compiler is never reused).
+uint64_t unit_hash(const std::string& src, const std::vector<const char*>& headers)
+{
+	uint64_t h = fnv1a(src.data(), src.size());
+	for (const char* f : k_compile_flags)
+		h = fnv1a(f, std::strlen(f), h);
+	for (const char* hd : headers)
+		h = fnv1a(hd, std::strlen(hd), h);
+	int v[3] = { 0, 0, 0 };
+	(void)hiprtcVersion(&v[0], &v[1]);
+	(void)hipRuntimeGetVersion(&v[2]);
+	return fnv1a(v, sizeof(v), h);
+}
+
+// The module of a translation unit: loaded from the cache file `name` when it
+// is there (a file that does not load, e.g. truncated, is removed and the
+// unit compiled again), else compiled and stored.
+hipModule_t load_unit(const std::string& src, const std::string& name, bool no_cache, const char* what)
+{
+	const std::string dir = cache_dir();
+	const std::string path = dir + "/" + name;
+	std::vector<char> co;
+	hipModule_t mod = nullptr;
+	if (!no_cache && read_all(path, co))
+	{
+		if (hipModuleLoadData(&mod, co.data()) == hipSuccess)
+			return mod;
+		(void)hipGetLastError();
+		std::remove(path.c_str());
+		mod = nullptr;
+	}
+	co = compile_to_code_object(src);
+	if (!no_cache)
+	{
+		mkdirs(dir);
+		write_atomic(path, co.data(), co.size());
+	}
+	hip_check(hipModuleLoadData(&mod, co.data()), what);
+	return mod;
+}
+
 } // namespace
 
 program::~program()
@@ -199,27 +243,11 @@ hipFunction_t program::grad_function()
 	const int waves = gc.n_params <= 32 ? 2 : 1;
 	const std::string src = "#define MBG_GRAD_WAVES " + std::to_string(waves) + "\n" + gen.source + gc.source
 							+ "\n#include \"mbg_grad_kernel.h\"\n";
-	uint64_t h = fnv1a(src.data(), src.size());
-	for (const char* f : k_compile_flags)
-		h = fnv1a(f, std::strlen(f), h);
-	for (const char* hd : { mbg_hdr_numeric, mbg_hdr_abi, mbg_hdr_kernel_args, mbg_hdr_stats_device,
-							mbg_hdr_lane_core, mbg_hdr_grad_kernel })
-		h = fnv1a(hd, std::strlen(hd), h);
+	const uint64_t h = unit_hash(src, { mbg_hdr_numeric, mbg_hdr_abi, mbg_hdr_kernel_args, mbg_hdr_stats_device,
+										mbg_hdr_lane_core, mbg_hdr_grad_kernel });
 	char name[64];
 	std::snprintf(name, sizeof(name), "%016llx_g.co", (unsigned long long)h);
-	const std::string dir = cache_dir();
-	const std::string path = dir + "/" + name;
-	std::vector<char> co;
-	if (no_cache || !read_all(path, co))
-	{
-		co = compile_to_code_object(src);
-		if (!no_cache)
-		{
-			mkdirs(dir);
-			write_atomic(path, co.data(), co.size());
-		}
-	}
-	hip_check(hipModuleLoadData(&mod_grad, co.data()), "hipModuleLoadData(simulate_grad)");
+	mod_grad = load_unit(src, name, no_cache, "hipModuleLoadData(simulate_grad)");
 	hip_check(hipModuleGetFunction(&fn_grad, mod_grad, "simulate_grad"), "hipModuleGetFunction(simulate_grad)");
 	compile_seconds += wall_seconds() - t0;
 	return fn_grad;
@@ -298,28 +326,11 @@ hipFunction_t program::function(int flavour, int win_mode, int fin_mode)
 			src += "#define MBG_WIN_MODE " + std::to_string(win_mode) + "\n#define MBG_FIN_MODE "
 				   + std::to_string(fin_mode) + "\n";
 		src += gen.source + "\n" + mbg_src_lane;
-		uint64_t h = fnv1a(src.data(), src.size());
-		for (const char* f : k_compile_flags)
-			h = fnv1a(f, std::strlen(f), h);
-		for (const char* hd :
-			 { mbg_hdr_numeric, mbg_hdr_abi, mbg_hdr_kernel_args, mbg_hdr_stats_device, mbg_hdr_lane_core })
-			h = fnv1a(hd, std::strlen(hd), h);
+		const uint64_t h =
+			unit_hash(src, { mbg_hdr_numeric, mbg_hdr_abi, mbg_hdr_kernel_args, mbg_hdr_stats_device, mbg_hdr_lane_core });
 		char name[64];
 		std::snprintf(name, sizeof(name), "%016llx_%d.co", (unsigned long long)h, flavour);
-		const std::string dir = cache_dir();
-		const std::string path = dir + "/" + name;
-
-		std::vector<char> co;
-		if (no_cache || !read_all(path, co))
-		{
-			co = compile_to_code_object(src);
-			if (!no_cache)
-			{
-				mkdirs(dir);
-				write_atomic(path, co.data(), co.size());
-			}
-		}
-		hip_check(hipModuleLoadData(&mod, co.data()), "hipModuleLoadData");
+		mod = load_unit(src, name, no_cache, "hipModuleLoadData");
 		if (param)
 		{
 			hip_check(hipModuleGetFunction(&fn, mod, "simulate_batched"), "hipModuleGetFunction(simulate_batched)");

@@ -12,6 +12,7 @@
 // its own dwell.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "../device/mbg_kernel_args.h"
@@ -409,13 +410,27 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 	auto ev = std::make_shared<run_events>();
 	const size_t fp_stride = 2 + (size_t)W;
 
+	// Clearing: every table starts a run cleared.  A run clears what it used
+	// right after it has been read (compaction, or the counters' read), so a
+	// following run usually finds its tables ready; `use` clears a buffer the
+	// last run did not leave cleared and marks it in use.
+	auto use = [](cleared_buf& c, const void* ptr, size_t bytes, auto&& clear) {
+		if (!c.holds(ptr, bytes))
+			clear();
+		c.ptr = nullptr;
+	};
+	auto memset0 = [&](void* ptr, size_t bytes) {
+		hip_check(hipMemsetAsync(ptr, 0, bytes, stream), "hipMemsetAsync");
+	};
+
 	for (int attempt = 0;; attempt++)
 	{
 		// ---- storage
 		auto* counters = (unsigned long long*)p.b_counters.ensure(MBG_NCOUNTERS * 8);
-		hip_check(hipMemsetAsync(counters, 0, MBG_NCOUNTERS * 8, stream), "hipMemsetAsync");
+		use(p.cl_counters, counters, MBG_NCOUNTERS * 8, [&] { memset0(counters, MBG_NCOUNTERS * 8); });
+		cleared_buf& cl_ent = probs ? p.cl_pent : slot.cl_ent;
 		auto* ent = (unsigned long long*)(probs ? p.b_pent : slot.ent).ensure((size_t)windows * 8);
-		hip_check(hipMemsetAsync(ent, 0, (size_t)windows * 8, stream), "hipMemsetAsync");
+		use(cl_ent, ent, (size_t)windows * 8, [&] { memset0(ent, (size_t)windows * 8); });
 		unsigned long long *win_vals = nullptr, *win_keys = nullptr, *fin_vals = nullptr, *fin_keys = nullptr;
 		unsigned long long *pk = nullptr, *pv = nullptr;
 		if (pl.win_mode == MBG_HIST_SORT)
@@ -426,42 +441,76 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 		else if (pl.win_mode == MBG_HIST_HASH)
 		{
 			win_keys = (unsigned long long*)p.b_win_keys.ensure(pl.win_cap * 16);
-			hip_fill_slots(p, win_keys, pl.win_cap, stream);
+			use(p.cl_win_keys, win_keys, pl.win_cap * 16, [&] { hip_fill_slots(p, win_keys, pl.win_cap, stream); });
 		}
 		else
 		{
 			win_vals = (unsigned long long*)p.b_win_vals.ensure(pl.win_cells * 8);
-			hip_check(hipMemsetAsync(win_vals, 0, pl.win_cells * 8, stream), "hipMemsetAsync");
+			use(p.cl_win_vals, win_vals, pl.win_cells * 8, [&] { memset0(win_vals, pl.win_cells * 8); });
 		}
 		if (pl.fin_mode == MBG_HIST_HASH)
 		{
 			fin_keys = (unsigned long long*)p.b_fin_keys.ensure(pl.fin_cap * 16);
-			hip_fill_slots(p, fin_keys, pl.fin_cap, stream);
+			use(p.cl_fin_keys, fin_keys, pl.fin_cap * 16, [&] { hip_fill_slots(p, fin_keys, pl.fin_cap, stream); });
 		}
 		else
 		{
 			fin_vals = (unsigned long long*)p.b_fin_vals.ensure(pl.fin_cells * 8);
-			hip_check(hipMemsetAsync(fin_vals, 0, pl.fin_cells * 8, stream), "hipMemsetAsync");
+			use(p.cl_fin_vals, fin_vals, pl.fin_cells * 8, [&] { memset0(fin_vals, pl.fin_cells * 8); });
 		}
 		unsigned long long* tail_tab = nullptr;
 		if (tails)
 		{
 			tail_tab = (unsigned long long*)p.b_tail.ensure(tail_cap * 16);
-			hip_fill_slots(p, tail_tab, tail_cap, stream);
+			use(p.cl_tail, tail_tab, tail_cap * 16, [&] { hip_fill_slots(p, tail_tab, tail_cap, stream); });
 		}
 		auto* fp = (unsigned int*)p.b_fp.ensure(pl.fp_cap * fp_stride * 4);
-		hip_check(hipMemsetAsync(fp, 0, pl.fp_cap * fp_stride * 4, stream), "hipMemsetAsync");
+		use(p.cl_fp, fp, pl.fp_cap * fp_stride * 4, [&] { memset0(fp, pl.fp_cap * fp_stride * 4); });
+		// the run's tables cleared again once they have been read
+		auto clear_used = [&] {
+			memset0(counters, MBG_NCOUNTERS * 8);
+			p.cl_counters = cleared_buf { counters, MBG_NCOUNTERS * 8 };
+			if (win_keys)
+			{
+				hip_fill_slots(p, win_keys, pl.win_cap, stream);
+				p.cl_win_keys = cleared_buf { win_keys, pl.win_cap * 16 };
+			}
+			if (win_vals)
+			{
+				memset0(win_vals, pl.win_cells * 8);
+				p.cl_win_vals = cleared_buf { win_vals, pl.win_cells * 8 };
+			}
+			if (fin_keys)
+			{
+				hip_fill_slots(p, fin_keys, pl.fin_cap, stream);
+				p.cl_fin_keys = cleared_buf { fin_keys, pl.fin_cap * 16 };
+			}
+			if (fin_vals)
+			{
+				memset0(fin_vals, pl.fin_cells * 8);
+				p.cl_fin_vals = cleared_buf { fin_vals, pl.fin_cells * 8 };
+			}
+			if (tail_tab)
+			{
+				hip_fill_slots(p, tail_tab, tail_cap, stream);
+				p.cl_tail = cleared_buf { tail_tab, tail_cap * 16 };
+			}
+			memset0(fp, pl.fp_cap * fp_stride * 4);
+			p.cl_fp = cleared_buf { fp, pl.fp_cap * fp_stride * 4 };
+		};
 
+		// the model's initial-state probabilities, uploaded once per buffer
+		// (from pinned memory the program owns: the copy is stream ordered)
 		auto* d_istate = (float*)p.b_istate.ensure(istate.size() * 4);
-		if (p.istate_host.size() != istate.size() * 4)
+		if (p.istate_dev != d_istate)
 		{
-			// pinned, owned by the program: the copy may run after this call returns
 			p.istate_pinned = p.pool->get(istate.size() * 4);
 			std::memcpy(p.istate_pinned.get(), istate.data(), istate.size() * 4);
-			p.istate_host.assign((const char*)istate.data(), (const char*)istate.data() + istate.size() * 4);
+			hip_check(hipMemcpyAsync(d_istate, p.istate_pinned.get(), istate.size() * 4, hipMemcpyHostToDevice,
+									 stream),
+					  "hipMemcpyAsync");
+			p.istate_dev = d_istate;
 		}
-		hip_check(hipMemcpyAsync(d_istate, p.istate_pinned.get(), istate.size() * 4, hipMemcpyHostToDevice, stream),
-				  "hipMemcpyAsync");
 
 		// ---- kernel arguments
 		mbg_sim_args a;
@@ -553,8 +602,14 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 		{
 			const wave_tables& wt = *p.wtab;
 			auto* img = (char*)p.b_wave.ensure(wt.image.size());
-			hip_check(hipMemcpyAsync(img, wt.image.data(), wt.image.size(), hipMemcpyHostToDevice, stream),
-					  "hipMemcpyAsync");
+			if (p.wave_dev != img)
+			{
+				// the tables are the program's (they outlive the stream-ordered copy)
+				hip_check(hipMemcpyAsync(img, wt.image.data(), wt.image.size(), hipMemcpyHostToDevice, stream),
+						  "hipMemcpyAsync");
+				hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+				p.wave_dev = img;
+			}
 			a.wt.nodes = (const mbg_wnode*)img;
 			a.wt.child_off = (const unsigned*)(img + wt.nodes.size() * sizeof(mbg_wnode));
 			a.wt.child = (const unsigned short*)(a.wt.child_off + wt.child_off.size());
@@ -606,6 +661,9 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 			const unsigned g = (unsigned)std::min<uint64_t>(4096, std::max<uint64_t>(1, (pl.win_cells + 255) / 256));
 			hip_check(hipModuleLaunchKernel(p.fn_cells_to_probs, g, 1, 1, 256, 1, 1, 0, stream, cargs, nullptr),
 					  "hipModuleLaunchKernel(mbg_cells_to_probs)");
+			clear_used();
+			memset0(ent, (size_t)windows * 8);
+			p.cl_pent = cleared_buf { ent, (size_t)windows * 8 };
 			p.mark_run_done(stream);
 			return nullptr;
 		}
@@ -614,7 +672,7 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 		auto* cnt = (unsigned long long*)slot.cnt.ensure(4 * 8);
 		if (!sp.device_only)
 		{
-			hip_check(hipMemsetAsync(cnt, 0, 4 * 8, stream), "hipMemsetAsync");
+			use(slot.cl_cnt, cnt, 4 * 8, [&] { memset0(cnt, 4 * 8); });
 			if (pl.win_mode == MBG_HIST_HASH)
 			{
 				auto* tk = (unsigned long long*)slot.pk.ensure(pl.win_cap * 8);
@@ -642,11 +700,24 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 			auto* fpc = (unsigned int*)slot.fpc.ensure(pl.fp_cap * (fp_stride - 1) * 4);
 			fp_compact(p, fp, pl.fp_cap, (int)fp_stride, fpc, pl.fp_cap, cnt + 2, stream);
 		}
-		else
-			hip_check(hipEventRecord(ev->e1, stream), "hipEventRecord");
 		hip_check(hipMemcpyAsync(mb, counters, MBG_NCOUNTERS * 8, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
 		if (!sp.device_only)
+		{
 			hip_check(hipMemcpyAsync(mb + 16, cnt, 4 * 8, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+			// the sorted mode's unique count (cnt[3]) is written and read after the sort
+			if (pl.win_mode != MBG_HIST_SORT)
+			{
+				memset0(cnt, 4 * 8);
+				slot.cl_cnt = cleared_buf { cnt, 4 * 8 };
+			}
+		}
+		clear_used();
+		if (sp.device_only)
+		{
+			memset0(ent, (size_t)windows * 8);
+			cl_ent = cleared_buf { ent, (size_t)windows * 8 };
+			hip_check(hipEventRecord(ev->e1, stream), "hipEventRecord");
+		}
 		hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 		unsigned long long ctr[MBG_NCOUNTERS];
 		std::memcpy(ctr, mb, sizeof(ctr));
@@ -767,6 +838,9 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 		d2h(o_fk, fk, n_fin * 8);
 		d2h(o_fv, fv, n_fin * 8);
 		d2h(o_fp, slot.fpc.ptr, n_fp * (fp_stride - 1) * 4);
+		// the slot's entropy sums cleared for its next run, after their copy
+		hip_check(hipMemsetAsync(ent, 0, (size_t)windows * 8, cs), "hipMemsetAsync");
+		slot.cl_ent = cleared_buf { ent, (size_t)windows * 8 };
 		// the slot keeps an event of its own (the result's goes with the result)
 		if (!slot.copied)
 			hip_check(hipEventCreateWithFlags(&slot.copied, hipEventDisableTiming), "hipEventCreateWithFlags");

@@ -65,7 +65,6 @@
 #include <sstream>
 #include <string>
 #include <unistd.h>
-#include <unordered_map>
 #include <vector>
 
 #ifdef _OPENMP
@@ -963,14 +962,61 @@ struct Sim
 	}
 };
 
+// Open-addressing map from 64-bit cell keys to counts (linear probing,
+// Fibonacci hashing, grown at half load); the key ~0 marks an empty slot (no
+// cell key reaches it: window and observed bits fit in 63 bits).
+struct CellMap
+{
+	std::vector<uint64_t> keys, vals;
+	size_t used = 0;
+	int shift = 64;
+
+	static constexpr uint64_t k_empty = ~0ull;
+	void rehash(size_t cap)
+	{
+		std::vector<uint64_t> ok, ov;
+		ok.swap(keys);
+		ov.swap(vals);
+		keys.assign(cap, k_empty);
+		vals.assign(cap, 0);
+		shift = 64 - __builtin_ctzll(cap);
+		used = 0;
+		for (size_t i = 0; i < ok.size(); i++)
+			if (ok[i] != k_empty)
+				add(ok[i], ov[i]);
+	}
+	void add(uint64_t key, uint64_t q)
+	{
+		if (2 * (used + 1) > keys.size())
+			rehash(keys.empty() ? 1024 : 2 * keys.size());
+		const size_t mask = keys.size() - 1;
+		size_t i = (size_t)((key * 0x9E3779B97F4A7C15ull) >> shift);
+		while (keys[i] != k_empty && keys[i] != key)
+			i = (i + 1) & mask;
+		if (keys[i] == k_empty)
+		{
+			keys[i] = key;
+			used++;
+		}
+		vals[i] += q;
+	}
+	template <typename F>
+	void each(F&& f) const
+	{
+		for (size_t i = 0; i < keys.size(); i++)
+			if (keys[i] != k_empty)
+				f(keys[i], vals[i]);
+	}
+};
+
 struct Stats
 {
 	int nobs = 0, windows = 0;
 	bool dense = false;
 	std::vector<uint64_t> dwin;                          // dense (w << nobs) | obs
-	std::unordered_map<uint64_t, uint64_t> hwin;         // sparse
+	CellMap hwin;                                        // sparse
 	std::vector<uint64_t> ent;
-	std::unordered_map<uint64_t, uint64_t> fin;
+	CellMap fin;
 	std::map<std::vector<uint32_t>, uint64_t> fixedp;   // key: words reversed (MSW first)
 	uint64_t transitions = 0, finished = 0, truncated = 0;
 
@@ -989,7 +1035,7 @@ struct Stats
 		if (dense)
 			dwin[key] += q;
 		else
-			hwin[key] += q;
+			hwin.add(key, q);
 	}
 	void merge(const Stats& o)
 	{
@@ -997,12 +1043,10 @@ struct Stats
 			for (size_t i = 0; i < dwin.size(); i++)
 				dwin[i] += o.dwin[i];
 		else
-			for (auto& kv : o.hwin)
-				hwin[kv.first] += kv.second;
+			o.hwin.each([&](uint64_t k, uint64_t v) { hwin.add(k, v); });
 		for (int w = 0; w < windows; w++)
 			ent[w] += o.ent[w];
-		for (auto& kv : o.fin)
-			fin[kv.first] += kv.second;
+		o.fin.each([&](uint64_t k, uint64_t v) { fin.add(k, v); });
 		for (auto& kv : o.fixedp)
 			fixedp[kv.first] += kv.second;
 		transitions += o.transitions;
@@ -1099,7 +1143,7 @@ void trajectory(const Sim& S, uint64_t id, Stats& st, std::vector<float>& r, std
 		if (t >= S.max_time)
 		{
 			st.finished++;
-			st.fin[o]++;
+			st.fin.add(o, 1);
 			if (fixed)
 			{
 				std::vector<uint32_t> key(s.rbegin(), s.rend());
@@ -1570,13 +1614,15 @@ int main(int argc, char** argv)
 	}
 	else
 	{
-		for (auto& kv : tot.hwin)
-			if (kv.second)
-				win[kv.first >> S.nobs].emplace_back(kv.first & omask, kv.second);
+		tot.hwin.each([&](uint64_t k, uint64_t v) {
+			if (v)
+				win[k >> S.nobs].emplace_back(k & omask, v);
+		});
 		for (auto& v : win)
 			std::sort(v.begin(), v.end());
 	}
-	std::vector<std::pair<uint64_t, uint64_t>> fin(tot.fin.begin(), tot.fin.end());
+	std::vector<std::pair<uint64_t, uint64_t>> fin;
+	tot.fin.each([&](uint64_t k, uint64_t v) { fin.emplace_back(k, v); });
 	std::sort(fin.begin(), fin.end());
 
 	if (timing)

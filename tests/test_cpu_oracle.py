@@ -236,3 +236,15 @@ def test_oracle_gradient_decay_closed_form(tmp_path):
     assert names == ["$d"]
     assert abs(mean[0] - exact) < 4.5 * se[0], (mean[0], exact, se[0])
     assert se[0] < 0.02 * abs(exact)
+
+
+@pytest.mark.parametrize("order", ["seq", "wave"])
+def test_compiled_formulas_equal_interpreter(tmp_path, order):
+    """--compiled (formulas emitted as C++ and built with g++, the CPU
+    baseline's mode) gives the interpreter's exact dump."""
+    from maboss.gpu_amd import netgen
+    for w in ("plain_100", "plain_small"):
+        bnd, cfg = netgen.write_workload(w, str(tmp_path))
+        run_oracle(bnd, cfg, "--n", 400, "--order", order, "--raw", tmp_path / "i.raw")
+        run_oracle(bnd, cfg, "--n", 400, "--order", order, "--compiled", "--raw", tmp_path / "c.raw")
+        assert open(tmp_path / "i.raw").read() == open(tmp_path / "c.raw").read(), w

@@ -363,3 +363,73 @@ def test_runtime_rates_bit_exact(tmp_path, wl):
     assert_same(g, oracle_raw(tmp_path, bnd, str(cfg2), n=4000))
     with pytest.raises(ValueError):
         prog.simulate(n_trajectories=10, rates=rates[:3])
+
+
+def test_scale_1000_bit_exact_2000(tmp_path, wl):
+    """The 1000-node workload on the wave kernel, 2000 trajectories, against
+    the oracle in the wave kernel's summation order (compiled formulas)."""
+    bnd, cfg = wl["scale_1000"]
+    g, r = gpu_raw(tmp_path, bnd, cfg, n_trajectories=2000)
+    assert r.info.kernel == 1 and r.info.finished == 2000
+    assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=2000, extra=("--order", "wave", "--compiled")))
+
+
+def _cells(res):
+    """All window cells of a result as (window << n_obs | state) keys and raw sums."""
+    n_obs = res.info.n_observed
+    keys, vals = [], []
+    for w in range(res.info.windows):
+        win = res.window(w)
+        keys.append((np.uint64(w) << np.uint64(n_obs)) | win.states)
+        vals.append(win.raw)
+    return np.concatenate(keys), np.concatenate(vals)
+
+
+def test_full_size_scale_1000_properties(tmp_path, wl):
+    """The 1000-node workload at its full size (1e6 trajectories): every
+    trajectory finishes, the final states count them all, each window's cells
+    cover its float32 extent for every trajectory, and two id-range shards add
+    up to the full run cell for cell."""
+    bnd, cfg = wl["scale_1000"]
+    prog = program(bnd, cfg)
+    n = 1000000
+    print("[gpu] scale_1000 full size", flush=True)
+    full = prog.simulate(n_trajectories=n)
+    info = full.info
+    assert info.kernel == 1 and info.finished == n and info.truncated == 0
+    assert int(full.final_states()[1].sum()) == n
+    m = prog.model.info
+    tick, mt = np.float32(m.time_tick), np.float32(m.max_time)
+    fk, fv = _cells(full)
+    for w in (0, 1, 37, m.windows - 1):
+        tot = sum(int(v) for v in full.window(w).raw)
+        lo = np.float32(w) * tick
+        hi = min(np.float32(w + 1) * tick, mt)
+        assert abs(tot / (n * TWO32) - float(hi - lo) / float(tick)) < 2e-6
+    print("[gpu] scale_1000 two shards", flush=True)
+    half = n // 2
+    a = prog.simulate(n_trajectories=half)
+    b = prog.simulate(n_trajectories=n - half, first_id=half)
+    ak, av = _cells(a)
+    bk, bv = _cells(b)
+    keys, inv = np.unique(np.concatenate([ak, bk]), return_inverse=True)
+    sums = np.zeros(len(keys), dtype=np.uint64)
+    np.add.at(sums, inv, np.concatenate([av, bv]))
+    assert np.array_equal(keys, fk) and np.array_equal(sums, fv)
+    assert a.info.transitions + b.info.transitions == info.transitions
+
+
+def test_lds_budget_edge(tmp_path):
+    """2048 windows of one observed node: the LDS entropy sums (16 KB) and a
+    window histogram in LDS (32 KB) would fill the 48 KB budget, leaving no
+    room for the per-wave tables; the plan puts the cells in HBM instead, and
+    forcing them into LDS is a clear error."""
+    b, c = tmp_path / "edge.bnd", tmp_path / "edge.cfg"
+    b.write_text("node A { logic = !A; rate_up = @logic ? 3 : 0; rate_down = @logic ? 0 : 2; }\n")
+    c.write_text("max_time = 204.8; time_tick = 0.1; sample_count = 500; seed_pseudorandom = 3;\n")
+    g, r = gpu_raw(tmp_path, str(b), str(c), n_trajectories=500)
+    assert r.info.windows == 2048 and r.info.window_mode == 0
+    assert_same(g, oracle_raw(tmp_path, str(b), str(c), n=500))
+    with pytest.raises(mg.MbgError) as e:
+        program(str(b), str(c)).simulate(n_trajectories=10, window_mode="lds")
+    assert e.value.code == 5
