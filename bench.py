@@ -18,8 +18,10 @@ trajectories, max_time 100, time_tick 1, continuous time.
 Prints one JSON line.  Besides ``value`` (the host-inclusive rate above):
 ``device_only`` (the same steps with the statistics left in HBM),
 ``latency`` (one synchronous step), ``roofline`` (the simulate kernel against
-the VALU issue rate with the per-transition op count 2*literals + 4N +
-5*n_obs + 40, which binds; ``hbm_effective`` inside it against HBM with the
+the VALU peak, MI355X's published FP32 vector rate as 78.6 T lane-ops/s,
+with the per-transition op count 2*literals + 4N + 5*n_obs + 40, which
+binds; ``issue_probe`` inside it with this GPU's measured single-instruction
+rates; ``hbm_effective`` inside it against HBM with the
 reference's record traffic of 8W+16 bytes per transition), ``scale_1000``
 (the 1000-node workload on the wave kernel, host-inclusive, with its own CPU
 baseline), ``discrete``, ``sorted_output``, ``batched`` (256 variants in one
@@ -43,7 +45,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz = 39.3 T lane-ops/s
+# MI355X's published FP32 vector rate, 157.3 TFLOP/s, as lane-ops (an FMA is 2 flops): 256 CUs x 128
+# lane-ops per clock x 2.4 GHz = 78.6 T lane-ops/s.  On gfx950 the 2-operand 32-bit integer ops and
+# v_fma_f32 issue at that rate (one wave64 instruction per SIMD every 2 cycles; measured 5.7-6.2e13
+# lane-ops/s by lib/mbg_valu_issue at the clock the part holds under that load), v_bfe_u32,
+# v_and_or_b32 and v_mul_hi_u32 at half of it.
+VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
+VALU_PROBE = os.path.join(ROOT, "maboss", "gpu_amd", "lib", "mbg_valu_issue")
 METRIC = "state transitions/sec on 1 MI355X (plain vs batched vs +grad) at matched stats"
 WORKLOAD_TEXT = {
     "plain_100": "100-node random network (seed 1, 4 state words, in-degree 1-4, AND/OR/NOT), 50 observed nodes "
@@ -187,13 +195,36 @@ def bench_scale(mg, netgen, wdir, steps, cpu, cpu_seconds):
            "steps": steps, "ms_per_step": wall / steps * 1e3, "kernel": f"simulate_wave_n{_rows(info.n_nodes)}",
            "kernel_ms": kernel_s * 1e3, "kernel_transitions_per_s": per / kernel_s,
            "transitions_per_step": per, "window_cells": li.n_window_cells,
-           "hbm_effective_frac": (8 * info.n_words + 16) * per / kernel_s / 1e9 / HBM_PEAK_GBS}
+           "hbm_effective_frac": (8 * info.n_words + 16) * per / kernel_s / 1e9 / HBM_PEAK_GBS,
+           "ops_per_transition": valu_ops(info), "valu_frac": valu_frac(info, per / kernel_s)}
     if cpu:
         cb = cpu_baseline(bnd, cfg, cpu_seconds, cpu_threads(), min_ids=20000)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = rate / cb["value"]
         out["target_100x_met"] = rate / cb["value"] >= 100.0
     return out
+
+
+def valu_ops(info):
+    """Per-transition VALU lane-ops of the path: 2 per literal (logic), 4 per node (rate select, enable
+    bit, running sum), 5 per observed node (index, entropy) and 40 for the step's random numbers, time
+    and flip."""
+    return 2 * info.total_literals + 4 * info.n_nodes + 5 * info.n_observed + 40
+
+
+def valu_frac(info, transitions_per_s):
+    return valu_ops(info) * transitions_per_s / (VALU_PEAK_TOPS * 1e12)
+
+
+def valu_issue_probe():
+    """Lane-instruction rates of single VALU instructions on this GPU (lib/mbg_valu_issue), or None."""
+    if not os.path.exists(VALU_PROBE):
+        return None
+    p = subprocess.run([VALU_PROBE], capture_output=True, text=True, timeout=120)
+    if p.returncode != 0:
+        return None
+    rows = [json.loads(line) for line in p.stdout.splitlines() if line.startswith("{")]
+    return {f"{r['op']}@{r['waves_per_simd']}w": r["lane_instr_per_s"] for r in rows if "op" in r}
 
 
 def _rows(n_nodes):
@@ -222,7 +253,8 @@ def bench_batched(mg, netgen, wdir, steps):
         trans += info.transitions
     one = {"transitions_per_s": trans / dev, "device_ms": dev / steps * 1e3, "wall_ms": wall / steps * 1e3,
            "block": info.block_size, "blocks_per_cu": info.blocks_per_cu, "traj_per_lane": info.traj_per_lane,
-           "graph": bool(info.used_graph)}
+           "graph": bool(info.used_graph), "ops_per_transition": valu_ops(model.info),
+           "valu_frac": valu_frac(model.info, trans / dev)}
     t0 = time.perf_counter()
     _, hinfo = prog.simulate_batched(variants)
     one["host_inclusive_transitions_per_s"] = hinfo.transitions / (time.perf_counter() - t0)
@@ -268,7 +300,9 @@ def bench_grad(mg, netgen, wdir, steps):
     return {"workload": "grad_30: 30 nodes, 60 rate parameters, 5 observed, 1e6 trajectories, max_time 20",
             "forward_transitions_per_s": ft / fwd, "backward_transitions_per_s": bt / bwd,
             "forward_ms": fwd / steps * 1e3, "backward_ms": bwd / steps * 1e3,
-            "backward_over_forward_throughput": (bt / bwd) / (ft / fwd), "accumulator": mgrad.ACCUMULATOR}
+            "backward_over_forward_throughput": (bt / bwd) / (ft / fwd), "accumulator": mgrad.ACCUMULATOR,
+            "ops_per_transition": valu_ops(info), "forward_valu_frac": valu_frac(info, ft / fwd),
+            "backward_valu_frac_at_forward_ops": valu_frac(info, bt / bwd)}
 
 
 def main():
@@ -349,7 +383,7 @@ def main():
 
     W, N, nobs = info.n_words, info.n_nodes, info.n_observed
     bytes_per = 8 * W + 16
-    ops_per = 2 * info.total_literals + 4 * N + 5 * nobs + 40
+    ops_per = valu_ops(info)
     kname = f"simulate_wave_n{_rows(N)}" if li.kernel == 1 else "simulate"
     rate_k = per_step / kernel_s
     bound_rate = min(HBM_PEAK_GBS * 1e9 / bytes_per, VALU_PEAK_TOPS * 1e12 / ops_per)
@@ -390,6 +424,8 @@ def main():
                      "kernel": kname, "kernel_ms": kernel_s * 1e3, "transitions_per_launch": per_step,
                      "ops_per_transition": ops_per, "achieved_transitions_per_s": rate_k,
                      "bound_transitions_per_s": bound_rate,
+                     "peak_note": "published FP32 vector rate (157.3 TFLOP/s) as lane-ops; issue_probe holds "
+                                  "measured single-instruction rates on this GPU",
                      "hbm_effective": {"achieved": hbm_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                        "frac": hbm_ach / HBM_PEAK_GBS, "bytes_per_transition": bytes_per,
                                        "note": "effective: the fused kernel writes no records; algorithmic "
@@ -417,6 +453,7 @@ def main():
         cb1 = cpu_baseline(b1, c1, 0.0, cpu_threads(), min_ids=10000)
         cb1["sample"] = "config 1 in full: " + cb1["sample"]
         out["cpu_config1"] = cb1
+    out["roofline"]["issue_probe"] = valu_issue_probe()
     print(json.dumps(out), flush=True)
 
 

@@ -19,13 +19,33 @@ struct mbg_eval_t
 	int pad;
 };
 
+// Tables a launch only reads (rates, masks, variant descriptors) are read
+// through the constant address space on the device: a load from a
+// wave-uniform address there is a scalar load whose value lands in an SGPR,
+// where a plain global pointer gives a vector load per lane (the compiler
+// cannot prove the table unchanged across the kernel's atomics).  The
+// pointers have the same 64-bit layout on the host.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MBG_CONST __attribute__((address_space(4)))
+#else
+#define MBG_CONST
+#endif
+
+// Address space of the rate tables a view points to: constant (scalar
+// loads) unless the unit defines MBG_RATES_VECTOR (vector loads).
+#if defined(MBG_RATES_VECTOR)
+#define MBG_RATE_AS
+#else
+#define MBG_RATE_AS MBG_CONST
+#endif
+
 // Per-variant runtime tables for the batched and differentiated forms.
 struct mbg_param_view
 {
-	const float* rate;      // [n_params] value of each $variable used by a rate
-	const float* rate_lr;   // [n_params] rate * log2(rate), precomputed
-	const uint32_t* up_en;  // [W] bit i clear: node i can never switch on
-	const uint32_t* dn_en;  // [W] bit i clear: node i can never switch off
+	const MBG_RATE_AS float* rate;      // [n_params] value of each $variable used by a rate
+	const MBG_RATE_AS float* rate_lr;   // [n_params] rate * log2(rate), precomputed
+	const MBG_RATE_AS uint32_t* up_en;  // [W] bit i clear: node i can never switch on
+	const MBG_RATE_AS uint32_t* dn_en;  // [W] bit i clear: node i can never switch off
 };
 
 // Hides a value from the optimiser (no instruction is emitted), so that

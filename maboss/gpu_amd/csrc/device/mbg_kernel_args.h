@@ -120,7 +120,7 @@ struct mbg_batch_variant
 	int windows;
 	float max_time;
 	unsigned int ctr_off; // first of its 3 counters in vctr
-	const float* istate;  // [n_nodes]
+	const MBG_CONST float* istate;  // [n_nodes]
 	mbg_param_view pv;
 };
 

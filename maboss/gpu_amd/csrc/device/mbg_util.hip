@@ -176,10 +176,10 @@ extern "C" __global__ void __launch_bounds__(256) mbg_rate_view(mbg_param_view* 
 	if (i == 0)
 	{
 		mbg_param_view v;
-		v.rate = rate;
-		v.rate_lr = rate_lr;
-		v.up_en = en;
-		v.dn_en = en + n_words;
+		v.rate = (const MBG_RATE_AS float*)rate;
+		v.rate_lr = (const MBG_RATE_AS float*)rate_lr;
+		v.up_en = (const MBG_RATE_AS uint32_t*)en;
+		v.dn_en = (const MBG_RATE_AS uint32_t*)(en + n_words);
 		*view = v;
 	}
 	if (i < n_params)

@@ -91,7 +91,7 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long* d_trans, u
 #endif
 
 #if MBG_PARAM == 2
-#define MBG_EVAL_STEP(s, u2, e) mbg_eval_p(s, u2, a.pv, e, nullptr)
+#define MBG_EVAL_STEP(s, u2, e) mbg_eval_p(s, u2, &pv, e, nullptr)
 #else
 #define MBG_EVAL_STEP(s, u2, e) mbg_eval(s, u2, e)
 #endif
@@ -109,7 +109,9 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 #if MBG_PARAM == 0
 	mbg_eval_setup();
 #else
-	mbg_eval_p_setup(a.pv);
+	// the view's pointers in SGPRs (scalar load of the read-only view)
+	const mbg_param_view pv = *(const MBG_CONST mbg_param_view*)a.pv;
+	mbg_eval_p_setup(&pv);
 #endif
 	__syncthreads();
 
@@ -303,9 +305,9 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 		c = __builtin_amdgcn_readfirstlane(c);
 		if (c >= a.n_chunks)
 			break;
-		const mbg_batch_chunk ch = a.chunks[c];
+		const mbg_batch_chunk ch = ((const MBG_CONST mbg_batch_chunk*)a.chunks)[c];
 		const uint32_t v = __builtin_amdgcn_readfirstlane(ch.variant);
-		const mbg_batch_variant& vd = a.bv[v];
+		const MBG_CONST mbg_batch_variant& vd = ((const MBG_CONST mbg_batch_variant*)a.bv)[v];
 		const float max_time = vd.max_time;
 		const int windows = vd.windows;
 		const unsigned long long win_off = vd.win_off, fin_off = vd.fin_off;

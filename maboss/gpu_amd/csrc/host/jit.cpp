@@ -60,6 +60,17 @@ const char* const k_compile_flags[] = { "--offload-arch=gfx950",
 										"-Xclang",
 										"-packed-fp32-ops" };
 
+// Run-time rate tables are read with scalar loads (constant address space)
+// above 40 nodes and with vector loads up to 40.  Measured on MI355X: scalar
+// loads make the batched 50-node launch 17 % faster and the 100-node
+// run-time-rates simulate 24 % faster; on a 30-node model with 60 rate
+// parameters they make the simulate 17 % slower (the rates hoisted into SGPRs
+// spill to VGPR lanes) and the gradient kernel 2 % slower.
+std::string rates_define(int n_nodes)
+{
+	return n_nodes <= 40 ? "#define MBG_RATES_VECTOR 1\n" : "";
+}
+
 uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull)
 {
 	const unsigned char* p = (const unsigned char*)data;
@@ -241,7 +252,8 @@ hipFunction_t program::grad_function()
 	// parameter; 2 waves per SIMD (256 registers) leave room for the rate
 	// code up to 32 parameters, 1 wave (512 registers) above.
 	const int waves = gc.n_params <= 32 ? 2 : 1;
-	const std::string src = "#define MBG_GRAD_WAVES " + std::to_string(waves) + "\n" + gen.source + gc.source
+	const std::string src = "#define MBG_GRAD_WAVES " + std::to_string(waves) + "\n" + rates_define(mdl->n_nodes())
+							+ gen.source + gc.source
 							+ "\n#include \"mbg_grad_kernel.h\"\n";
 	const uint64_t h = unit_hash(src, { mbg_hdr_numeric, mbg_hdr_abi, mbg_hdr_kernel_args, mbg_hdr_stats_device,
 										mbg_hdr_lane_core, mbg_hdr_grad_kernel });
@@ -322,6 +334,8 @@ hipFunction_t program::function(int flavour, int win_mode, int fin_mode)
 	{
 		std::string src = "#define MBG_KW " + std::to_string(W) + "\n#define MBG_PARAM " + std::to_string(flavour)
 						  + "\n#define MBG_LANE_WAVES " + std::to_string(waves) + "\n";
+		if (flavour != 0)
+			src += rates_define(n_nodes);
 		if (!param)
 			src += "#define MBG_WIN_MODE " + std::to_string(win_mode) + "\n#define MBG_FIN_MODE "
 				   + std::to_string(fin_mode) + "\n";
