@@ -454,6 +454,8 @@ public:
 					// src/parser/driver.cpp:65-77
 					if (a.text == "istate")
 						m_.nodes[idx].istate = eval(*e, a);
+					else if (a.text == "mutant")
+						m_.nodes[idx].mutant = mutant_value(*e, a);
 					else
 						m_.nodes[idx].attrs.emplace_back(a.text, std::move(e));
 				}
@@ -475,6 +477,30 @@ public:
 	}
 
 private:
+	// ON / OFF (any case) or a constant expression equal to 1 / 0
+	int mutant_value(const expr& e, const token& where)
+	{
+		if (e.kind == op::node)
+		{
+			std::string v = e.name;
+			for (char& ch : v)
+				ch = (char)std::toupper((unsigned char)ch);
+			if (v == "ON")
+				return 1;
+			if (v == "OFF")
+				return -1;
+		}
+		else
+		{
+			const float v = eval(e, where);
+			if (v == 1.0f)
+				return 1;
+			if (v == 0.0f)
+				return -1;
+		}
+		lex_.fail(where.line, where.col, "mutant must be ON or OFF (1 or 0)");
+	}
+
 	static bool literal_value(const expr& e, double& out)
 	{
 		if (e.kind == op::lit)
@@ -794,11 +820,36 @@ std::string read_file(const std::string& path)
 	return s;
 }
 
+namespace
+{
+// Mutant nodes of the .cfg: the blocked transition's rate becomes the
+// literal 0 and the initial state the forced value, whatever else the
+// .cfg said about the node.
+void apply_mutants(model& m)
+{
+	for (node_t& n : m.nodes)
+	{
+		if (!n.mutant)
+			continue;
+		const char* blocked = n.mutant > 0 ? "rate_down" : "rate_up";
+		for (auto& a : n.attrs)
+			if (a.first == blocked)
+			{
+				a.second = expr::make(op::lit);
+				a.second->value = 0.0f;
+				a.second->exact = 0.0;
+			}
+		n.istate = n.mutant > 0 ? 1.0f : 0.0f;
+	}
+}
+} // namespace
+
 std::unique_ptr<model> load_model_strings(const std::string& bnd, const std::string& cfg)
 {
 	auto m = std::make_unique<model>();
 	parse_bnd(*m, bnd, "<bnd>");
 	parse_cfg(*m, cfg, "<cfg>");
+	apply_mutants(*m);
 	validate(*m);
 	return m;
 }
@@ -808,6 +859,7 @@ std::unique_ptr<model> load_model_files(const std::string& bnd_path, const std::
 	auto m = std::make_unique<model>();
 	parse_bnd(*m, read_file(bnd_path), bnd_path);
 	parse_cfg(*m, read_file(cfg_path), cfg_path);
+	apply_mutants(*m);
 	validate(*m);
 	return m;
 }

@@ -85,6 +85,11 @@ struct node_t
 	std::string name;
 	std::vector<std::pair<std::string, expr_ptr>> attrs;
 	float istate = 0.5f; // src/parser/parse_types.cpp:3
+	// `X.mutant = ON;` / `X.mutant = OFF;` in the .cfg (no counterpart in the
+	// reference, whose README.md:33 lists mutants as a TODO): +1 forced on,
+	// -1 forced off, 0 free.  Applied after the .cfg is read: forced on sets
+	// rate_down to 0 and istate to 1, forced off rate_up to 0 and istate to 0.
+	int mutant = 0;
 
 	// First attribute with this name, like node_t::get_attr
 	// (src/parser/parse_types.cpp:5-12); nullptr when absent.

@@ -217,6 +217,8 @@ struct Model
 										  { "time_tick", 0.1f },
 										  { "seed_pseudorandom", 0.f } };
 	std::map<std::string, double> consts_exact { { "sample_count", 1e6 }, { "seed_pseudorandom", 0 } };
+	// `X.mutant = ON | OFF | 1 | 0;` of the .cfg, node index -> +1 on / -1 off
+	std::map<int, int> mutants;
 	int find(const std::string& n) const
 	{
 		for (size_t i = 0; i < nodes.size(); i++)
@@ -590,6 +592,19 @@ struct Parser
 						die("Unknown node " + id);
 					if (an == "istate")
 						m.nodes[i].istate = eval(e);
+					else if (an == "mutant")
+					{
+						std::string v = e->k == NODE ? e->name : "";
+						for (char& ch : v)
+							ch = (char)toupper((unsigned char)ch);
+						const float f = e->k == NODE ? -1.0f : eval(e);
+						if (v == "ON" || f == 1.0f)
+							m.mutants[i] = 1;
+						else if (v == "OFF" || f == 0.0f)
+							m.mutants[i] = -1;
+						else
+							die("mutant must be ON or OFF");
+					}
 					else
 						m.nodes[i].attrs.emplace_back(an, e);
 				}
@@ -1452,6 +1467,25 @@ int main(int argc, char** argv)
 		pb.bnd();
 		Parser pc(slurp(pos[1]), pos[1], m);
 		pc.cfg();
+	}
+	// A forced-on node never switches off and starts on; forced off, the
+	// reverse.  The blocked rate is replaced by the constant 0, so the rest
+	// of the oracle sees an ordinary model.
+	for (auto& [i, dir] : m.mutants)
+	{
+		auto zero = std::make_shared<Ex>();
+		zero->k = LIT;
+		Node& nd = m.nodes[i];
+		bool found = false;
+		for (auto& a : nd.attrs)
+			if (a.first == (dir > 0 ? "rate_down" : "rate_up"))
+			{
+				a.second = zero;
+				found = true;
+			}
+		if (!found)
+			die("mutant node without the rate it blocks");
+		nd.istate = dir > 0 ? 1.0f : 0.0f;
 	}
 
 	Sim S;
