@@ -130,13 +130,25 @@ typedef struct mbg_sim_params
 	int32_t kernel;          /* -1 auto (lane if N <= 256), 0 lane per trajectory, 1 wave per trajectory */
 	const float* rates;      /* [n_params] run-time values of the rate parameters (mbg_model_param_name
 	                            order), host memory; NULL: the model's.  Lane kernel only. */
-	int32_t reserved[4];
+	int32_t moment_states;   /* with MBG_SIM_ERRORS: observed states a lane holds per window in the
+	                            second-moment replay, 8 (0) or 1; more states in a window go to a
+	                            host-side list, so both give the same sums */
+	int32_t reserved[3];
 } mbg_sim_params;
 
 /* Keep the statistics in device memory and return only the counters and
  * timings (no device-to-host copy of the histograms); used to time the
  * device-side path. */
 #define MBG_SIM_DEVICE_ONLY 1
+/* Fill the ErrorTH / ErrorProba columns: after the run its trajectories are
+ * replayed to sum, exactly, the square of each trajectory's time in each
+ * window cell and of its entropy sum in each window; the columns are the
+ * standard errors of the means, sqrt((E[x^2] - E[x]^2) / (n - 1)).  No
+ * counterpart in the reference, which prints 0 in both columns
+ * (src/statistics/window_average_small.cpp:235, :244-245).  Lane kernel
+ * (up to 256 nodes) and windows x 2^n_obs <= 2^24; without the flag the
+ * columns are 0, as in the reference. */
+#define MBG_SIM_ERRORS 2
 
 /* Fill p with the defaults above. */
 void mbg_sim_params_default(mbg_sim_params* p);
@@ -298,6 +310,10 @@ typedef struct mbg_result_info
 	int32_t pad;
 	double simulate_ms;              /* device time of the simulate kernel alone */
 	uint64_t window_pairs;           /* sorted window mode: pair slots filled by the kernels */
+	uint64_t moment_spills;          /* MBG_SIM_ERRORS: partial times the second-moment replay
+	                                    summed on the host (windows with more states than a lane holds) */
+	int32_t has_errors;              /* 1: the error columns come from second moments */
+	int32_t pad2;
 } mbg_result_info;
 
 int mbg_result_get_info(const mbg_result* r, mbg_result_info* out);
@@ -311,6 +327,10 @@ int mbg_result_window(const mbg_result* r, int32_t w, uint64_t* states, uint64_t
 int mbg_result_window_probs(const mbg_result* r, int32_t w, uint64_t* states, double* probs);
 /* Per-window transition entropy TH and state entropy H. */
 int mbg_result_entropies(const mbg_result* r, double* th, double* h);
+/* Standard errors (runs with MBG_SIM_ERRORS; zeros otherwise): ErrorTH per
+ * window, and ErrorProba of window w's cells in mbg_result_window's order. */
+int mbg_result_error_th(const mbg_result* r, double* err_th);
+int mbg_result_window_errors(const mbg_result* r, int32_t w, double* err);
 /* Final observed states and counts, sorted by index. */
 int64_t mbg_result_final_size(const mbg_result* r);
 int mbg_result_final(const mbg_result* r, uint64_t* states, uint64_t* counts);

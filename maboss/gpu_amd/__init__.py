@@ -143,12 +143,15 @@ class Program:
                  window_capacity: int = 0, final_capacity: int = 0, fixed_capacity: int = 0,
                  block_size: int = 0, blocks_per_cu: int = 0, normalization: float = 0.0,
                  device_only: bool = False, kernel: str = "auto", rates=None, stream=None,
-                 wait: bool = True) -> "Result":
+                 wait: bool = True, errors: bool = False, moment_states: int = 0) -> "Result":
         """Simulate; ``rates`` (one float per ``Model.param_names``) replaces the model's
         parameter values at run time (lane kernel).  With ``wait=False`` the call returns
         once the run is enqueued (``mbg_simulate_async``): the statistics reach the host on
         the program's copy stream while the next run's kernels go ahead, and the result
-        completes in :meth:`Result.wait` (which every accessor calls)."""
+        completes in :meth:`Result.wait` (which every accessor calls).  ``errors`` fills the
+        ErrorTH / ErrorProba columns from exact per-trajectory second moments
+        (``MBG_SIM_ERRORS``: a replay of the trajectories after the run); ``moment_states``
+        (8 or 1) is the replay's per-lane state buffer, which changes no result."""
         p = SimParams()
         lib().mbg_sim_params_default(ctypes.byref(p))
         p.n_trajectories = n_trajectories
@@ -164,7 +167,8 @@ class Program:
         p.block_size = block_size
         p.blocks_per_cu = blocks_per_cu
         p.normalization = normalization
-        p.flags = 1 if device_only else 0
+        p.flags = (1 if device_only else 0) | (2 if errors else 0)
+        p.moment_states = moment_states
         p.kernel = {"auto": -1, "lane": 0, "wave": 1}[kernel]
         keep = None
         if rates is not None:
@@ -277,6 +281,21 @@ class Result:
         h = np.zeros(nw)
         check(lib().mbg_result_entropies(self._h, th.ctypes.data, h.ctypes.data))
         return th, h
+
+    def error_th(self) -> np.ndarray:
+        """ErrorTH per window (zeros unless the run had ``errors=True``)."""
+        out = np.zeros(self.info.windows)
+        check(lib().mbg_result_error_th(self._h, out.ctypes.data))
+        return out
+
+    def window_errors(self, w: int) -> np.ndarray:
+        """ErrorProba of window w's cells, in :meth:`window`'s order."""
+        n = lib().mbg_result_window_size(self._h, w)
+        if n < 0:
+            raise IndexError(w)
+        out = np.zeros(n)
+        check(lib().mbg_result_window_errors(self._h, w, out.ctypes.data))
+        return out
 
     def final_states(self) -> tuple[np.ndarray, np.ndarray]:
         n = lib().mbg_result_final_size(self._h)

@@ -34,7 +34,8 @@ EXPORTED = [
     "mbg_batch_params_default", "mbg_simulate_batched",
     "mbg_grad_params_default", "mbg_grad_check", "mbg_backward", "mbg_backward_noalloc",
     "mbg_result_free", "mbg_result_wait", "mbg_result_get_info", "mbg_result_window_size", "mbg_result_window",
-    "mbg_result_window_probs", "mbg_result_entropies", "mbg_result_final_size",
+    "mbg_result_window_probs", "mbg_result_entropies", "mbg_result_error_th", "mbg_result_window_errors",
+    "mbg_result_final_size",
     "mbg_result_final", "mbg_result_fixed_size", "mbg_result_fixed",
     "mbg_result_write_csv", "mbg_result_print", "mbg_result_write_raw",
 ]
@@ -77,7 +78,8 @@ class SimParams(ctypes.Structure):
         ("final_capacity", ctypes.c_uint64), ("fixed_capacity", ctypes.c_uint64),
         ("block_size", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
         ("normalization", ctypes.c_double), ("flags", ctypes.c_int32), ("kernel", ctypes.c_int32),
-        ("rates", ctypes.POINTER(ctypes.c_float)), ("reserved", ctypes.c_int32 * 4),
+        ("rates", ctypes.POINTER(ctypes.c_float)), ("moment_states", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -136,6 +138,7 @@ class ResultInfo(ctypes.Structure):
         ("window_mode", ctypes.c_int32), ("final_mode", ctypes.c_int32),
         ("window_capacity", ctypes.c_uint64), ("kernel", ctypes.c_int32), ("pad", ctypes.c_int32),
         ("simulate_ms", ctypes.c_double), ("window_pairs", ctypes.c_uint64),
+        ("moment_spills", ctypes.c_uint64), ("has_errors", ctypes.c_int32), ("pad2", ctypes.c_int32),
     ]
 
 
@@ -184,6 +187,8 @@ def _declare(lib):
         "mbg_result_window": ([P, i32, P, P], ctypes.c_int),
         "mbg_result_window_probs": ([P, i32, P, P], ctypes.c_int),
         "mbg_result_entropies": ([P, P, P], ctypes.c_int),
+        "mbg_result_error_th": ([P, P], ctypes.c_int),
+        "mbg_result_window_errors": ([P, ctypes.c_int32, P], ctypes.c_int),
         "mbg_result_final_size": ([P], i64),
         "mbg_result_final": ([P, P, P], ctypes.c_int),
         "mbg_result_fixed_size": ([P], i64),

@@ -135,6 +135,27 @@ struct mbg_batch_chunk
 // lane per trajectory replays the forward trajectory (same stream, same
 // float arithmetic) and accumulates, per rate parameter, the score terms of
 // sum_w Gw * S(e_w) in the suffix form described in mbg_grad_kernel.h.
+// Second-moment replay (no counterpart in the reference): the run's
+// trajectories again, squaring each trajectory's per-window totals.
+#define MBG_MOM_CTR_QUEUE 0 // trajectory work queue
+#define MBG_MOM_CTR_SPILL 1 // spill records written (may exceed the capacity)
+#define MBG_MOM_NCOUNTERS 2
+
+struct mbg_moments_args
+{
+	unsigned long long n_traj, id_base;
+	unsigned int seed_lo, seed_hi, variant, max_steps;
+	int n_nodes, n_obs, windows, discrete;
+	float max_time, tick, qscale, pad;
+	const float* istate;
+	mbg_param_view pv;              // rates (run-time values) and rate * log2(rate), masks all enabled
+	unsigned long long* sq;         // [windows << n_obs][2]: sum of P^2, low and high words
+	unsigned long long* esq;        // [windows][2]: sum of T^2
+	unsigned long long* spill;      // [spill_cap][4]: id, window, obs, partial P
+	unsigned long long spill_cap;
+	unsigned long long* counters;   // [MBG_MOM_NCOUNTERS]
+};
+
 struct mbg_grad_args
 {
 	unsigned long long n_traj, id_base;

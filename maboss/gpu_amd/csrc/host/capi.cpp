@@ -295,6 +295,8 @@ static mbg::sim_params to_sim_params(mbg_program* p, const mbg_sim_params* param
 		sp.blocks_per_cu = params->blocks_per_cu;
 		sp.normalization = params->normalization;
 		sp.device_only = (params->flags & MBG_SIM_DEVICE_ONLY) != 0;
+		sp.errors = (params->flags & MBG_SIM_ERRORS) != 0;
+		sp.moment_states = params->moment_states;
 		sp.kernel = params->kernel;
 		if (params->rates)
 			sp.rates.assign(params->rates, params->rates + p->p->gen.param_names.size());
@@ -310,9 +312,17 @@ static int simulate_entry(mbg_program* p, const mbg_sim_params* params, void* st
 		if (!p || !out)
 			throw mbg::error(mbg::MBG_E_ARG, "null argument");
 		const mbg::sim_params sp = to_sim_params(p, params);
+		if (sp.errors && sp.device_only)
+			throw mbg::error(mbg::MBG_E_ARG, "MBG_SIM_ERRORS needs the statistics on the host");
+		if (sp.moment_states != 0 && sp.moment_states != 1 && sp.moment_states != 8)
+			throw mbg::error(mbg::MBG_E_ARG, "moment_states must be 0, 1 or 8");
 		auto rr = std::make_unique<mbg_result>();
-		rr->r = async ? mbg::simulate_async(*p->p, sp, (hipStream_t)stream)
-					  : mbg::simulate(*p->p, sp, (hipStream_t)stream);
+		// with error columns the run completes first: the replay reads nothing
+		// of it but its trajectory count, and the result carries both
+		rr->r = async && !sp.errors ? mbg::simulate_async(*p->p, sp, (hipStream_t)stream)
+									: mbg::simulate(*p->p, sp, (hipStream_t)stream);
+		if (sp.errors)
+			mbg::run_moments(*p->p, sp, (hipStream_t)stream, *rr->r);
 		*out = rr.release();
 	});
 }
@@ -512,6 +522,8 @@ int mbg_result_get_info(const mbg_result* rr, mbg_result_info* out)
 		out->total_ms = r.total_ms;
 		out->simulate_ms = r.simulate_ms;
 		out->window_pairs = r.window_pairs;
+		out->moment_spills = r.moment_spills;
+		out->has_errors = r.has_moments ? 1 : 0;
 		out->window_mode = r.window_mode;
 		out->final_mode = r.final_mode;
 		out->window_capacity = r.window_capacity;
@@ -586,6 +598,31 @@ int mbg_result_entropies(const mbg_result* rr, double* th, double* h)
 			if (h)
 				h[w] = rr->r->h(w);
 		}
+	});
+}
+
+int mbg_result_error_th(const mbg_result* rr, double* err_th)
+{
+	return guarded([&] {
+		if (!rr || !err_th)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		rr->r->wait();
+		for (int w = 0; w < rr->r->windows; w++)
+			err_th[w] = rr->r->error_th(w);
+	});
+}
+
+int mbg_result_window_errors(const mbg_result* rr, int32_t w, double* err)
+{
+	return guarded([&] {
+		if (!rr || !err || w < 0 || w >= rr->r->windows)
+			throw mbg::error(mbg::MBG_E_ARG, "bad window");
+		const mbg::result& r = *rr->r;
+		r.wait();
+		const uint64_t om = r.obs_mask();
+		size_t i = 0;
+		for (uint64_t c = r.woff[w]; c < r.woff[w + 1]; c++, i++)
+			err[i] = r.error_prob(w, r.win.k(c) & om, r.win.v(c));
 	});
 }
 

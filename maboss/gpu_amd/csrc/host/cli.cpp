@@ -2,7 +2,9 @@
 // (src/main.cpp:101-169).  With -o the results go to <prefix>_probtraj.csv
 // and <prefix>_fp.csv, otherwise the report is printed to stdout.  Built on
 // the C-ABI only.  Extra options (not in the reference): --finals also writes
-// <prefix>_finalstates.csv, --raw <path> writes the exact integer dump.
+// <prefix>_finalstates.csv, --raw <path> writes the exact integer dump,
+// --errors fills the ErrorTH / ErrorProba columns (MBG_SIM_ERRORS; 0 without,
+// as the reference prints them).
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -19,13 +21,15 @@ static int fail(const char* what)
 int main(int argc, char** argv)
 {
 	std::vector<std::string> args;
-	bool finals = false;
+	bool finals = false, errors = false;
 	std::string raw;
 	for (int i = 1; i < argc; i++)
 	{
 		std::string a = argv[i];
 		if (a == "--finals")
 			finals = true;
+		else if (a == "--errors")
+			errors = true;
 		else if (a == "--raw" && i + 1 < argc)
 			raw = argv[++i];
 		else
@@ -53,8 +57,12 @@ int main(int argc, char** argv)
 		mbg_model_free(m);
 		return fail("compilation");
 	}
+	mbg_sim_params sp;
+	mbg_sim_params_default(&sp);
+	if (errors)
+		sp.flags |= MBG_SIM_ERRORS;
 	mbg_result* r = nullptr;
-	int rc = mbg_simulate(p, nullptr, nullptr, &r);
+	int rc = mbg_simulate(p, &sp, nullptr, &r);
 	if (rc != MBG_OK)
 	{
 		mbg_program_free(p);

@@ -191,6 +191,12 @@ struct program
 	const grad_code& grad();          // throws MBG_E_LIMIT when the model has no gradient form
 	hipFunction_t grad_function();
 	device_buffer b_gen;                 // gradient workspace: counters, tables
+
+	// second-moment replay (compiled per model and per states-per-lane on first use)
+	hipModule_t mod_mom[2] = {};
+	hipFunction_t fn_mom[2] = {};
+	hipFunction_t moments_function(int states_per_lane);
+	device_buffer b_mom;
 	grad_host_tables gtab;                    // the gradient kernel's tables, built on first use
 	const grad_host_tables& grad_tables();
 	size_t lds_per_block = 0;            // device property, read once
@@ -231,6 +237,8 @@ struct sim_params
 	int kernel = -1; // -1 auto, 0 lane per trajectory, 1 wave per trajectory
 	bool device_only = false; // leave the histograms in HBM
 	std::vector<float> rates;  // [n_params] run-time rate values; empty: the model's
+	bool errors = false;       // second moments for ErrorTH / ErrorProba (a replay after the run)
+	int moment_states = 0;     // observed states a lane holds per window in that replay (0: 8)
 };
 
 struct result
@@ -259,6 +267,16 @@ struct result
 	int kernel = 0; // 0 lane, 1 wave
 	uint64_t window_capacity = 0;
 
+	// Second moments (sim_params::errors): per window cell and per window,
+	// the exact sum over trajectories of the square of the trajectory's
+	// fixed-point time in the cell, and of its entropy sum in the window.
+	bool has_moments = false;
+	std::vector<unsigned __int128> msq;  // [windows << n_obs], dense
+	std::vector<unsigned __int128> mesq; // [windows]
+	uint64_t moment_spills = 0;          // partial times the replay sent to the host
+	double error_th(int w) const;                       // standard error of TH (0 without moments)
+	double error_prob(int w, uint64_t obs, uint64_t raw) const; // standard error of a cell's probability
+
 	double prob(uint64_t raw) const;     // window cell -> probability
 	double th(int w) const;              // transition entropy of window w
 	double h(int w) const;               // state entropy of window w (float, as the reference)
@@ -286,6 +304,10 @@ struct result
 };
 
 std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t stream);
+// The second moments of a finished run (sim_params::errors): replays its
+// trajectories with the moments kernel and fills r.msq / r.mesq.  Lane
+// kernel, dense windows (windows << n_obs <= 2^24).
+void run_moments(program& p, const sim_params& sp, hipStream_t stream, result& r);
 // As simulate(), but returns once the run is enqueued: the statistics are
 // copied to pinned host memory on the program's copy stream and the result
 // completes in wait().

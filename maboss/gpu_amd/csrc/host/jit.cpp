@@ -32,6 +32,7 @@ extern const char mbg_hdr_kernel_args[];
 extern const char mbg_hdr_stats_device[];
 extern const char mbg_hdr_lane_core[];
 extern const char mbg_hdr_grad_kernel[];
+extern const char mbg_hdr_moments_kernel[];
 extern const char mbg_src_lane[];
 extern const unsigned char mbg_co_wave[];
 extern const unsigned char mbg_co_util[];
@@ -140,13 +141,13 @@ void rtc_check(hiprtcResult r, const char* what)
 // the device headers supplied by name) straight to a code object.
 std::vector<char> compile_to_code_object(const std::string& source)
 {
-	const char* headers[] = { mbg_hdr_numeric,		mbg_hdr_abi,	   mbg_hdr_kernel_args,
-							  mbg_hdr_stats_device, mbg_hdr_lane_core, mbg_hdr_grad_kernel };
-	const char* names[] = { "mbg_numeric.h",	  "mbg_abi.h",		"mbg_kernel_args.h",
-							"mbg_stats_device.h", "mbg_lane_core.h", "mbg_grad_kernel.h" };
+	const char* headers[] = { mbg_hdr_numeric,	  mbg_hdr_abi,		 mbg_hdr_kernel_args,	mbg_hdr_stats_device,
+							  mbg_hdr_lane_core, mbg_hdr_grad_kernel, mbg_hdr_moments_kernel };
+	const char* names[] = { "mbg_numeric.h",	 "mbg_abi.h",		  "mbg_kernel_args.h",		"mbg_stats_device.h",
+							"mbg_lane_core.h", "mbg_grad_kernel.h", "mbg_moments_kernel.h" };
 	std::vector<const char*> flags(std::begin(k_compile_flags), std::end(k_compile_flags));
 	hiprtcProgram prog;
-	rtc_check(hiprtcCreateProgram(&prog, source.c_str(), "mbg_grad.hip", 6, headers, names), "hiprtcCreateProgram");
+	rtc_check(hiprtcCreateProgram(&prog, source.c_str(), "mbg_unit.hip", 7, headers, names), "hiprtcCreateProgram");
 	hiprtcResult r = hiprtcCompileProgram(prog, (int)flags.size(), flags.data());
 	size_t log_size = 0;
 	hiprtcGetProgramLogSize(prog, &log_size);
@@ -232,6 +233,9 @@ program::~program()
 		(void)hipModuleUnload(mod_util);
 	if (mod_grad)
 		(void)hipModuleUnload(mod_grad);
+	for (hipModule_t m : mod_mom)
+		if (m)
+			(void)hipModuleUnload(m);
 }
 
 const grad_code& program::grad()
@@ -263,6 +267,26 @@ hipFunction_t program::grad_function()
 	hip_check(hipModuleGetFunction(&fn_grad, mod_grad, "simulate_grad"), "hipModuleGetFunction(simulate_grad)");
 	compile_seconds += wall_seconds() - t0;
 	return fn_grad;
+}
+
+hipFunction_t program::moments_function(int states_per_lane)
+{
+	const int slot = states_per_lane == 1 ? 1 : 0;
+	if (fn_mom[slot])
+		return fn_mom[slot];
+	scope_timer tm("compiler> moments kernel");
+	const double t0 = wall_seconds();
+	const std::string src = "#define MBG_MOM_K " + std::to_string(slot ? 1 : 8) + "\n" + rates_define(mdl->n_nodes())
+							+ gen.source + "\n#include \"mbg_moments_kernel.h\"\n";
+	const uint64_t h = unit_hash(src, { mbg_hdr_numeric, mbg_hdr_abi, mbg_hdr_kernel_args, mbg_hdr_stats_device,
+										mbg_hdr_lane_core, mbg_hdr_moments_kernel });
+	char name[64];
+	std::snprintf(name, sizeof(name), "%016llx_m.co", (unsigned long long)h);
+	mod_mom[slot] = load_unit(src, name, no_cache, "hipModuleLoadData(simulate_moments)");
+	hip_check(hipModuleGetFunction(&fn_mom[slot], mod_mom[slot], "simulate_moments"),
+			  "hipModuleGetFunction(simulate_moments)");
+	compile_seconds += wall_seconds() - t0;
+	return fn_mom[slot];
 }
 
 void program::load_util()

@@ -146,13 +146,16 @@ void result::write_csv(const std::string& prefix, bool with_finals) const
 		const uint64_t om = obs_mask();
 		for (int w = 0; w < windows; w++)
 		{
-			ofs << (float)w * tick << "\t" << (float)th(w) << "\t" << 0.f << "\t" << (float)h(w) << "\t" << 0.f;
+			// the error columns are 0 without second moments, as the reference prints them
+			ofs << (float)w * tick << "\t" << (float)th(w) << "\t" << (float)error_th(w) << "\t" << (float)h(w) << "\t"
+				<< 0.f;
 			for (uint64_t i = woff[w]; i < woff[w + 1]; i++)
 			{
 				const float p = (float)prob(win.v(i));
 				if (p == 0.f)
 					continue;
-				ofs << "\t" << state_name(win.k(i) & om) << "\t" << p << "\t" << 0.f;
+				ofs << "\t" << state_name(win.k(i) & om) << "\t" << p << "\t"
+					<< (float)error_prob(w, win.k(i) & om, win.v(i));
 			}
 			ofs << "\n";
 		}
@@ -227,6 +230,19 @@ void result::write_raw(std::ostream& os) const
 			os << "W " << w << " " << (win.k(i) & om) << " " << win.v(i) << "\n";
 	for (uint64_t i = 0; i < fin.n; i++)
 		os << "F " << fin.k(i) << " " << fin.v(i) << "\n";
+	if (has_moments)
+	{
+		// second moments: R w hi lo (entropy), Q w obs hi lo (cells)
+		auto u128s = [](unsigned __int128 v) {
+			return std::to_string((uint64_t)(v >> 64)) + " " + std::to_string((uint64_t)v);
+		};
+		for (int w = 0; w < windows; w++)
+			if (mesq[w])
+				os << "R " << w << " " << u128s(mesq[w]) << "\n";
+		for (uint64_t c = 0; c < msq.size(); c++)
+			if (msq[c])
+				os << "Q " << (c >> n_obs) << " " << (c & om) << " " << u128s(msq[c]) << "\n";
+	}
 	for (uint64_t i = 0; i < n_fixed(); i++)
 	{
 		os << "P";

@@ -38,6 +38,8 @@
 //   --raw PATH       exact integer dump (same format as the engine's)
 //   -o PREFIX        write PREFIX_probtraj.csv and PREFIX_fp.csv
 //   --print PATH     write the stdout report to PATH
+//   --errors         exact per-trajectory second moments: R / Q lines in the
+//                    raw dump, ErrorTH / ErrorProba in the CSV (0 without)
 //   --time           print a JSON timing line to stderr
 //   --eval-states    read lines "w0 .. w{W-1} u2bits" on stdin, print
 //                    "total H flip obs" (float bits in hex) per line
@@ -73,6 +75,22 @@
 
 namespace orc
 {
+
+using u128 = unsigned __int128;
+
+// Standard error of a mean over n trajectories from exact sums: s1 = sum of
+// the per-trajectory values, s2 = sum of their squares, both in fixed-point
+// units of `unit`: sqrt((E[x^2] - E[x]^2) / (n - 1)), 0 for n < 2 or a
+// negative rounding of the variance.
+double std_error(uint64_t s1, u128 s2, double n, double unit)
+{
+	if (n < 2.0)
+		return 0.0;
+	const double mean = (double)s1 / (n * unit);
+	const double m2 = ((double)(uint64_t)(s2 >> 64) * 18446744073709551616.0 + (double)(uint64_t)s2) / (n * unit * unit);
+	const double var = (m2 - mean * mean) / (n - 1.0);
+	return var > 0.0 ? std::sqrt(var) : 0.0;
+}
 
 [[noreturn]] void die(const std::string& m)
 {
@@ -1034,6 +1052,12 @@ struct Stats
 	CellMap fin;
 	std::map<std::vector<uint32_t>, uint64_t> fixedp;   // key: words reversed (MSW first)
 	uint64_t transitions = 0, finished = 0, truncated = 0;
+	// --errors: per-trajectory second moments, exact.  sq[(w << nobs) | obs]
+	// = sum over trajectories of P^2, P the trajectory's fixed-point time in
+	// obs during window w; esq[w] = sum of T^2, T its entropy sum in window w.
+	bool moments = false;
+	std::unordered_map<uint64_t, u128> sq;
+	std::vector<u128> esq;
 
 	void init(int nobs_, int windows_)
 	{
@@ -1043,6 +1067,7 @@ struct Stats
 		if (dense)
 			dwin.assign((size_t)windows << nobs, 0);
 		ent.assign(windows, 0);
+		esq.assign(windows, 0);
 	}
 	void add(int w, uint64_t o, uint64_t q)
 	{
@@ -1067,6 +1092,50 @@ struct Stats
 		transitions += o.transitions;
 		finished += o.finished;
 		truncated += o.truncated;
+		for (auto& kv : o.sq)
+			sq[kv.first] += kv.second;
+		for (int w = 0; w < windows; w++)
+			esq[w] += o.esq[w];
+	}
+};
+
+// One trajectory's contributions to the window it is in: its time in each
+// observed state and its entropy sum, squared into Stats::sq / esq when the
+// trajectory leaves the window (or ends).  A state visited twice in one
+// window is one entry: the square is of the trajectory's total.
+struct WindowMoments
+{
+	int w = -1;
+	std::vector<std::pair<uint64_t, uint64_t>> cells; // (obs, P)
+	uint64_t T = 0;
+
+	void at(int w_, Stats& st)
+	{
+		if (w_ != w)
+			flush(st);
+		w = w_;
+	}
+	void add(uint64_t o, uint64_t q)
+	{
+		for (auto& c : cells)
+			if (c.first == o)
+			{
+				c.second += q;
+				return;
+			}
+		cells.emplace_back(o, q);
+	}
+	void flush(Stats& st)
+	{
+		if (w >= 0)
+		{
+			for (auto& c : cells)
+				st.sq[((uint64_t)w << st.nobs) | c.first] += (u128)c.second * c.second;
+			st.esq[w] += (u128)T * T;
+		}
+		cells.clear();
+		T = 0;
+		w = -1;
 	}
 };
 
@@ -1074,6 +1143,7 @@ struct Stats
 // window accumulation (window_average_small.cu:72-91 / :123-129).
 void trajectory(const Sim& S, uint64_t id, Stats& st, std::vector<float>& r, std::vector<uint32_t>& s)
 {
+	WindowMoments wm;
 	const uint32_t idlo = (uint32_t)id, idhi = (uint32_t)(id >> 32);
 	std::fill(s.begin(), s.end(), 0u);
 	for (int i = 0; i < S.N; i += 4)
@@ -1131,7 +1201,14 @@ void trajectory(const Sim& S, uint64_t id, Stats& st, std::vector<float>& r, std
 			if (w < S.windows)
 			{
 				st.add(w, o, 1);
-				st.ent[w] += (uint64_t)std::rint(H * qscale);
+				const uint64_t qe = (uint64_t)std::rint(H * qscale);
+				st.ent[w] += qe;
+				if (st.moments)
+				{
+					wm.at(w, st);
+					wm.add(o, 1);
+					wm.T += qe;
+				}
 			}
 		}
 		else
@@ -1148,7 +1225,15 @@ void trajectory(const Sim& S, uint64_t id, Stats& st, std::vector<float>& r, std
 					const uint64_t q = (uint64_t)std::rint(sl * qscale);
 					if (q)
 						st.add(w, o, q);
-					st.ent[w] += (uint64_t)std::rint((H * sl) * qscale);
+					const uint64_t qe = (uint64_t)std::rint((H * sl) * qscale);
+					st.ent[w] += qe;
+					if (st.moments)
+					{
+						wm.at(w, st);
+						if (q)
+							wm.add(o, q);
+						wm.T += qe;
+					}
 				}
 				w++;
 				b = en;
@@ -1164,11 +1249,13 @@ void trajectory(const Sim& S, uint64_t id, Stats& st, std::vector<float>& r, std
 				std::vector<uint32_t> key(s.rbegin(), s.rend());
 				st.fixedp[key]++;
 			}
+			wm.flush(st);
 			return;
 		}
 		if (k >= S.max_steps)
 		{
 			st.truncated++;
+			wm.flush(st);
 			return;
 		}
 		const int flip = S.select_flip(r.data(), total, u2 * total, L, P);
@@ -1412,7 +1499,7 @@ int main(int argc, char** argv)
 	std::vector<std::string> pos;
 	long long n_override = -1, first = 0, seed_override = -1;
 	int threads = 0;
-	bool compiled = false, timing = false, eval_states = false;
+	bool compiled = false, timing = false, eval_states = false, errors = false;
 	std::string raw, prefix, print_path, entropy = "log_sum", order = "seq", grad_in, grad_out;
 	unsigned variant = 0, max_steps = 1u << 30;
 	for (int i = 1; i < argc; i++)
@@ -1451,6 +1538,8 @@ int main(int argc, char** argv)
 			timing = true;
 		else if (a == "--eval-states")
 			eval_states = true;
+		else if (a == "--errors")
+			errors = true;
 		else if (a == "--grad-in")
 			grad_in = next();
 		else if (a == "--grad-out")
@@ -1617,7 +1706,10 @@ int main(int argc, char** argv)
 #endif
 	std::vector<Stats> per(nth);
 	for (auto& st : per)
+	{
 		st.init(S.nobs, S.windows);
+		st.moments = errors;
+	}
 	const auto t0 = std::chrono::steady_clock::now();
 #pragma omp parallel
 	{
@@ -1680,6 +1772,18 @@ int main(int argc, char** argv)
 				f << "W " << w << " " << c.first << " " << c.second << "\n";
 		for (auto& c : fin)
 			f << "F " << c.first << " " << c.second << "\n";
+		if (errors)
+		{
+			auto u128s = [](u128 v) { return std::to_string((uint64_t)(v >> 64)) + " " + std::to_string((uint64_t)v); };
+			for (int w = 0; w < S.windows; w++)
+				if (tot.esq[w])
+					f << "R " << w << " " << u128s(tot.esq[w]) << "\n";
+			std::vector<std::pair<uint64_t, u128>> q(tot.sq.begin(), tot.sq.end());
+			std::sort(q.begin(), q.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+			for (auto& c : q)
+				if (c.second)
+					f << "Q " << (c.first >> S.nobs) << " " << (c.first & omask) << " " << u128s(c.second) << "\n";
+		}
 		for (auto& kv : tot.fixedp)
 		{
 			f << "P";
@@ -1703,6 +1807,16 @@ int main(int argc, char** argv)
 		return e;
 	};
 	auto THw = [&](int w) { return (float)((double)tot.ent[w] / ((double)n * 4294967296.0)); };
+	// error columns: 0 without --errors, as the reference prints
+	auto err_th = [&](int w) {
+		return errors ? (float)std_error(tot.ent[w], tot.esq[w], (double)n, 4294967296.0) : 0.f;
+	};
+	auto err_p = [&](int w, uint64_t obs, uint64_t s1) {
+		if (!errors)
+			return 0.f;
+		auto it = tot.sq.find(((uint64_t)w << S.nobs) | obs);
+		return (float)std_error(s1, it == tot.sq.end() ? 0 : it->second, (double)n, S.discrete ? 1.0 : 4294967296.0);
+	};
 	std::vector<int> all(S.N);
 	for (int i = 0; i < S.N; i++)
 		all[i] = i;
@@ -1761,12 +1875,12 @@ int main(int argc, char** argv)
 		f << "\n";
 		for (int w = 0; w < S.windows; w++)
 		{
-			f << (float)w * S.tick << "\t" << THw(w) << "\t" << 0.f << "\t" << Hw(w) << "\t" << 0.f;
+			f << (float)w * S.tick << "\t" << THw(w) << "\t" << err_th(w) << "\t" << Hw(w) << "\t" << 0.f;
 			for (auto& c : win[w])
 			{
 				float p = (float)prob(c.second);
 				if (p != 0.f)
-					f << "\t" << name_of(m, S.obs, c.first) << "\t" << p << "\t" << 0.f;
+					f << "\t" << name_of(m, S.obs, c.first) << "\t" << p << "\t" << err_p(w, c.first, c.second);
 			}
 			f << "\n";
 		}

@@ -36,7 +36,7 @@ def run_oracle(bnd, cfg, *args, timeout=600):
 
 def read_raw(path):
     """Parse an exact integer dump (engine or oracle) into a dict."""
-    out = {"W": {}, "E": {}, "F": {}, "P": {}}
+    out = {"W": {}, "E": {}, "F": {}, "P": {}, "Q": {}, "R": {}}
     with open(path) as f:
         assert f.readline().strip() == "MBG_RAW 1"
         head = f.readline().split()
@@ -53,6 +53,10 @@ def read_raw(path):
                 out["F"][int(t[1])] = int(t[2])
             elif t[0] == "P":
                 out["P"][tuple(map(int, t[1:-1]))] = int(t[-1])
+            elif t[0] == "Q":
+                out["Q"][(int(t[1]), int(t[2]))] = (int(t[3]) << 64) | int(t[4])
+            elif t[0] == "R":
+                out["R"][int(t[1])] = (int(t[2]) << 64) | int(t[3])
     return out
 
 
