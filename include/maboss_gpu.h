@@ -127,7 +127,9 @@ typedef struct mbg_sim_params
 	int32_t blocks_per_cu;   /* 0 = occupancy limit */
 	double normalization;    /* <=0: n_trajectories (reference divides by sample_count) */
 	int32_t flags;           /* MBG_SIM_* bits */
-	int32_t kernel;          /* -1 auto (lane if N <= 256), 0 lane per trajectory, 1 wave per trajectory */
+	int32_t kernel;          /* -1 auto: lane up to 256 nodes; above, the wave kernel when it can
+	                            run the model (fast-form rates, nodes of up to 6 inputs), else
+	                            lane up to 512; 0 lane per trajectory, 1 wave per trajectory */
 	const float* rates;      /* [n_params] run-time values of the rate parameters (mbg_model_param_name
 	                            order), host memory; NULL: the model's.  Lane kernel only. */
 	int32_t moment_states;   /* with MBG_SIM_ERRORS: observed states a lane holds per window in the
@@ -146,7 +148,7 @@ typedef struct mbg_sim_params
  * standard errors of the means, sqrt((E[x^2] - E[x]^2) / (n - 1)).  No
  * counterpart in the reference, which prints 0 in both columns
  * (src/statistics/window_average_small.cpp:235, :244-245).  Lane kernel
- * (up to 256 nodes) and windows x 2^n_obs <= 2^24; without the flag the
+ * (up to 512 nodes) and windows x 2^n_obs <= 2^24; without the flag the
  * columns are 0, as in the reference. */
 #define MBG_SIM_ERRORS 2
 

@@ -49,10 +49,55 @@ __device__ void init_states_wave(uint32_t (&s)[W], bool take, uint64_t id, uint3
 								 uint32_t seed_hi, int n_nodes, const float* istate)
 {
 	const int lane = __lane_id();
-	const int B = (n_nodes + 3) >> 2; // Philox blocks per trajectory, <= 8 * W <= 64
-	const int per = 64 / B;           // trajectories per round
-	const int q = lane / B, b = lane - q * B;
+	const int B = (n_nodes + 3) >> 2; // Philox blocks per trajectory, at most 8 * W
 	uint64_t todo = __ballot(take);
+	if (B > 64)
+	{
+		// above 256 nodes: one trajectory per round, its blocks b = lane +
+		// 64 * pass (at most 2 passes for 16 words); the 8 blocks of state
+		// word w = 8 * pass + (lane >> 3) meet in lane 8 * (w & 7)
+		while (todo)
+		{
+			const int target = __ffsll((unsigned long long)todo) - 1;
+			const uint32_t lo = __shfl((uint32_t)id, target), hi = __shfl((uint32_t)(id >> 32), target);
+			for (int pass = 0; 64 * pass < B; ++pass)
+			{
+				const int b = lane + 64 * pass;
+				uint32_t v = 0u;
+				if (b < B)
+				{
+					const mbg_u32x4 r = mbg_philox4x32_10(mbg_u32x4 { (uint32_t)b, lo, hi, variant }, seed_lo, seed_hi);
+					const uint32_t rw[4] = { r.x, r.y, r.z, r.w };
+#pragma unroll
+					for (int j = 0; j < 4; ++j)
+					{
+						const int i = 4 * b + j;
+						if (i < n_nodes && mbg_u01(rw[j]) <= istate[i])
+							v |= 1u << (4 * (b & 7) + j);
+					}
+				}
+#pragma unroll
+				for (int off = 1; off < 8; off <<= 1)
+				{
+					const uint32_t o = __shfl_down(v, off);
+					if ((b & 7) + off < 8 && b + off < B)
+						v |= o;
+				}
+#pragma unroll
+				for (int w = 0; w < W; ++w)
+					if ((w >> 3) == pass)
+					{
+						const uint32_t word = __shfl(v, 8 * (w & 7));
+						if (lane == target)
+							s[w] = 8 * w < B ? word : 0u;
+					}
+			}
+			todo &= todo - 1ull;
+		}
+		return;
+	}
+	const int per = 64 / B; // trajectories per round
+	const int q = lane / B, b = lane - q * B;
 	while (todo)
 	{
 		// this round: the first `per` lanes of todo

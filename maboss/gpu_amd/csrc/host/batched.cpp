@@ -75,7 +75,7 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 	const model& m = *p.mdl;
 	p.order_after_last_run(stream);
 	const int W = m.n_words(), N = m.n_nodes(), n_obs = m.n_observed();
-	if (W > max_lane_words)
+	if (W > max_form_words)
 		throw error(MBG_E_LIMIT, "the batched form serves models of up to 256 nodes");
 	if (n_obs > 20)
 		throw error(MBG_E_LIMIT, "the batched form keeps dense per-variant windows: at most 20 observed nodes");

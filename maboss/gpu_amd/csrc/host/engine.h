@@ -220,8 +220,15 @@ struct program
 
 std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const gen_options& opt, bool no_cache);
 
-// Largest state-word count served by the lane-per-trajectory kernels.
-constexpr int max_lane_words = 8;
+// Largest state-word count served by the lane-per-trajectory kernels (512
+// nodes, the reference's limit, CMakeLists.txt:12-13); above auto_lane_words
+// the automatic choice is the wave kernel when it can run the model (its
+// table-driven evaluator covers fast-form rates of nodes with up to 6
+// inputs), else the lane kernel with the model's generated code.  The
+// batched and gradient forms keep the 256-node limit.
+constexpr int max_lane_words = 16;
+constexpr int auto_lane_words = 8;
+constexpr int max_form_words = 8;
 
 struct sim_params
 {

@@ -74,7 +74,7 @@ void simulate_grad(program& p, const grad_params& gp, const float* rates, const 
 	if (scratch && !p.fn_grad)
 		throw error(MBG_E_ARG, "the gradient kernel is not loaded: call mbg_grad_check first");
 	const grad_code& gc = p.grad();
-	if (m.n_words() > max_lane_words)
+	if (m.n_words() > max_form_words)
 		throw error(MBG_E_LIMIT, "the gradient form serves models of up to 256 nodes");
 	const int n_obs = m.n_observed(), windows = m.windows(), N = m.n_nodes();
 	if (n_obs > 24)

@@ -420,7 +420,7 @@ std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const g
 	p->num_cus = prop.multiProcessorCount;
 	// the no-allocation entry point's scratch on this device (once per device)
 	noalloc_setup(p->device);
-	if (p->W <= max_lane_words)
+	if (p->W <= auto_lane_words || !p->wtab)
 	{
 		int wm = 0, fm = 0;
 		default_storage_modes(*p->mdl, wm, fm);

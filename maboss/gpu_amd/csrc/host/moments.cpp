@@ -78,7 +78,7 @@ void run_moments(program& p, const sim_params& sp, hipStream_t stream, result& r
 	scope_timer tm("moments> replay");
 	const model& m = *p.mdl;
 	if (r.kernel != 0 || m.n_words() > max_lane_words)
-		throw error(MBG_E_LIMIT, "error columns are computed for the lane kernel (models of up to 256 nodes)");
+		throw error(MBG_E_LIMIT, "error columns are computed for lane-kernel runs (up to 512 nodes)");
 	const int n_obs = m.n_observed(), windows = m.windows(), N = m.n_nodes(), W = m.n_words();
 	if (n_obs > 24 || ((uint64_t)windows << n_obs) > k_max_moment_cells)
 		throw error(MBG_E_LIMIT, "error columns need windows x 2^n_obs <= 2^24 window cells");

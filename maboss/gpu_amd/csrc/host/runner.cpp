@@ -313,13 +313,14 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 	res->ent.assign(windows, 0);
 	res->woff.assign((size_t)windows + 1, 0);
 
-	const bool use_wave = sp.kernel == 1 || (sp.kernel < 0 && W > max_lane_words);
+	const bool use_wave =
+		sp.kernel == 1 || (sp.kernel < 0 && (W > max_lane_words || (W > auto_lane_words && p.wtab != nullptr)));
 	if (!use_wave && W > max_lane_words)
 		throw error(MBG_E_LIMIT, "the lane kernel serves up to " + std::to_string(32 * max_lane_words) + " nodes");
 	if (use_wave && p.opts.entropy == entropy_formula::reference)
 		throw error(MBG_E_ARG, "the wave kernel computes the log-sum entropy only");
 	if (!sp.rates.empty() && use_wave)
-		throw error(MBG_E_LIMIT, "run-time rates are served by the lane kernel (up to 256 nodes)");
+		throw error(MBG_E_LIMIT, "run-time rates are served by the lane kernel (up to 512 nodes)");
 	if (!sp.rates.empty() && sp.rates.size() != p.gen.param_names.size())
 		throw error(MBG_E_ARG, "expected " + std::to_string(p.gen.param_names.size()) + " rates");
 	res->kernel = use_wave ? 1 : 0;
