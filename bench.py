@@ -253,6 +253,7 @@ def bench_batched(mg, netgen, wdir, steps):
         trans += info.transitions
     one = {"transitions_per_s": trans / dev, "device_ms": dev / steps * 1e3, "wall_ms": wall / steps * 1e3,
            "block": info.block_size, "blocks_per_cu": info.blocks_per_cu, "traj_per_lane": info.traj_per_lane,
+           "lds_cells": info.lds_cells, "tuned_from_cache": bool(info.tuned_cached),
            "graph": bool(info.used_graph), "ops_per_transition": valu_ops(model.info),
            "valu_frac": valu_frac(model.info, trans / dev)}
     t0 = time.perf_counter()
@@ -262,7 +263,8 @@ def bench_batched(mg, netgen, wdir, steps):
     t0 = time.perf_counter()
     for v in variants:
         _, vi = prog.simulate_batched([v], device_only=True, use_graph=False, block_size=info.block_size,
-                                      blocks_per_cu=info.blocks_per_cu, traj_per_lane=info.traj_per_lane)
+                                      blocks_per_cu=info.blocks_per_cu, traj_per_lane=info.traj_per_lane,
+                                      lds_cells=info.lds_cells or -1)
         sdev += vi.kernel_ms / 1e3
         strans += vi.transitions
     swall = time.perf_counter() - t0

@@ -207,7 +207,11 @@ typedef struct mbg_batch_params
 	int32_t traj_per_lane;    /* a work chunk holds 64*traj_per_lane ids of one variant; 0: autotuned */
 	int32_t use_graph;        /* 1: capture the launch in a hipGraph and replay it */
 	int32_t flags;            /* MBG_SIM_DEVICE_ONLY */
-	int32_t reserved[8];
+	int32_t lds_cells;        /* window cells each wave caches in LDS (a power of two, 64-4096) before
+	                             the HBM atomics, flushed when the wave's work chunk ends; 0: autotuned
+	                             with the block size and trajectories per lane (none when both are
+	                             given), -1: none */
+	int32_t reserved[7];
 } mbg_batch_params;
 
 typedef struct mbg_batch_info
@@ -216,11 +220,18 @@ typedef struct mbg_batch_info
 	double total_ms;          /* host wall time of mbg_simulate_batched */
 	uint64_t transitions;     /* all variants */
 	int32_t block_size, blocks_per_cu, traj_per_lane, used_graph;
+	int32_t lds_cells;        /* the launch's LDS cell cache slots per wave (0: none) */
+	int32_t tuned_cached;     /* 1: the launch shape came from the on-disk tuning cache, which keeps
+	                             the autotuner's choice per generated model code, batch shape and
+	                             device in $MBG_CACHE_DIR (else ~/.cache/maboss_gpu) */
 } mbg_batch_info;
 
 void mbg_batch_params_default(mbg_batch_params* p);
 /* Simulate n_variants variants in one launch.  results[v] receives variant
- * v's statistics (free each with mbg_result_free); info may be NULL. */
+ * v's statistics (free each with mbg_result_free); info may be NULL.  The
+ * autotuner times every block size (64, 128, 256) x trajectories per lane
+ * (1, 2, 4) x LDS cell cache (none, 256, 1024 slots) on the batch and keeps
+ * the fastest; results do not depend on the choice (integer sums). */
 int mbg_simulate_batched(mbg_program* p, const mbg_batch_params* params, int32_t n_variants,
 						 const mbg_variant* variants, void* stream, mbg_result** results, mbg_batch_info* info);
 

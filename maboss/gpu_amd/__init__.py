@@ -183,7 +183,7 @@ class Program:
         return Result(h, self.model)
 
     def simulate_batched(self, variants, seed: int = -1, max_steps: int = 0, block_size: int = 0,
-                         blocks_per_cu: int = 0, traj_per_lane: int = 0, use_graph: bool = True,
+                         blocks_per_cu: int = 0, traj_per_lane: int = 0, use_graph: bool = True, lds_cells: int = 0,
                          device_only: bool = False, stream=None):
         """Run several variants of the model in one launch.
 
@@ -218,6 +218,7 @@ class Program:
         bp.block_size = block_size
         bp.blocks_per_cu = blocks_per_cu
         bp.traj_per_lane = traj_per_lane
+        bp.lds_cells = lds_cells
         bp.use_graph = int(use_graph)
         bp.flags = 1 if device_only else 0
         outs = (ctypes.c_void_p * len(variants))()

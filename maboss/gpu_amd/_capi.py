@@ -96,7 +96,7 @@ class BatchParams(ctypes.Structure):
         ("seed", ctypes.c_int64), ("max_steps", ctypes.c_uint32),
         ("block_size", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
         ("traj_per_lane", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-        ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32 * 8),
+        ("flags", ctypes.c_int32), ("lds_cells", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
     ]
 
 
@@ -105,7 +105,7 @@ class BatchInfo(ctypes.Structure):
         ("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
         ("transitions", ctypes.c_uint64), ("block_size", ctypes.c_int32),
         ("blocks_per_cu", ctypes.c_int32), ("traj_per_lane", ctypes.c_int32),
-        ("used_graph", ctypes.c_int32),
+        ("used_graph", ctypes.c_int32), ("lds_cells", ctypes.c_int32), ("tuned_cached", ctypes.c_int32),
     ]
 
 

@@ -105,7 +105,8 @@ struct mbg_sim_args
 	// batched form only
 	const struct mbg_batch_variant* bv;  // [n_variants]
 	const struct mbg_batch_chunk* chunks; // work list, each chunk inside one variant
-	unsigned int n_chunks, pad3;
+	unsigned int n_chunks;
+	unsigned int bcache_slots;           // per-wave LDS window-cell cache slots (power of two; 0: none)
 	unsigned long long* vctr;            // per-variant {transitions, finished, truncated}
 };
 

@@ -276,6 +276,8 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 
 #else // MBG_PARAM == 1
 
+#define MBG_BCACHE_EMPTY 0xffffffffu
+
 // Batched variants.  Statistics of variant v live at its offsets in the
 // dense window / entropy / final arrays; fixed points share one table keyed
 // by (state, variant); the stream of variant v is the plain run's stream
@@ -292,6 +294,19 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 	// histogram atomics in flight)
 	__shared__ float ist_tab[4][32 * W];
 	float* const istate = ist_tab[(threadIdx.x >> 6) & 3];
+	// per wave: the window-cell cache, S values (u64) then S keys (u32) in
+	// dynamic LDS; cell index (w << n_obs) | obs within the chunk's variant
+	extern __shared__ unsigned long long bcache[];
+	const uint32_t S = a.bcache_slots;
+	const int sbits = S ? 31 - __builtin_clz(S) : 0;
+	unsigned long long* const cval = bcache + (size_t)(threadIdx.x >> 6) * (S + S / 2);
+	uint32_t* const ckey = (uint32_t*)(cval + S);
+	for (uint32_t i = (uint32_t)lane; i < S; i += 64u)
+	{
+		cval[i] = 0ull;
+		ckey[i] = MBG_BCACHE_EMPTY;
+	}
+	MBG_WAVE_SYNC();
 	uint32_t s[W];
 #pragma unroll
 	for (int w = 0; w < W; ++w)
@@ -318,8 +333,22 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 		const mbg_param_view pv = vd.pv;
 		mbg_eval_p_setup(&pv);
 
+		// window cells through the wave's LDS cache when it has one: the
+		// chunk's trajectories (one variant) add to a slot keyed by the cell,
+		// and a cell whose slot another cell holds goes to HBM directly
 		auto fw = [&](int w, uint64_t o, uint64_t q) {
-			atomicAdd(a.win_vals + win_off + (((uint64_t)w << a.n_obs) | o), (unsigned long long)q);
+			const uint32_t cell = ((uint32_t)w << a.n_obs) | (uint32_t)o;
+			if (S)
+			{
+				const uint32_t slot = (cell * 0x9E3779B1u) >> (32 - sbits);
+				const uint32_t k = atomicCAS(ckey + slot, MBG_BCACHE_EMPTY, cell);
+				if (k == MBG_BCACHE_EMPTY || k == cell)
+				{
+					atomicAdd(cval + slot, (unsigned long long)q);
+					return;
+				}
+			}
+			atomicAdd(a.win_vals + win_off + cell, (unsigned long long)q);
 		};
 		auto fe = [&](int w, uint64_t q) { atomicAdd(a.win_ent + ent_off + w, (unsigned long long)q); };
 		auto emit = [&](bool has, int w, uint64_t o, uint64_t q) {
@@ -420,6 +449,22 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 			}
 		}
 		pend.flush_all(fw, fe);
+		if (S)
+		{
+			// the chunk is done: its cached cells to HBM, the slots emptied for the next chunk
+			MBG_WAVE_SYNC();
+			for (uint32_t i = (uint32_t)lane; i < S; i += 64u)
+			{
+				const uint32_t k = ckey[i];
+				if (k != MBG_BCACHE_EMPTY)
+				{
+					atomicAdd(a.win_vals + win_off + k, cval[i]);
+					ckey[i] = MBG_BCACHE_EMPTY;
+					cval[i] = 0ull;
+				}
+			}
+			MBG_WAVE_SYNC();
+		}
 		wave_add_counters(a.vctr + vd.ctr_off, a.vctr + vd.ctr_off + 1, a.vctr + vd.ctr_off + 2, ntrans, nfinished,
 						  ntrunc);
 	}

@@ -15,8 +15,13 @@
 // per batch layout.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <numeric>
+#include <string>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include "../device/mbg_kernel_args.h"
 #include "../device/mbg_numeric.h"
@@ -62,6 +67,50 @@ void fill(program& p, hipStream_t stream, void* ptr, uint64_t words, uint64_t va
 	const unsigned grid = (unsigned)std::min<uint64_t>(2048, (words + 255) / 256);
 	hip_check(hipModuleLaunchKernel(p.fn_fill, grid, 1, 1, 256, 1, 1, 0, stream, args, nullptr),
 			  "hipModuleLaunchKernel(mbg_fill_u64)");
+}
+
+uint64_t fnv(const std::string& s, uint64_t h)
+{
+	h ^= 1469598103934665603ull;
+	for (unsigned char c : s)
+	{
+		h ^= c;
+		h *= 1099511628211ull;
+	}
+	return h;
+}
+
+// The tuning cache: one line "block per_cu traj_per_lane lds_cells" per
+// (generated code, batch shape, device); a missing or unreadable file is a miss.
+bool read_tuned(const std::string& path, program::tuned& t)
+{
+	std::ifstream f(path);
+	int b = 0, pc = 0, tpl = 0, sl = -1;
+	if (!(f >> b >> pc >> tpl >> sl))
+		return false;
+	if (b <= 0 || b > 256 || b % 64 || pc <= 0 || tpl <= 0 || sl < 0 || sl > 4096)
+		return false;
+	t.block = b;
+	t.per_cu = pc;
+	t.tpl = tpl;
+	t.slots = sl;
+	return true;
+}
+
+void write_tuned(const std::string& path, const program::tuned& t)
+{
+	const std::string dir = path.substr(0, path.rfind('/'));
+	(void)::mkdir(dir.c_str(), 0755);
+	const std::string tmp = path + ".tmp" + std::to_string((unsigned long long)::getpid());
+	{
+		std::ofstream f(tmp);
+		if (!f)
+			return;
+		f << t.block << " " << t.per_cu << " " << t.tpl << " " << t.slots << "\n";
+		if (!f)
+			return;
+	}
+	std::rename(tmp.c_str(), path.c_str());
 }
 
 } // namespace
@@ -232,7 +281,12 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 		(void)waves;
 		return grid;
 	};
-	auto launch = [&](int block, uint64_t grid) {
+	// the per-wave LDS cell cache keys cells by their 32-bit index in a variant
+	const int max_windows = L.windows.empty() ? 0 : *std::max_element(L.windows.begin(), L.windows.end());
+	const bool cache_ok = ((uint64_t)max_windows << n_obs) < 0xffffffffull;
+	auto cache_lds = [&](int block, int slots) { return (size_t)(block / 64) * (size_t)slots * 12; };
+	auto launch = [&](int block, uint64_t grid, int slots) {
+		a.bcache_slots = (unsigned)slots;
 		fill(p, stream, counters, MBG_NCOUNTERS, 0);
 		fill(p, stream, win, L.win_cells, 0);
 		fill(p, stream, ent, L.ent_cells, 0);
@@ -242,13 +296,14 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 		if (a.n_chunks)
 		{
 			void* args[] = { &a };
-			hip_check(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, block, 1, 1, 0, stream, args, nullptr),
+			hip_check(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, block, 1, 1, (unsigned)cache_lds(block, slots),
+											stream, args, nullptr),
 					  "hipModuleLaunchKernel(simulate_batched)");
 		}
 	};
-	auto occupancy = [&](int block) {
+	auto occupancy = [&](int block, int slots) {
 		int per = 0;
-		hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, block, 0),
+		hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, block, cache_lds(block, slots)),
 				  "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
 		return std::max(per, 1);
 	};
@@ -267,41 +322,72 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 		return (double)ms;
 	};
 
-	int block = bp.block, per_cu = bp.per_cu, tpl = bp.tpl;
+	if (bp.lds_cells > 0 && (bp.lds_cells < 64 || bp.lds_cells > 4096 || (bp.lds_cells & (bp.lds_cells - 1))))
+		throw error(MBG_E_ARG, "lds_cells must be -1, 0 or a power of two from 64 to 4096");
+	if (bp.lds_cells > 0 && !cache_ok)
+		throw error(MBG_E_LIMIT, "the LDS cell cache needs windows x 2^n_obs below 2^32 per variant");
+	int block = bp.block, per_cu = bp.per_cu, tpl = bp.tpl, slots = bp.lds_cells < 0 ? 0 : bp.lds_cells;
+	bool tuned_cached = false;
 	if (!block || !tpl)
 	{
+		// the tuned shape of this batch: this program's, else the on-disk
+		// tuning cache's (keyed by the generated code and the batch shape),
+		// else a timed search over block x trajectories per lane x LDS layout
+		uint64_t fkey = fnv(p.gen.source, tune_key ^ (cache_ok ? 1u : 0u));
+		char fname[64];
+		std::snprintf(fname, sizeof(fname), "/%016llx.tune", (unsigned long long)fkey);
+		const std::string path = code_cache_dir() + fname;
+		if (p.batch_tuned.key != tune_key)
+		{
+			program::tuned t;
+			if (read_tuned(path, t))
+			{
+				t.key = tune_key;
+				t.cached = true;
+				p.batch_tuned = t;
+			}
+		}
 		if (p.batch_tuned.key != tune_key)
 		{
 			// autotuner: time every candidate on this batch, keep the fastest
 			double best = 1e300;
-			for (int b : { 64, 128, 256 })
-				for (int t : { 1, 2, 4 })
-				{
-					const int pc = occupancy(b);
-					const uint64_t g = enqueue(b, pc, t);
-					const double ms = timed([&] { launch(b, g); });
-					if (ms < best)
+			for (int sl : { 0, 256, 1024 })
+			{
+				if (sl && !cache_ok)
+					continue;
+				for (int b : { 64, 128, 256 })
+					for (int t : { 1, 2, 4 })
 					{
-						best = ms;
-						p.batch_tuned = program::tuned { tune_key, b, pc, t };
+						const int pc = occupancy(b, sl);
+						const uint64_t g = enqueue(b, pc, t);
+						const double ms = timed([&] { launch(b, g, sl); });
+						if (ms < best)
+						{
+							best = ms;
+							p.batch_tuned = program::tuned { tune_key, b, pc, t, sl, false };
+						}
 					}
-				}
+			}
+			write_tuned(path, p.batch_tuned);
 		}
+		tuned_cached = p.batch_tuned.cached;
 		block = block ? block : p.batch_tuned.block;
 		tpl = tpl ? tpl : p.batch_tuned.tpl;
 		per_cu = per_cu ? per_cu : p.batch_tuned.per_cu;
+		if (!bp.lds_cells)
+			slots = cache_ok ? p.batch_tuned.slots : 0;
 	}
 	if (block <= 0 || block > 256 || block % 64)
 		throw error(MBG_E_ARG, "batched block size must be 64, 128, 192 or 256");
 	if (!per_cu)
-		per_cu = occupancy(block);
+		per_cu = occupancy(block, slots);
 	const uint64_t grid = enqueue(block, per_cu, tpl);
 
 	double ms = 0;
 	bool used_graph = false;
 	if (bp.graph)
 	{
-		uint64_t gkey = mix(mix(mix(tune_key, (uint64_t)block), (uint64_t)per_cu), (uint64_t)tpl);
+		uint64_t gkey = mix(mix(mix(mix(tune_key, (uint64_t)block), (uint64_t)per_cu), (uint64_t)tpl), (uint64_t)slots);
 		gkey = mix(mix(mix(gkey, (uint64_t)(uintptr_t)win), (uint64_t)(uintptr_t)fp), (uint64_t)(uintptr_t)dbase);
 		gkey = mix(mix(gkey, (uint64_t)(uintptr_t)a.chunks), seed);
 		gkey = mix(gkey, a.max_steps);
@@ -319,7 +405,7 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 			hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
 			hipStream_t saved = stream;
 			stream = cs;
-			launch(block, grid);
+			launch(block, grid, slots);
 			stream = saved;
 			hip_check(hipStreamEndCapture(cs, &g), "hipStreamEndCapture");
 			hip_check(hipGraphInstantiate(&p.batch_graph, g, nullptr, nullptr, 0), "hipGraphInstantiate");
@@ -331,7 +417,7 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 		used_graph = true;
 	}
 	else
-		ms = timed([&] { launch(block, grid); });
+		ms = timed([&] { launch(block, grid, slots); });
 
 	// ---- occupied cells compacted on the device (in key order: the dense
 	// arrays are compacted in index order), then one copy into pinned memory
@@ -386,7 +472,9 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 		timing->block = block;
 		timing->per_cu = per_cu;
 		timing->tpl = tpl;
+		timing->lds_cells = slots;
 		timing->graph = used_graph;
+		timing->tuned_cached = tuned_cached;
 	}
 
 	const uint64_t n_cells = bp.device_only ? 0 : mb[16], n_fin = bp.device_only ? 0 : mb[17],

@@ -373,6 +373,7 @@ int mbg_simulate_batched(mbg_program* p, const mbg_batch_params* params, int32_t
 			bp.block = params->block_size;
 			bp.per_cu = params->blocks_per_cu;
 			bp.tpl = params->traj_per_lane;
+			bp.lds_cells = params->lds_cells;
 			bp.graph = params->use_graph != 0;
 			bp.device_only = (params->flags & MBG_SIM_DEVICE_ONLY) != 0;
 		}
@@ -405,6 +406,8 @@ int mbg_simulate_batched(mbg_program* p, const mbg_batch_params* params, int32_t
 			info->blocks_per_cu = bt.per_cu;
 			info->traj_per_lane = bt.tpl;
 			info->used_graph = bt.graph ? 1 : 0;
+			info->lds_cells = bt.lds_cells;
+			info->tuned_cached = bt.tuned_cached ? 1 : 0;
 		}
 	});
 }

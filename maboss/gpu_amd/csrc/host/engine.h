@@ -176,7 +176,8 @@ struct program
 	struct tuned
 	{
 		uint64_t key = 0;
-		int block = 0, per_cu = 0, tpl = 0;
+		int block = 0, per_cu = 0, tpl = 0, slots = 0;
+		bool cached = false;
 	} batch_tuned;
 	// captured batched launch, replayed while the batch layout is unchanged
 	hipGraphExec_t batch_graph = nullptr;
@@ -219,6 +220,9 @@ struct program
 };
 
 std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const gen_options& opt, bool no_cache);
+// Directory of the on-disk caches (code objects, tuned launch shapes):
+// $MBG_CACHE_DIR, else ~/.cache/maboss_gpu.
+std::string code_cache_dir();
 
 // Largest state-word count served by the lane-per-trajectory kernels (512
 // nodes, the reference's limit, CMakeLists.txt:12-13); above auto_lane_words
@@ -342,6 +346,7 @@ struct batch_params
 	int64_t seed = -1;
 	uint32_t max_steps = 0;
 	int block = 0, per_cu = 0, tpl = 0; // 0: autotuned
+	int lds_cells = 0;                  // per-wave LDS window-cell cache slots; 0: autotuned, -1: none
 	bool graph = true;                  // capture fills + launch in a hipGraph, replay it
 	bool device_only = false;
 };
@@ -350,8 +355,9 @@ struct batch_timing
 {
 	double kernel_ms = 0, total_ms = 0;
 	uint64_t transitions = 0;
-	int block = 0, per_cu = 0, tpl = 0;
+	int block = 0, per_cu = 0, tpl = 0, lds_cells = 0;
 	bool graph = false;
+	bool tuned_cached = false; // the launch shape came from the on-disk tuning cache
 };
 
 std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_params& bp,

@@ -390,6 +390,8 @@ hipFunction_t program::function(int flavour, int win_mode, int fin_mode)
 	return fn;
 }
 
+std::string code_cache_dir() { return cache_dir(); }
+
 std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const gen_options& opt, bool no_cache)
 {
 	auto p = std::make_unique<program>();

@@ -208,3 +208,75 @@ def test_bad_variants_rejected(b50):
         prog.simulate_batched([{"n_trajectories": -5}])
     res, _ = prog.simulate_batched([])
     assert res == []
+
+
+def _mutant_lines(prog, mut):
+    names = prog.model.node_names
+    return "".join(f"\n{names[i]}.mutant = {'ON' if m > 0 else 'OFF'};" for i, m in enumerate(mut) if m)
+
+
+def test_benchmark_variant_set_vs_oracle(tmp_path, b50):
+    """The batched workload's own 256 variants (netgen.batched_variants: rates x
+    lognormal(0, 0.5), 0-3 mutants, 1e3-1e5 trajectories, max_time 10-200) in one
+    launch; the variants with mutants and the fewest trajectories are checked against
+    the oracle (rates and horizon edited into the .cfg, mutants as X.mutant lines)."""
+    bnd, cfg = b50
+    cfg_text = open(cfg).read()
+    prog = program(bnd, cfg)
+    names = prog.model.param_names
+    variants = netgen.batched_variants(list(prog.model.param_values), prog.model.info.n_nodes)
+    assert len(variants) == 256 and sum(v["n_trajectories"] for v in variants) > 5_000_000
+    print("[gpu] batched_50 benchmark variant set, one launch", flush=True)
+    results, info = prog.simulate_batched(variants)
+    assert info.transitions == sum(r.info.transitions for r in results)
+    with_mut = sorted((v for v in range(256) if any(variants[v]["mutations"])),
+                      key=lambda v: variants[v]["n_trajectories"])
+    assert len(with_mut) > 100
+    for v in with_mut[:3]:
+        spec = variants[v]
+        text = edited_cfg(cfg_text, params=dict(zip(names, spec["rates"])), max_time=spec["max_time"])
+        text += _mutant_lines(prog, spec["mutations"]) + "\n"
+        print(f"[gpu] oracle variant {v}: {spec['n_trajectories']} trajectories", flush=True)
+        o = oracle_variant(tmp_path, bnd, text, v, spec["n_trajectories"], f"b{v}")
+        assert_raw_equal(gpu_raw(tmp_path, results[v], f"b{v}"), o, f"variant {v}")
+
+
+@pytest.mark.parametrize("cells", [-1, 64, 256, 4096])
+def test_lds_cell_cache_layouts_agree(tmp_path, b50, cells):
+    """The per-wave LDS window-cell cache (none, a small one where most cells collide
+    and go to HBM, and larger ones) changes no sum."""
+    bnd, cfg = b50
+    prog = program(bnd, cfg)
+    variants = netgen.batched_variants(list(prog.model.param_values), prog.model.info.n_nodes, count=24, seed=9)
+    for v in variants:
+        v["n_trajectories"] = min(v["n_trajectories"], 3000)
+    ref, _ = prog.simulate_batched(variants, block_size=128, traj_per_lane=2, lds_cells=-1, use_graph=False)
+    got, info = prog.simulate_batched(variants, block_size=128, traj_per_lane=2, lds_cells=cells, use_graph=False)
+    assert info.lds_cells == max(cells, 0)
+    for v in range(len(variants)):
+        assert_raw_equal(gpu_raw(tmp_path, got[v], f"c{v}"), gpu_raw(tmp_path, ref[v], f"r{v}"), f"variant {v}")
+
+
+def test_tuned_shape_is_kept_on_disk(tmp_path, b50):
+    """The autotuner's choice (block, trajectories per lane, LDS layout) is written to the
+    tuning cache; a new process with the same model and batch reads it instead of timing."""
+    import json
+    import subprocess
+    import sys
+    from conftest import ROOT
+    bnd, cfg = b50
+    code = ("import json, maboss.gpu_amd as mg; from maboss.gpu_amd import netgen; "
+            f"p = mg.Model.load({bnd!r}, {cfg!r}).compile(); "
+            "vs = netgen.batched_variants(list(p.model.param_values), p.model.info.n_nodes, count=16, seed=4); "
+            "_, i = p.simulate_batched(vs, device_only=True); "
+            "print(json.dumps([i.block_size, i.traj_per_lane, i.lds_cells, i.tuned_cached]))")
+    env = dict(os.environ, MBG_CACHE_DIR=str(tmp_path / "cache"))
+    runs = []
+    for k in range(2):
+        print(f"[gpu] tuning-cache process {k}", flush=True)
+        p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300, cwd=ROOT, env=env)
+        assert p.returncode == 0, p.stderr[-3000:]
+        runs.append(json.loads(p.stdout.strip().splitlines()[-1]))
+    assert runs[0][3] == 0 and runs[1][3] == 1
+    assert runs[0][:3] == runs[1][:3]
+    assert any(f.endswith(".tune") for f in os.listdir(tmp_path / "cache"))
