@@ -327,6 +327,8 @@ typedef struct mbg_result_info
 	                                    summed on the host (windows with more states than a lane holds) */
 	int32_t has_errors;              /* 1: the error columns come from second moments */
 	int32_t pad2;
+	uint64_t window_spills;          /* hashed window mode: cells whose probe passed 256 slots, kept in a
+	                                    spill list and summed with the table's cells after the run */
 } mbg_result_info;
 
 int mbg_result_get_info(const mbg_result* r, mbg_result_info* out);
@@ -362,6 +364,32 @@ int mbg_result_write_csv(const mbg_result* r, const char* prefix, int with_final
 int mbg_result_print(const mbg_result* r, const char* path);
 /* Exact integer dump used by the parity tests (one record per line). */
 int mbg_result_write_raw(const mbg_result* r, const char* path);
+
+/* ---- checkpoint / resume at trajectory-id granularity.  The reference's
+ *      runner keeps its state only between epochs of one process
+ *      (src/simulation_runner.cu:123-161) and saves nothing.  A result's
+ *      statistics are exact integers and trajectory id i always draws the
+ *      same stream, so a run split into id ranges and added back together
+ *      equals the single run bit for bit. ---- */
+/* The result's statistics (window cells, entropy sums, final states, fixed
+ * points, second moments) and identity (model and rate fingerprint, seed,
+ * stream tag, trajectory ids, kernel) to a file, replaced atomically; the
+ * file ends with a hash of its contents. */
+int mbg_result_save(const mbg_result* r, const char* path);
+/* A saved result, for a program of the same model (MBG_E_IO for a damaged or
+ * truncated file, MBG_E_ARG for another model's). */
+int mbg_result_load(const mbg_program* p, const char* path, mbg_result** out);
+/* Add `other` into `into`: the same run identity and adjacent trajectory-id
+ * ranges (MBG_E_ARG otherwise); `into` then covers both. */
+int mbg_result_merge(mbg_result* into, const mbg_result* other);
+/* Simulate ids [first_id, first_id + n_trajectories) in chunks of `chunk`
+ * ids (0: one chunk), saving the merged statistics to `path` after each
+ * chunk.  When `path` already holds a checkpoint of the same run identity
+ * and first id, the run resumes after its last id (a checkpoint that covers
+ * every id returns at once).  Honours MBG_SIM_ERRORS; not
+ * MBG_SIM_DEVICE_ONLY. */
+int mbg_simulate_checkpointed(mbg_program* p, const mbg_sim_params* params, uint64_t chunk, const char* path,
+							  void* stream, mbg_result** out);
 
 #ifdef __cplusplus
 }

@@ -152,6 +152,18 @@ class Program:
         ErrorTH / ErrorProba columns from exact per-trajectory second moments
         (``MBG_SIM_ERRORS``: a replay of the trajectories after the run); ``moment_states``
         (8 or 1) is the replay's per-lane state buffer, which changes no result."""
+        p, keep = self._params(n_trajectories, first_id, seed, variant, max_steps, window_mode, final_mode,
+                               window_capacity, final_capacity, fixed_capacity, block_size, blocks_per_cu,
+                               normalization, device_only, kernel, rates, errors, moment_states)
+        h = ctypes.c_void_p()
+        s = ctypes.c_void_p(stream) if stream else None
+        entry = lib().mbg_simulate if wait else lib().mbg_simulate_async
+        check(entry(self._h, ctypes.byref(p), s, ctypes.byref(h)))
+        return Result(h, self.model)
+
+    def _params(self, n_trajectories, first_id, seed, variant, max_steps, window_mode, final_mode, window_capacity,
+                final_capacity, fixed_capacity, block_size, blocks_per_cu, normalization, device_only, kernel, rates,
+                errors, moment_states):
         p = SimParams()
         lib().mbg_sim_params_default(ctypes.byref(p))
         p.n_trajectories = n_trajectories
@@ -176,10 +188,27 @@ class Program:
             if keep.shape != (self.model.info.n_params,):
                 raise ValueError(f"rates must have {self.model.info.n_params} entries")
             p.rates = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        return p, keep
+
+    def simulate_checkpointed(self, path: str, chunk: int = 0, n_trajectories: int = -1, first_id: int = 0,
+                              seed: int = -1, variant: int = 0, max_steps: int = 0, kernel: str = "auto", rates=None,
+                              errors: bool = False, stream=None) -> "Result":
+        """Simulate in chunks of ``chunk`` trajectory ids, saving the merged statistics to
+        ``path`` after each (``mbg_simulate_checkpointed``); started again with the same
+        ``path`` it resumes after the last saved id.  The result equals one
+        :meth:`simulate` of the whole range bit for bit."""
+        p, keep = self._params(n_trajectories, first_id, seed, variant, max_steps, "auto", "auto", 0, 0, 0, 0, 0,
+                               0.0, False, kernel, rates, errors, 0)
         h = ctypes.c_void_p()
         s = ctypes.c_void_p(stream) if stream else None
-        entry = lib().mbg_simulate if wait else lib().mbg_simulate_async
-        check(entry(self._h, ctypes.byref(p), s, ctypes.byref(h)))
+        check(lib().mbg_simulate_checkpointed(self._h, ctypes.byref(p), chunk, os.fsencode(path), s, ctypes.byref(h)))
+        del keep
+        return Result(h, self.model)
+
+    def load_result(self, path: str) -> "Result":
+        """A result saved with :meth:`Result.save` (``mbg_result_load``)."""
+        h = ctypes.c_void_p()
+        check(lib().mbg_result_load(self._h, os.fsencode(path), ctypes.byref(h)))
         return Result(h, self.model)
 
     def simulate_batched(self, variants, seed: int = -1, max_steps: int = 0, block_size: int = 0,
@@ -321,6 +350,15 @@ class Result:
 
     def write_raw(self, path: str):
         check(lib().mbg_result_write_raw(self._h, os.fsencode(path)))
+
+    def save(self, path: str):
+        """The exact statistics and run identity to a checkpoint file (``mbg_result_save``)."""
+        check(lib().mbg_result_save(self._h, os.fsencode(path)))
+
+    def merge(self, other: "Result") -> "Result":
+        """Add ``other`` (same run, adjacent trajectory ids) into this result."""
+        check(lib().mbg_result_merge(self._h, other._h))
+        return self
 
 
 def simulate(bnd_path: str, cfg_path: str, **kw) -> Result:

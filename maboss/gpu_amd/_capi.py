@@ -38,6 +38,7 @@ EXPORTED = [
     "mbg_result_final_size",
     "mbg_result_final", "mbg_result_fixed_size", "mbg_result_fixed",
     "mbg_result_write_csv", "mbg_result_print", "mbg_result_write_raw",
+    "mbg_result_save", "mbg_result_load", "mbg_result_merge", "mbg_simulate_checkpointed",
 ]
 
 
@@ -139,6 +140,7 @@ class ResultInfo(ctypes.Structure):
         ("window_capacity", ctypes.c_uint64), ("kernel", ctypes.c_int32), ("pad", ctypes.c_int32),
         ("simulate_ms", ctypes.c_double), ("window_pairs", ctypes.c_uint64),
         ("moment_spills", ctypes.c_uint64), ("has_errors", ctypes.c_int32), ("pad2", ctypes.c_int32),
+        ("window_spills", ctypes.c_uint64),
     ]
 
 
@@ -196,6 +198,11 @@ def _declare(lib):
         "mbg_result_write_csv": ([P, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
         "mbg_result_print": ([P, ctypes.c_char_p], ctypes.c_int),
         "mbg_result_write_raw": ([P, ctypes.c_char_p], ctypes.c_int),
+        "mbg_result_save": ([P, ctypes.c_char_p], ctypes.c_int),
+        "mbg_result_load": ([P, ctypes.c_char_p, pp], ctypes.c_int),
+        "mbg_result_merge": ([P, P], ctypes.c_int),
+        "mbg_simulate_checkpointed": ([P, ctypes.POINTER(SimParams), ctypes.c_uint64, ctypes.c_char_p, P, pp],
+                                      ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -21,8 +21,9 @@
 #define MBG_CTR_FINISHED 7       // trajectories ended at max_time or a fixed point
 #define MBG_CTR_TAIL_OVERFLOW 8  // tail table full
 #define MBG_CTR_PAIRS 9          // window pair slots reserved (MBG_HIST_SORT; may exceed the capacity)
+#define MBG_CTR_WIN_SPILL 10     // window cells sent to the spill list (MBG_HIST_HASH; may exceed the capacity)
 #define MBG_PAIR_PAD_KEY 0xffffffffffffffffull // key of reserved slots left unused
-#define MBG_NCOUNTERS 10
+#define MBG_NCOUNTERS 11
 
 #define MBG_EMPTY_KEY 0xffffffffffffffffull
 
@@ -90,6 +91,12 @@ struct mbg_sim_args
 	int fp_lds_off, fp_lds_slots;     // u64 offset, slots of (2 + key words) u32 (power of two; 0: none)
 	int fin_lds_cache_off, fin_lds_cache_slots; // hashed final states: u64 offset, slots {key + 1, count}
 	unsigned long long* counters;   // [MBG_NCOUNTERS]
+	// MBG_HIST_HASH: cells whose probe ran past MBG_WIN_PROBES slots, appended
+	// as (key, value) pairs and sum-reduced with the table's cells after the
+	// run, so a table that fills up loses nothing and needs no second run
+	unsigned long long* win_spill_keys;
+	unsigned long long* win_spill_vals;
+	unsigned long long win_spill_cap;
 	unsigned long long* pair_keys;  // MBG_HIST_SORT: appended cell keys
 	unsigned long long* pair_vals;  //                and their values
 	unsigned long long pair_cap;    //                capacity in pairs

@@ -51,6 +51,40 @@ __device__ __forceinline__ void hash_add(unsigned long long* table, uint64_t mas
 	atomicAdd(overflow, 1ull);
 }
 
+// Probe bound of the window-cell table: a cell that finds neither its key
+// nor an empty slot within this many slots goes to the spill list.
+#define MBG_WIN_PROBES 256
+
+__device__ __forceinline__ void win_spill(const mbg_sim_args& a, uint64_t key, uint64_t q)
+{
+	const unsigned long long i = atomicAdd(a.counters + MBG_CTR_WIN_SPILL, 1ull);
+	if (i < a.win_spill_cap)
+	{
+		a.win_spill_keys[i] = key;
+		a.win_spill_vals[i] = q;
+	}
+	else
+		atomicAdd(a.counters + MBG_CTR_WIN_OVERFLOW, 1ull);
+}
+
+// hash_add for the window-cell table, with the probe bound and the spill list
+__device__ __forceinline__ void hash_add_win(const mbg_sim_args& a, uint64_t key, uint64_t q)
+{
+	uint64_t slot = mix64(key) & a.win_mask;
+	for (uint64_t p = 0; p <= a.win_mask && p < MBG_WIN_PROBES; ++p)
+	{
+		unsigned long long* e = a.win_table + 2 * slot;
+		const unsigned long long k = atomicCAS(e, MBG_EMPTY_KEY, (unsigned long long)key);
+		if (k == MBG_EMPTY_KEY || k == key)
+		{
+			atomicAdd(e + 1, (unsigned long long)q);
+			return;
+		}
+		slot = (slot + 1) & a.win_mask;
+	}
+	win_spill(a, key, q);
+}
+
 // One lane's window-cell insert into the hashed table as a sequence of
 // non-blocking steps, so each memory round trip overlaps the lane's next
 // steps: the slot's key is read first (a key already in the table, about half
@@ -78,8 +112,10 @@ struct hash_inflight
 		state = 1;
 	}
 
-	__device__ __forceinline__ void step(unsigned long long* table, uint64_t mask, unsigned long long* overflow)
+	__device__ __forceinline__ void step(const mbg_sim_args& a)
 	{
+		unsigned long long* const table = a.win_table;
+		const uint64_t mask = a.win_mask;
 		if (state == 0u)
 			return;
 		if (ret == key || (state == 2u && ret == MBG_EMPTY_KEY))
@@ -93,9 +129,9 @@ struct hash_inflight
 			ret = atomicCAS(table + 2 * slot, MBG_EMPTY_KEY, (unsigned long long)key);
 			state = 2;
 		}
-		else if (++probes > mask)
+		else if (++probes > mask || probes >= MBG_WIN_PROBES)
 		{
-			atomicAdd(overflow, 1ull);
+			win_spill(a, key, val);
 			state = 0;
 		}
 		else
@@ -106,10 +142,10 @@ struct hash_inflight
 		}
 	}
 
-	__device__ __forceinline__ void finish(unsigned long long* table, uint64_t mask, unsigned long long* overflow)
+	__device__ __forceinline__ void finish(const mbg_sim_args& a)
 	{
 		while (state != 0u)
-			step(table, mask, overflow);
+			step(a);
 	}
 };
 
@@ -408,7 +444,7 @@ struct lane_hist_t
 			append_pair_chunked(lds + a.pair_lds_off + 4 * (threadIdx.x >> 6), a.pair_keys, a.pair_vals, a.pair_cap,
 								a.counters + MBG_CTR_PAIRS, (uint64_t)a.pair_chunk, cell, q);
 		else
-			hash_add(a.win_table, a.win_mask, cell, q, a.counters + MBG_CTR_WIN_OVERFLOW);
+			hash_add_win(a, cell, q);
 	}
 
 	__device__ void ent(int w, uint64_t q) const

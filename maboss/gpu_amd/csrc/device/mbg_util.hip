@@ -73,7 +73,7 @@ extern "C" __global__ void __launch_bounds__(256) mbg_expand_tails(mbg_sim_args 
 			else if (a.win_mode == MBG_HIST_SORT)
 				append_pair(a.pair_keys, a.pair_vals, a.pair_cap, a.counters + MBG_CTR_PAIRS, cell, q * cnt);
 			else
-				hash_add(a.win_table, a.win_mask, cell, q * cnt, a.counters + MBG_CTR_WIN_OVERFLOW);
+				hash_add_win(a, cell, q * cnt);
 		}
 	}
 }

@@ -132,17 +132,15 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 	const bool staged = (MBG_WIN_MODE == MBG_HIST_SORT || hashed) && a.pair_stage;
 	pair_stage_reg stage { lds + a.pair_stage_off + 2 * a.pair_stage * wave, a.pair_stage };
 	hash_inflight ins;
-	unsigned long long* const win_over = a.counters + MBG_CTR_WIN_OVERFLOW;
 	struct
 	{
 		const mbg_sim_args& a;
 		unsigned long long* cur;
 		hash_inflight& ins;
-		unsigned long long* over;
 		__device__ void step()
 		{
 			if (MBG_WIN_MODE == MBG_HIST_HASH)
-				ins.step(a.win_table, a.win_mask, over);
+				ins.step(a);
 		}
 		__device__ bool ready() const { return MBG_WIN_MODE != MBG_HIST_HASH || !ins.busy(); }
 		__device__ void take(uint64_t key, uint64_t q)
@@ -153,7 +151,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 				append_pair_chunked(cur, a.pair_keys, a.pair_vals, a.pair_cap, a.counters + MBG_CTR_PAIRS,
 									(uint64_t)a.pair_chunk, key, q);
 		}
-	} sink { a, cur, ins, win_over };
+	} sink { a, cur, ins };
 	auto emit = [&](bool has, int w, uint64_t o, uint64_t q) {
 		if (staged)
 			stage.add(has, ((uint64_t)w << a.n_obs) | o, q, sink);
@@ -260,7 +258,7 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 	if (staged)
 		stage.drain(sink);
 	if (hashed)
-		ins.finish(a.win_table, a.win_mask, win_over);
+		ins.finish(a);
 	if (MBG_WIN_MODE == MBG_HIST_SORT)
 		pad_pair_chunk(lds + a.pair_lds_off + 4 * (threadIdx.x >> 6), a.pair_keys, a.pair_vals, a.pair_cap);
 

@@ -526,6 +526,7 @@ int mbg_result_get_info(const mbg_result* rr, mbg_result_info* out)
 		out->simulate_ms = r.simulate_ms;
 		out->window_pairs = r.window_pairs;
 		out->moment_spills = r.moment_spills;
+		out->window_spills = r.window_spills;
 		out->has_errors = r.has_moments ? 1 : 0;
 		out->window_mode = r.window_mode;
 		out->final_mode = r.final_mode;
@@ -601,6 +602,48 @@ int mbg_result_entropies(const mbg_result* rr, double* th, double* h)
 			if (h)
 				h[w] = rr->r->h(w);
 		}
+	});
+}
+
+int mbg_result_save(const mbg_result* rr, const char* path)
+{
+	return guarded([&] {
+		if (!rr || !path)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		mbg::save_result(*rr->r, path);
+	});
+}
+
+int mbg_result_load(const mbg_program* p, const char* path, mbg_result** out)
+{
+	return guarded([&] {
+		if (!p || !path || !out)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		auto rr = std::make_unique<mbg_result>();
+		rr->r = mbg::load_result(*p->p, path);
+		*out = rr.release();
+	});
+}
+
+int mbg_result_merge(mbg_result* into, const mbg_result* other)
+{
+	return guarded([&] {
+		if (!into || !other || into == other)
+			throw mbg::error(mbg::MBG_E_ARG, "bad argument");
+		mbg::merge_into(*into->r, *other->r);
+	});
+}
+
+int mbg_simulate_checkpointed(mbg_program* p, const mbg_sim_params* params, uint64_t chunk, const char* path,
+							  void* stream, mbg_result** out)
+{
+	return guarded([&] {
+		if (!p || !path || !out)
+			throw mbg::error(mbg::MBG_E_ARG, "null argument");
+		const mbg::sim_params sp = to_sim_params(p, params);
+		auto rr = std::make_unique<mbg_result>();
+		rr->r = mbg::simulate_checkpointed(*p->p, sp, chunk, path, (hipStream_t)stream);
+		*out = rr.release();
 	});
 }
 

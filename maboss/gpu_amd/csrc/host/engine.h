@@ -171,6 +171,7 @@ struct program
 	// waits for every copy in flight (before buffers are released)
 	void drain_copies();
 	device_buffer b_tail;
+	device_buffer b_win_spill; // hashed window mode: spilled cells, keys then values
 
 	// batched form: launch shape chosen by the autotuner, keyed by the batch
 	struct tuned
@@ -220,6 +221,9 @@ struct program
 };
 
 std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const gen_options& opt, bool no_cache);
+// The memory budget of a run's tables: a quarter of the device memory free
+// when it starts (8 GiB when the runtime cannot tell).
+uint64_t device_budget();
 // Directory of the on-disk caches (code objects, tuned launch shapes):
 // $MBG_CACHE_DIR, else ~/.cache/maboss_gpu.
 std::string code_cache_dir();
@@ -259,6 +263,12 @@ struct result
 	float tick = 0, max_time = 0;
 	double norm = 0;
 	uint64_t n_traj = 0;
+	// identity of the run, for checkpoints: trajectories [first_id, first_id
+	// + n_traj) of the stream (seed, variant) of the model and rates whose
+	// fingerprint this is; user_norm: the probabilities divide by norm, not n_traj
+	uint64_t first_id = 0, seed = 0, fingerprint = 0;
+	uint32_t variant = 0;
+	bool user_norm = false;
 	std::vector<std::string> names;
 	std::vector<int> observed;
 
@@ -274,6 +284,7 @@ struct result
 	uint64_t transitions = 0, finished = 0, truncated = 0;
 	double kernel_ms = 0, total_ms = 0, simulate_ms = 0;
 	uint64_t window_pairs = 0;
+	uint64_t window_spills = 0; // hashed window cells that went through the spill list
 	int window_mode = 0, final_mode = 0;
 	int kernel = 0; // 0 lane, 1 wave
 	uint64_t window_capacity = 0;
@@ -315,6 +326,18 @@ struct result
 };
 
 std::unique_ptr<result> simulate(program& p, const sim_params& sp, hipStream_t stream);
+
+// Checkpoints (trajectory-id granularity): a result's exact statistics and
+// identity to a file and back; two results of adjacent id ranges of the same
+// run added into one; and a run in chunks of ids that saves the merged
+// statistics after each chunk and, started again with the same file,
+// resumes after its last id.
+uint64_t run_fingerprint(const program& p, const sim_params& sp);
+void save_result(const result& r, const std::string& path);
+std::unique_ptr<result> load_result(const program& p, const std::string& path);
+void merge_into(result& into, const result& other);
+std::unique_ptr<result> simulate_checkpointed(program& p, const sim_params& sp, uint64_t chunk,
+											  const std::string& path, hipStream_t stream);
 // The second moments of a finished run (sim_params::errors): replays its
 // trajectories with the moments kernel and fills r.msq / r.mesq.  Lane
 // kernel, dense windows (windows << n_obs <= 2^24).

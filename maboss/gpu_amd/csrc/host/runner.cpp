@@ -98,9 +98,15 @@ int cache_slots(size_t budget_left, size_t slot_bytes, int limit, int min_slots)
 	return n >= min_slots ? n : 0;
 }
 
-plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int windows, int n_obs, bool wave_kernel)
+plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int windows, int n_obs, bool wave_kernel,
+			   uint64_t budget)
 {
 	plan p;
+	// tables in HBM are sized from the run's memory budget (a quarter of the
+	// device memory free when the run starts): a table slot costs its 16
+	// bytes plus the compacted and sorted copies of its cell (32 bytes)
+	const uint64_t slot_budget = std::max<uint64_t>(1ull << 16, budget / 48);
+	const uint64_t dense_limit = std::min<uint64_t>(k_dense_limit, std::max<uint64_t>(budget, 1ull << 20));
 	const uint64_t win_dense_bytes = n_obs <= 40 ? ((uint64_t)windows << n_obs) * 8 : ~0ull;
 	const uint64_t fin_dense_bytes = n_obs <= 40 ? (1ull << n_obs) * 8 : ~0ull;
 	size_t lds = 0;
@@ -116,11 +122,11 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 	// the LDS histogram leaves room for the per-wave tables and caches below
 	if (p.win_mode < 0)
 		p.win_mode = win_dense_bytes <= 32 * 1024 && win_dense_bytes + 2048 <= lds_left(lds) ? MBG_HIST_LDS
-					 : win_dense_bytes <= k_dense_limit ? MBG_HIST_DENSE
-														: MBG_HIST_HASH;
+					 : win_dense_bytes <= dense_limit ? MBG_HIST_DENSE
+													   : MBG_HIST_HASH;
 	if (p.win_mode < 0 || p.win_mode > MBG_HIST_SORT)
 		throw error(MBG_E_ARG, "unknown window mode");
-	if ((p.win_mode == MBG_HIST_DENSE || p.win_mode == MBG_HIST_LDS) && win_dense_bytes > k_dense_limit)
+	if ((p.win_mode == MBG_HIST_DENSE || p.win_mode == MBG_HIST_LDS) && win_dense_bytes > std::max(dense_limit, budget))
 		throw error(MBG_E_LIMIT, "dense window histogram too large; use the hashed or sorted mode");
 	if (p.win_mode == MBG_HIST_LDS)
 	{
@@ -137,7 +143,7 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 		p.fin_mode = fin_dense_bytes <= lds_left(lds) ? MBG_HIST_LDS
 					 : fin_dense_bytes <= (1ull << 30)	 ? MBG_HIST_DENSE
 														 : MBG_HIST_HASH;
-	if (p.fin_mode != MBG_HIST_HASH && fin_dense_bytes > k_dense_limit)
+	if (p.fin_mode != MBG_HIST_HASH && fin_dense_bytes > std::max(dense_limit, budget))
 		throw error(MBG_E_LIMIT, "dense final-state histogram too large; use the hashed mode");
 	if (p.fin_mode == MBG_HIST_LDS)
 	{
@@ -201,7 +207,7 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 	if (p.win_mode == MBG_HIST_HASH)
 		p.win_cap = sp.window_capacity ? next_pow2(sp.window_capacity)
 									   : next_pow2(std::min<uint64_t>(std::max<uint64_t>(n_traj * windows, 1 << 16),
-																	   1ull << 28));
+																	   slot_budget));
 	else if (p.win_mode != MBG_HIST_SORT) // the sorted mode's pair capacity is the runner's
 		p.win_cells = (uint64_t)windows << n_obs;
 	if (p.fin_mode == MBG_HIST_HASH)
@@ -209,12 +215,13 @@ plan make_plan(const model& m, const sim_params& sp, uint64_t n_traj, int window
 		if (n_obs > 63)
 			throw error(MBG_E_LIMIT, "more than 63 observed nodes");
 		p.fin_cap = sp.final_capacity ? next_pow2(sp.final_capacity)
-									  : next_pow2(std::min<uint64_t>(std::max<uint64_t>(2 * n_traj, 1 << 12), 1ull << 27));
+									  : next_pow2(std::min<uint64_t>(std::max<uint64_t>(2 * n_traj, 1 << 12), slot_budget));
 	}
 	else
 		p.fin_cells = 1ull << n_obs;
 	p.fp_cap = sp.fixed_capacity ? next_pow2(sp.fixed_capacity)
-								 : next_pow2(std::min<uint64_t>(std::max<uint64_t>(2 * n_traj, 1 << 12), 1ull << 24));
+								 : next_pow2(std::min<uint64_t>(std::max<uint64_t>(2 * n_traj, 1 << 12),
+																std::max<uint64_t>(1ull << 16, budget / (8 * (2 + (uint64_t)m.n_words())))));
 	(void)m;
 	return p;
 }
@@ -270,9 +277,18 @@ size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 } // namespace
 
+uint64_t device_budget()
+{
+	size_t free_b = 0, total_b = 0;
+	if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || !free_b)
+		return 8ull << 30;
+	return (uint64_t)free_b / 4;
+}
+
 void default_storage_modes(const model& m, int& win_mode, int& fin_mode)
 {
-	const plan pl = make_plan(m, sim_params {}, (uint64_t)m.sample_count(), m.windows(), m.n_observed(), false);
+	const plan pl = make_plan(m, sim_params {}, (uint64_t)m.sample_count(), m.windows(), m.n_observed(), false,
+							  device_budget());
 	win_mode = pl.win_mode;
 	fin_mode = pl.fin_mode;
 }
@@ -307,6 +323,11 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 	res->max_time = m.max_time();
 	res->n_traj = n_traj;
 	res->norm = sp.normalization > 0 ? sp.normalization : (double)n_traj;
+	res->user_norm = sp.normalization > 0;
+	res->first_id = sp.first_id;
+	res->seed = seed;
+	res->variant = sp.variant;
+	res->fingerprint = run_fingerprint(p, sp);
 	for (auto& n : m.nodes)
 		res->names.push_back(n.name);
 	res->observed = m.observed_nodes();
@@ -324,7 +345,7 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 	if (!sp.rates.empty() && sp.rates.size() != p.gen.param_names.size())
 		throw error(MBG_E_ARG, "expected " + std::to_string(p.gen.param_names.size()) + " rates");
 	res->kernel = use_wave ? 1 : 0;
-	plan pl = make_plan(m, sp, n_traj, windows, n_obs, use_wave);
+	plan pl = make_plan(m, sp, n_traj, windows, n_obs, use_wave, device_budget());
 	if (probs && (use_wave || (pl.win_mode != MBG_HIST_DENSE && pl.win_mode != MBG_HIST_LDS) || n_obs > 24))
 		throw error(MBG_E_LIMIT, "device window probabilities need the lane kernel and dense window cells "
 								 "(at most 24 observed nodes)");
@@ -405,6 +426,8 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 											  : std::max<uint64_t>((uint64_t)windows, 16);
 		pair_cap = sp.window_capacity ? sp.window_capacity : std::max<uint64_t>(n_traj * per, 1 << 16);
 	}
+	// hashed window mode: the spill list of cells past the probe bound
+	uint64_t spill_cap = pl.win_mode == MBG_HIST_HASH ? std::max<uint64_t>(1 << 16, pl.win_cap / 8) : 0;
 	std::vector<float> istate;
 	for (auto& n : m.nodes)
 		istate.push_back(n.istate);
@@ -449,6 +472,7 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 			win_vals = (unsigned long long*)p.b_win_vals.ensure(pl.win_cells * 8);
 			use(p.cl_win_vals, win_vals, pl.win_cells * 8, [&] { memset0(win_vals, pl.win_cells * 8); });
 		}
+		auto* win_spill = spill_cap ? (unsigned long long*)p.b_win_spill.ensure(spill_cap * 16) : nullptr;
 		if (pl.fin_mode == MBG_HIST_HASH)
 		{
 			fin_keys = (unsigned long long*)p.b_fin_keys.ensure(pl.fin_cap * 16);
@@ -540,6 +564,9 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 		a.win_vals = win_vals;
 		a.win_table = win_keys;
 		a.win_mask = pl.win_cap ? pl.win_cap - 1 : 0;
+		a.win_spill_keys = win_spill;
+		a.win_spill_vals = win_spill ? win_spill + spill_cap : nullptr;
+		a.win_spill_cap = spill_cap;
 		a.win_ent = ent;
 		a.fin_vals = fin_vals;
 		a.fin_table = fin_keys;
@@ -676,8 +703,9 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 			use(slot.cl_cnt, cnt, 4 * 8, [&] { memset0(cnt, 4 * 8); });
 			if (pl.win_mode == MBG_HIST_HASH)
 			{
-				auto* tk = (unsigned long long*)slot.pk.ensure(pl.win_cap * 8);
-				auto* tv = (unsigned long long*)slot.pv.ensure(pl.win_cap * 8);
+				// room after the table's cells for the spilled ones
+				auto* tk = (unsigned long long*)slot.pk.ensure((pl.win_cap + spill_cap) * 8);
+				auto* tv = (unsigned long long*)slot.pv.ensure((pl.win_cap + spill_cap) * 8);
 				table_compact(p, win_keys, pl.win_cap, tk, tv, pl.win_cap, cnt + 0, stream);
 			}
 			else if (pl.win_mode != MBG_HIST_SORT)
@@ -727,6 +755,9 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 		if (pl.win_mode == MBG_HIST_SORT && n_traj)
 			p.pairs_per_traj = (n_pairs + n_traj - 1) / n_traj;
 
+		// the window table's own overflow is the spill list's: cells past the
+		// probe bound that found the list full
+		const uint64_t n_spill = ctr[MBG_CTR_WIN_SPILL];
 		const bool win_over = ctr[MBG_CTR_WIN_OVERFLOW] != 0, fin_over = ctr[MBG_CTR_FIN_OVERFLOW] != 0,
 				   fp_over = ctr[MBG_CTR_FP_OVERFLOW] != 0, tail_over = ctr[MBG_CTR_TAIL_OVERFLOW] != 0;
 		if (ctr[MBG_CTR_FP_TIMEOUT])
@@ -748,9 +779,11 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 			}
 			if (win_over)
 			{
-				if (sp.window_capacity)
-					throw error(MBG_E_OVERFLOW, "window hash table overflow; raise window_capacity");
-				pl.win_cap *= 2;
+				// the next launch's list takes every spilled cell; the table grows
+				// unless the caller fixed its capacity
+				spill_cap = next_pow2(n_spill + n_spill / 8 + 1);
+				if (!sp.window_capacity)
+					pl.win_cap *= 2;
 			}
 			if (fin_over)
 			{
@@ -768,6 +801,7 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 		}
 
 		res->window_pairs = pl.win_mode == MBG_HIST_SORT ? n_pairs : 0;
+		res->window_spills = n_spill;
 		res->transitions = ctr[MBG_CTR_TRANSITIONS];
 		res->finished = ctr[MBG_CTR_FINISHED];
 		res->truncated = ctr[MBG_CTR_TRUNCATED];
@@ -791,7 +825,22 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 		const int key_bits = n_obs + bits_for((uint64_t)windows - 1);
 		uint64_t n_cells = mb[16], n_fin = mb[17], n_fp = mb[18];
 		unsigned long long *wk = (unsigned long long*)slot.pk.ptr, *wv = (unsigned long long*)slot.pv.ptr;
-		if (pl.win_mode == MBG_HIST_HASH && n_cells)
+		if (pl.win_mode == MBG_HIST_HASH && n_spill)
+		{
+			// the spilled cells after the table's, sorted together and equal keys summed
+			hip_check(hipMemcpyAsync(wk + n_cells, win_spill, n_spill * 8, hipMemcpyDeviceToDevice, stream),
+					  "hipMemcpyAsync");
+			hip_check(hipMemcpyAsync(wv + n_cells, win_spill + spill_cap, n_spill * 8, hipMemcpyDeviceToDevice, stream),
+					  "hipMemcpyAsync");
+			auto* ka = (unsigned long long*)slot.pka.ensure((n_cells + n_spill) * 8);
+			auto* va = (unsigned long long*)slot.pva.ensure((n_cells + n_spill) * 8);
+			sort_reduce_pairs(p.b_sort_tmp, wk, wv, ka, va, n_cells + n_spill, key_bits, cnt + 3, stream, &wk, &wv);
+			hip_check(hipMemcpyAsync(mb + 19, cnt + 3, 8, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+			hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+			n_cells = mb[19];
+			memset0(cnt + 3, 8); // the slot's counts stay cleared for its next run
+		}
+		else if (pl.win_mode == MBG_HIST_HASH && n_cells)
 		{
 			auto* ka = (unsigned long long*)slot.pka.ensure(n_cells * 8);
 			auto* va = (unsigned long long*)slot.pva.ensure(n_cells * 8);

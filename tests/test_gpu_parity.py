@@ -234,13 +234,20 @@ def test_sorted_pairs_capacity(tmp_path, wl):
     assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=2000))
 
 
-def test_hash_growth_and_overflow_error(tmp_path, wl):
+def test_hash_table_full_spills_instead_of_rerunning(tmp_path, wl):
+    """The hashed window table with the automatic capacity spills nothing; a fixed
+    capacity far below the run's cells sends the cells past the probe bound to the
+    spill list (growing the list, not the caller's table, if it fills) and the
+    statistics stay exact."""
     bnd, cfg = wl["plain_100"]
+    want = oracle_raw(tmp_path, bnd, cfg, n=2000)
     a, r = gpu_raw(tmp_path, bnd, cfg, "a", n_trajectories=2000, window_mode="hash", window_capacity=0)
-    with pytest.raises(mg.MbgError) as e:
-        program(bnd, cfg).simulate(n_trajectories=2000, window_mode="hash", window_capacity=1024)
-    assert e.value.code == 8
-    assert_same(a, oracle_raw(tmp_path, bnd, cfg, n=2000))
+    assert r.info.window_spills == 0
+    assert_same(a, want)
+    for cap in (1 << 14, 1024):
+        b, rb = gpu_raw(tmp_path, bnd, cfg, f"b{cap}", n_trajectories=2000, window_mode="hash", window_capacity=cap)
+        assert rb.info.window_capacity == cap and rb.info.window_spills > rb.info.n_window_cells // 2
+        assert_same(b, want)
 
 
 def test_edge_cases(tmp_path):
