@@ -223,7 +223,7 @@ typedef struct mbg_batch_info
 	int32_t lds_cells;        /* the launch's LDS cell cache slots per wave (0: none) */
 	int32_t tuned_cached;     /* 1: the launch shape came from the on-disk tuning cache, which keeps
 	                             the autotuner's choice per generated model code, batch shape and
-	                             device in $MBG_CACHE_DIR (else ~/.cache/maboss_gpu) */
+	                             device in $MBG_CACHE_DIR (else ~/.cache/maboss_gpu_amd) */
 } mbg_batch_info;
 
 void mbg_batch_params_default(mbg_batch_params* p);

@@ -225,7 +225,7 @@ std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const g
 // when it starts (8 GiB when the runtime cannot tell).
 uint64_t device_budget();
 // Directory of the on-disk caches (code objects, tuned launch shapes):
-// $MBG_CACHE_DIR, else ~/.cache/maboss_gpu.
+// $MBG_CACHE_DIR, else ~/.cache/maboss_gpu_amd.
 std::string code_cache_dir();
 
 // Largest state-word count served by the lane-per-trajectory kernels (512
