@@ -135,7 +135,10 @@ typedef struct mbg_sim_params
 	int32_t moment_states;   /* with MBG_SIM_ERRORS: observed states a lane holds per window in the
 	                            second-moment replay, 8 (0) or 1; more states in a window go to a
 	                            host-side list, so both give the same sums */
-	int32_t reserved[3];
+	int32_t refill_lanes;    /* lane kernel: idle lanes a wave gathers before it starts new
+	                            trajectories (one round of initial-state draws serves up to
+	                            64 / ceil(N/4)); 0: min(that, 8).  Changes no result */
+	int32_t reserved[2];
 } mbg_sim_params;
 
 /* Keep the statistics in device memory and return only the counters and
@@ -252,7 +255,10 @@ typedef struct mbg_grad_params
 	uint32_t max_steps;       /* 0: default cap */
 	int32_t block_size;       /* 0: from the LDS budget (64..256) */
 	int32_t blocks_per_cu;    /* 0: occupancy limit */
-	int32_t reserved[6];
+	int32_t fold_lanes;       /* ended trajectories a wave gathers before folding them into the
+	                             lanes' running totals (1-64); 0: 8.  Changes no result beyond
+	                             the order of double additions */
+	int32_t reserved[5];
 } mbg_grad_params;
 
 typedef struct mbg_grad_info

@@ -143,7 +143,8 @@ class Program:
                  window_capacity: int = 0, final_capacity: int = 0, fixed_capacity: int = 0,
                  block_size: int = 0, blocks_per_cu: int = 0, normalization: float = 0.0,
                  device_only: bool = False, kernel: str = "auto", rates=None, stream=None,
-                 wait: bool = True, errors: bool = False, moment_states: int = 0) -> "Result":
+                 wait: bool = True, errors: bool = False, moment_states: int = 0,
+                 refill_lanes: int = 0) -> "Result":
         """Simulate; ``rates`` (one float per ``Model.param_names``) replaces the model's
         parameter values at run time (lane kernel).  With ``wait=False`` the call returns
         once the run is enqueued (``mbg_simulate_async``): the statistics reach the host on
@@ -155,6 +156,7 @@ class Program:
         p, keep = self._params(n_trajectories, first_id, seed, variant, max_steps, window_mode, final_mode,
                                window_capacity, final_capacity, fixed_capacity, block_size, blocks_per_cu,
                                normalization, device_only, kernel, rates, errors, moment_states)
+        p.refill_lanes = refill_lanes
         h = ctypes.c_void_p()
         s = ctypes.c_void_p(stream) if stream else None
         entry = lib().mbg_simulate if wait else lib().mbg_simulate_async

@@ -80,7 +80,7 @@ class SimParams(ctypes.Structure):
         ("block_size", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
         ("normalization", ctypes.c_double), ("flags", ctypes.c_int32), ("kernel", ctypes.c_int32),
         ("rates", ctypes.POINTER(ctypes.c_float)), ("moment_states", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("refill_lanes", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -115,7 +115,7 @@ class GradParams(ctypes.Structure):
         ("n_trajectories", ctypes.c_int64), ("first_id", ctypes.c_uint64),
         ("seed", ctypes.c_int64), ("variant", ctypes.c_uint32),
         ("max_steps", ctypes.c_uint32), ("block_size", ctypes.c_int32),
-        ("blocks_per_cu", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6),
+        ("blocks_per_cu", ctypes.c_int32), ("fold_lanes", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5),
     ]
 
 

@@ -46,12 +46,11 @@
 #define MBG_GRAD_WAVES 1
 #endif
 
-// Trajectories of a wave that have ended wait, without stepping, until this
-// many of the wave's lanes have ended (or no lane is still running), and are
-// then folded into the lanes' running totals together: the fold touches
-// every parameter, so doing it for one lane at a time would run it on most
-// steps with one or two lanes active.
-#define MBG_GRAD_FOLD_LANES 16
+// Trajectories of a wave that have ended wait, without stepping, until
+// a.fold_lanes of the wave's lanes have ended (or no lane is still running),
+// and are then folded into the lanes' running totals together: the fold
+// touches every parameter, so doing it for one lane at a time would run it on
+// most steps with one or two lanes active.
 
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MBG_GRAD_WAVES)))
 simulate_grad(mbg_grad_args a)
@@ -115,7 +114,7 @@ simulate_grad(mbg_grad_args a)
 	{
 		// fold the ended trajectories: Gtot * (n_j / theta_j - T_j) joins R_j
 		const uint64_t wm = __ballot(waiting);
-		if (wm && (__popcll(wm) >= MBG_GRAD_FOLD_LANES || __ballot(active) == 0ull))
+		if (wm && (__popcll(wm) >= a.fold_lanes || __ballot(active) == 0ull))
 		{
 			if (waiting)
 			{

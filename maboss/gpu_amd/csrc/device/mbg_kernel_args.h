@@ -94,6 +94,8 @@ struct mbg_sim_args
 	// MBG_HIST_HASH: cells whose probe ran past MBG_WIN_PROBES slots, appended
 	// as (key, value) pairs and sum-reduced with the table's cells after the
 	// run, so a table that fills up loses nothing and needs no second run
+	int refill_lanes;               // lane kernel: idle lanes a wave gathers before it refills them
+	int pad_refill;
 	unsigned long long* win_spill_keys;
 	unsigned long long* win_spill_vals;
 	unsigned long long win_spill_cap;
@@ -172,6 +174,8 @@ struct mbg_grad_args
 	float max_time, tick;
 	int g_lds;         // 1: the upstream gradient table is staged in LDS
 	int lds_bytes;     // dynamic LDS of a block
+	int fold_lanes;    // ended trajectories a wave gathers before folding them
+	int pad_fold;
 	const float* istate;
 	mbg_param_view pv; // rates (runtime values), masks all enabled
 	const double* gprob;           // [windows << n_obs] dL/dP

@@ -175,9 +175,12 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 	id_chunks queue;
 	for (;;)
 	{
-		// Refill lanes whose trajectory ended, from the wave's chunk of ids.
+		// Refill lanes whose trajectory ended, from the wave's chunk of ids, in
+		// whole rounds: one round of the initial state's Philox blocks starts
+		// up to 64 / ceil(N/4) trajectories for the cost of one, so the wave
+		// waits until refill_lanes lanes are idle, or none is running.
 		const uint64_t need = __ballot(!active && !exhausted);
-		if (need)
+		if (need && ((uint64_t)__popcll(need) >= (uint64_t)a.refill_lanes || __ballot(active) == 0))
 		{
 			const uint64_t mine = queue.take(need, a.counters + MBG_CTR_QUEUE, a.n_traj, k_id_chunk);
 			bool fresh = false;
@@ -365,8 +368,9 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 
 		for (;;)
 		{
+			// new trajectories in whole initial-state rounds, as in the plain kernel
 			const uint64_t need = __ballot(!active);
-			if (need && next < end)
+			if (need && next < end && ((uint64_t)__popcll(need) >= (uint64_t)a.refill_lanes || __ballot(active) == 0))
 			{
 				const uint64_t mine = next + __popcll(need & ((1ull << lane) - 1ull));
 				bool fresh = false;

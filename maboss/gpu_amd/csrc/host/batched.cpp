@@ -264,6 +264,7 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 	a.fp_table = fp;
 	a.fp_mask = L.fp_cap - 1;
 	a.counters = counters;
+	a.refill_lanes = default_refill_lanes(N);
 	a.bv = (const mbg_batch_variant*)dbase;
 	a.vctr = vctr;
 

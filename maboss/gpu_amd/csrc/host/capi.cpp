@@ -297,6 +297,7 @@ static mbg::sim_params to_sim_params(mbg_program* p, const mbg_sim_params* param
 		sp.device_only = (params->flags & MBG_SIM_DEVICE_ONLY) != 0;
 		sp.errors = (params->flags & MBG_SIM_ERRORS) != 0;
 		sp.moment_states = params->moment_states;
+		sp.refill_lanes = params->refill_lanes;
 		sp.kernel = params->kernel;
 		if (params->rates)
 			sp.rates.assign(params->rates, params->rates + p->p->gen.param_names.size());
@@ -450,6 +451,7 @@ int mbg_backward(mbg_program* p, const mbg_grad_params* params, const float* rat
 			gp.max_steps = params->max_steps;
 			gp.block = params->block_size;
 			gp.per_cu = params->blocks_per_cu;
+			gp.fold_lanes = params->fold_lanes;
 		}
 		mbg::grad_timing gt;
 		mbg::simulate_grad(*p->p, gp, rates, grad_probs, grad_params, (hipStream_t)stream, info ? &gt : nullptr);
@@ -480,6 +482,7 @@ int mbg_backward_noalloc(mbg_program* p, const mbg_grad_params* params, const fl
 			gp.max_steps = params->max_steps;
 			gp.block = params->block_size;
 			gp.per_cu = params->blocks_per_cu;
+			gp.fold_lanes = params->fold_lanes;
 		}
 		void* s = mbg::noalloc_scratch(p->p->device);
 		if (!s)

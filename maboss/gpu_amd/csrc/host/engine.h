@@ -224,6 +224,8 @@ std::unique_ptr<program> compile_program(std::shared_ptr<const model> m, const g
 // The memory budget of a run's tables: a quarter of the device memory free
 // when it starts (8 GiB when the runtime cannot tell).
 uint64_t device_budget();
+// Idle lanes a lane-kernel wave gathers before it starts new trajectories.
+int default_refill_lanes(int n_nodes);
 // Directory of the on-disk caches (code objects, tuned launch shapes):
 // $MBG_CACHE_DIR, else ~/.cache/maboss_gpu_amd.
 std::string code_cache_dir();
@@ -252,6 +254,7 @@ struct sim_params
 	int kernel = -1; // -1 auto, 0 lane per trajectory, 1 wave per trajectory
 	bool device_only = false; // leave the histograms in HBM
 	std::vector<float> rates;  // [n_params] run-time rate values; empty: the model's
+	int refill_lanes = 0;      // lane kernel: idle lanes a wave gathers before refilling (0: automatic)
 	bool errors = false;       // second moments for ErrorTH / ErrorProba (a replay after the run)
 	int moment_states = 0;     // observed states a lane holds per window in that replay (0: 8)
 };
@@ -416,6 +419,7 @@ struct grad_params
 	uint32_t variant = 0;
 	uint32_t max_steps = 0;
 	int block = 0, per_cu = 0;
+	int fold_lanes = 0; // 0: 16
 };
 
 struct grad_timing

@@ -277,6 +277,17 @@ size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 } // namespace
 
+int default_refill_lanes(int n_nodes)
+{
+	// a refill round initialises 64 / ceil(N/4) trajectories; wait for 4 to 8
+	// idle lanes: waiting for m costs about (m - 1) / 128 of the wave's work.
+	// Measured on MI355X: grad_30's simulate 3.27 -> 2.49 ms at 8, plain_100
+	// 10.18 -> 9.76 ms at 4, no gain past 8 and a loss at 16 on plain_100.
+	const int blocks = (n_nodes + 3) / 4;
+	const int per_round = blocks <= 64 ? 64 / blocks : 1;
+	return std::min(std::max(per_round, 4), 8);
+}
+
 uint64_t device_budget()
 {
 	size_t free_b = 0, total_b = 0;
@@ -549,6 +560,7 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 		a.n_nodes = m.n_nodes();
 		a.n_obs = n_obs;
 		a.windows = windows;
+		a.refill_lanes = sp.refill_lanes > 0 ? sp.refill_lanes : default_refill_lanes(m.n_nodes());
 		a.discrete = m.discrete_time();
 		a.max_time = m.max_time();
 		a.tick = m.time_tick();
