@@ -156,7 +156,8 @@ struct mbg_moments_args
 	unsigned long long n_traj, id_base;
 	unsigned int seed_lo, seed_hi, variant, max_steps;
 	int n_nodes, n_obs, windows, discrete;
-	float max_time, tick, qscale, pad;
+	float max_time, tick, qscale;
+	int refill_lanes;               // idle lanes a wave gathers before it starts new trajectories
 	const float* istate;
 	mbg_param_view pv;              // rates (run-time values) and rate * log2(rate), masks all enabled
 	unsigned long long* sq;         // [windows << n_obs][2]: sum of P^2, low and high words

@@ -146,8 +146,9 @@ extern "C" __global__ void __launch_bounds__(256) simulate_moments(mbg_moments_a
 	mbg_dev::id_chunks queue;
 	for (;;)
 	{
+		// new trajectories in whole initial-state rounds, as in the simulate kernel
 		const uint64_t need = __ballot(!active && !exhausted);
-		if (need)
+		if (need && ((uint64_t)__popcll(need) >= (uint64_t)a.refill_lanes || __ballot(active) == 0))
 		{
 			const uint64_t mine = queue.take(need, a.counters + MBG_MOM_CTR_QUEUE, a.n_traj, 64);
 			bool fresh = false;

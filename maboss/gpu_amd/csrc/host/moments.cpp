@@ -145,6 +145,7 @@ void run_moments(program& p, const sim_params& sp, hipStream_t stream, result& r
 		a.max_time = m.max_time();
 		a.tick = m.time_tick();
 		a.qscale = a.discrete ? 0x1p32f : 0x1p32f / a.tick;
+		a.refill_lanes = default_refill_lanes(N);
 		a.istate = (const float*)(ws + o_ist);
 		a.pv = view;
 		a.sq = (unsigned long long*)(ws + o_sq);
