@@ -104,7 +104,7 @@ extern "C" __device__ void mbg_eval(const uint32_t* __restrict__ s, float u2, mb
 extern "C" __device__ void mbg_eval_p_setup(const mbg_param_view* __restrict__ pv);
 // Rates read from runtime tables (batched variants, gradients).
 // en (may be null) receives the W words of enabled bits: bit i set when node
-// i's logic differs from its value (before the mutant masks).
+// i's logic differs from its value and up_en / dn_en allow that direction.
 extern "C" __device__ void mbg_eval_p(const uint32_t* __restrict__ s, float u2, const mbg_param_view* __restrict__ pv,
 									  mbg_eval_t* __restrict__ out, uint32_t* __restrict__ en);
 // Parallel bit-extract of the observed nodes, in node order (bit j of the

@@ -439,9 +439,6 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 	// the selected rate r (and l) of a fast-form node from the masks xm (x)
 	// and msk (enabled), branch-free: same value as (x == L) ? 0 : (x ? D : U)
 	auto select_rate = [&](emitter& em, int i, bool with_l) {
-		const int w = i / 32, b = i % 32;
-		if (param)
-			os << "\t\tmsk &= 0u - (((ue" << w << " ^ ((ue" << w << " ^ de" << w << ") & xm)) >> " << b << "u) & 1u);\n";
 		os << "\t\tconst uint32_t ub = " << bits_of(em, nc[i].U, nc[i].up_p, false)
 		   << ", db = " << bits_of(em, nc[i].D, nc[i].dn_p, false) << ";\n";
 		os << "\t\tr = mbg_u2f((ub ^ ((ub ^ db) & xm)) & msk);\n";
@@ -475,7 +472,9 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 		os << "\tconst uint32_t sw" << w << " = s[" << w << "];\n\tuint32_t E" << w << " = 0u;\n";
 	if (param)
 		for (int w = 0; w < W; w++)
-			os << "\tconst uint32_t ue" << w << " = pv->up_en[" << w << "], de" << w << " = pv->dn_en[" << w << "];\n";
+			os << "\tconst uint32_t ue" << w << " = pv->up_en[" << w << "], de" << w << " = pv->dn_en[" << w
+			   << "];\n\tconst uint32_t al" << w << " = ue" << w << " ^ ((ue" << w << " ^ de" << w << ") & sw" << w
+			   << "); // directions allowed in the current state\n";
 	// with baked rates and log-sum entropy, the count of observed nodes with a
 	// nonzero rate comes from E, x and masks of the nonzero up / down rates
 	const bool nnz_masks = log_sum && !param && all_fast;
@@ -493,11 +492,15 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 		os << "\t{\n\t\tfloat r" << (lw ? ", l" : "") << ";\n";
 		if (nc[i].fast)
 		{
-			os << "\t\tconst uint32_t ew = (sw" << w << " ^ " << em.as_aligned(*n.attr("logic"), b) << ") & " << (1u << b)
-			   << "u;\n";
-			os << "\t\tE" << w << " |= ew;\n";
+			// xv: bit b set when the node is enabled (its logic differs from
+			// its value and, with run-time masks, that direction is allowed);
+			// one value feeds both the enabled word and the mask, so the
+			// compiler does not evaluate the logic twice
+			os << "\t\tuint32_t xv = (sw" << w << " ^ " << em.as_aligned(*n.attr("logic"), b) << ")"
+			   << (param ? " & al" + std::to_string(w) : std::string()) << ";\n\t\tMBG_MASK(xv);\n";
+			os << "\t\tE" << w << " |= xv & " << (1u << b) << "u;\n";
 			os << "\t\tuint32_t xm = 0u - ((sw" << w << " >> " << b << "u) & 1u);\n\t\tMBG_MASK(xm);\n";
-			os << "\t\tuint32_t msk = 0u - (ew >> " << b << "u);\n";
+			os << "\t\tuint32_t msk = 0u - ((xv >> " << b << "u) & 1u);\n";
 			select_rate(em, i, lw);
 		}
 		else
@@ -511,15 +514,14 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 			os << "\t\tS = S + r; T = T + l; nnz += (r != 0.0f) ? 1 : 0;\n";
 		else if (is_obs[i])
 			os << "\t\tS = S + r;\n";
-		// the sums are pinned after every node (up to 40 nodes, whose kernels
-		// run at 4 waves per SIMD with 128 registers) or every 4th: each
+		// the sums and the enabled word are pinned after every 4th node: each
 		// group's terms are added before the next group is computed (no sum
 		// sunk to the end with every term kept live), while the scheduler may
 		// still interleave the group's nodes (with the selection pass below
 		// unpinned: the 50-node batched kernel 8 % faster, the 100-node plain
 		// kernel about 1 %)
-		if (two_pass && (N <= 40 || i % 4 == 3 || i == N - 1))
-			os << "\t\tMBG_PIN4(acc, S, T, nnz);\n";
+		if (two_pass && (i % 4 == 3 || i == N - 1))
+			os << "\t\tMBG_PIN4(acc, S, T, nnz);\n\t\tMBG_OPAQUE(E" << w << ");\n";
 		os << "\t}\n";
 		if (!two_pass)
 			os << "\tconst float c" << i << " = acc;\n";
@@ -536,7 +538,8 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 	os << "\tconst float total = acc;\n";
 	if (param)
 	{
-		// the enabled bits, for callers that need them (the gradient kernel)
+		// the enabled bits (directions a mutant forbids excluded), for callers
+		// that need them (the gradient kernel)
 		os << "\tif (en)\n\t{\n";
 		for (int w = 0; w < W; w++)
 			os << "\t\ten[" << w << "] = E" << w << ";\n";
@@ -597,19 +600,10 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 		static_assert(32 % k_group == 0, "groups must not straddle state words");
 		os << "\tconst int gw = gsel / " << 32 / k_group << ", gb = (gsel % " << 32 / k_group << ") * " << k_group
 		   << ";\n";
+		// (with run-time masks E already excludes the directions a mutant forbids)
 		os << "\tuint32_t ge = E0, gx = sw0;\n";
-		if (param)
-			os << "\tuint32_t gu = ue0, gd = de0;\n";
 		for (int w = 1; w < W; w++)
-		{
 			os << "\tge = (gw == " << w << ") ? E" << w << " : ge;\n\tgx = (gw == " << w << ") ? sw" << w << " : gx;\n";
-			if (param)
-				os << "\tgu = (gw == " << w << ") ? ue" << w << " : gu;\n\tgd = (gw == " << w << ") ? de" << w
-				   << " : gd;\n";
-		}
-		// mutants: a node that may not switch in its direction has rate 0
-		if (param)
-			os << "\tge &= (gx & gd) | (~gx & gu);\n";
 		os << "\tge >>= gb;\n\tgx >>= gb;\n";
 		if (param)
 			os << "\tconst unsigned long long* gud = mbg_udp_lds[(threadIdx.x >> 6) & 3] + gsel * " << k_group << ";\n";

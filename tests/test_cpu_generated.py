@@ -217,3 +217,58 @@ def test_single_unit_kernels_compile_with_hiprtc(native_bin, tmp_path):
         src.write_text(text)
         p = subprocess.run([exe, str(src), *headers], capture_output=True, text=True, timeout=600)
         assert p.returncode == 0 and p.stdout.startswith("OK"), (name, p.stderr[-3000:])
+
+
+def _mutant_masks(m, rng, k):
+    """k random mutant nodes: (cfg text, up_en words, dn_en words)."""
+    N, W = m.info.n_nodes, m.info.n_words
+    up = [0xFFFFFFFF] * W
+    dn = [0xFFFFFFFF] * W
+    lines = []
+    names = m.node_names
+    for i in rng.choice(N, size=k, replace=False):
+        on = bool(rng.integers(0, 2))
+        if on:
+            dn[i // 32] &= ~(1 << (i % 32))
+        else:
+            up[i // 32] &= ~(1 << (i % 32))
+        lines.append(f"{names[i]}.mutant = {'ON' if on else 'OFF'};\n")
+    return "".join(lines), up, dn
+
+
+@pytest.mark.parametrize("workload", ["plain_100", "batched_50", "grad_30", "cellcycle"])
+def test_generated_runtime_rates_with_mutant_masks(native_bin, tmp_path, workload):
+    """mbg_eval_p (rates from a table, mutant masks as up_en / dn_en words),
+    run on the host, equals the oracle's evaluation of the same model with
+    those nodes written as mutants in the .cfg (rate_down or rate_up 0) bit
+    for bit; and its enabled words equal, for every node, whether the
+    engine's interpreter gives that mutant model a nonzero rate."""
+    if workload in REF_MODELS:
+        bnd, cfg = model_paths(workload)
+    else:
+        bnd, cfg = netgen.write_workload(workload, str(tmp_path))
+    m = mg.Model.load(bnd, cfg)
+    rng = np.random.default_rng(11)
+    mut, up, dn = _mutant_masks(m, rng, min(6, m.info.n_nodes // 3))
+    mcfg = tmp_path / "mutant.cfg"
+    mcfg.write_text(open(cfg).read() + "\n" + mut)
+    gen = tmp_path / f"{workload}_p.hip"
+    gen.write_text(m.generate_code())
+    exe = native_bin("gen_harness", extra=[f'-DGEN_FILE="{gen}"', "-DGEN_PARAM"], tag=str(gen))
+    W, N = m.info.n_words, m.info.n_nodes
+    st, u2 = _states(N, 2000, 7)
+    pv = m.param_values
+    head = f"{len(pv)} " + " ".join(f"{v.view(np.uint32):x}" for v in pv) + " " + \
+        " ".join(f"{w:x}" for w in up) + " " + " ".join(f"{w:x}" for w in dn) + "\n"
+    text = "".join(" ".join(f"{w:x}" for w in row) + f" {u.view(np.uint32):08x}\n" for row, u in zip(st, u2))
+    a = pipe(exe, head + text)
+    b = pipe(ORACLE, text, "--eval-states", bnd, str(mcfg))
+    assert len(a) == len(b) == len(st)
+    assert [" ".join(x.split()[:4]) for x in a] == b
+    mm = mg.Model.load(bnd, str(mcfg))
+    for row, line in list(zip(st, a))[:200]:
+        en = [int(x, 16) for x in line.split()[4:]]
+        words = [int(x) for x in row]
+        for i in range(N):
+            bit = (en[i // 32] >> (i % 32)) & 1
+            assert bit == (mm.rate(i, words) != 0.0), (workload, i)
