@@ -93,7 +93,9 @@ typedef struct mbg_compile_opts
 	                            (register budget 512 / waves per lane); 0 = automatic:
 	                            4, 3 or 2 by node count, one fewer while the kernel
 	                            would spill more than 4 registers (down to 2) */
-	int32_t reserved[4];
+	int32_t grad_occupancy;  /* waves per SIMD the gradient kernel is compiled for;
+	                            0 = automatic: 2 up to 64 rate parameters, 1 above */
+	int32_t reserved[3];
 } mbg_compile_opts;
 
 /* Generated HIP source of the model (caller frees with mbg_free_string). */
@@ -256,7 +258,7 @@ typedef struct mbg_grad_params
 	int32_t block_size;       /* 0: from the LDS budget (64..256) */
 	int32_t blocks_per_cu;    /* 0: occupancy limit */
 	int32_t fold_lanes;       /* ended trajectories a wave gathers before folding them into the
-	                             lanes' running totals (1-64); 0: 8.  Changes no result beyond
+	                             block's accumulators (1-64); 0: 16.  Changes no result beyond
 	                             the order of double additions */
 	int32_t reserved[5];
 } mbg_grad_params;

@@ -114,15 +114,19 @@ class Model:
         finally:
             lib().mbg_free_string(p)
 
-    def compile(self, entropy_formula: int = 0, no_cache: bool = False, lane_occupancy: int = 0) -> "Program":
-        return Program(self, entropy_formula=entropy_formula, no_cache=no_cache, lane_occupancy=lane_occupancy)
+    def compile(self, entropy_formula: int = 0, no_cache: bool = False, lane_occupancy: int = 0,
+                grad_occupancy: int = 0) -> "Program":
+        return Program(self, entropy_formula=entropy_formula, no_cache=no_cache, lane_occupancy=lane_occupancy,
+                       grad_occupancy=grad_occupancy)
 
 
 class Program:
     """Generated + JIT-compiled kernels of a model (the reference's kernel_compiler)."""
 
-    def __init__(self, model: Model, entropy_formula: int = 0, no_cache: bool = False, lane_occupancy: int = 0):
-        opts = CompileOpts(entropy_formula=entropy_formula, no_cache=int(no_cache), lane_occupancy=lane_occupancy)
+    def __init__(self, model: Model, entropy_formula: int = 0, no_cache: bool = False, lane_occupancy: int = 0,
+                 grad_occupancy: int = 0):
+        opts = CompileOpts(entropy_formula=entropy_formula, no_cache=int(no_cache), lane_occupancy=lane_occupancy,
+                           grad_occupancy=grad_occupancy)
         h = ctypes.c_void_p()
         check(lib().mbg_program_compile(model._h, ctypes.byref(opts), ctypes.byref(h)))
         self._h = h

@@ -66,7 +66,8 @@ class CompileOpts(ctypes.Structure):
     _fields_ = [
         ("entropy_formula", ctypes.c_int32), ("no_cache", ctypes.c_int32),
         ("with_gradient", ctypes.c_int32), ("lane_occupancy", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("grad_occupancy", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 

@@ -18,16 +18,22 @@
 //   sum_w Gw S_j(e_w) = Gtot * (n_j / theta_j - T_j)
 //                       - sum_{jumps of j} C(t_k) / theta_j + BT_j,
 //   T_j = integral c_j du,  BT_j = integral c_j C du.
-// Per lane and parameter T_j and BT_j live in registers in double precision
-// (the per-trajectory score accumulator), fed with each record's dwell and
-// C-weighted dwell summed in double over its window slices; n_j lives in LDS
-// (u16, lane private).  A trajectory's total is folded into the BT registers
-// when it ends (they then carry the lane's running sum over its
-// trajectories, and the next trajectory's BT adds to it), so ending a
-// trajectory is register work; the C(t_k) term goes to per-block double
-// accumulators in LDS at each jump, the lanes' sums at the end, and the block
-// totals to the output with one atomic per parameter.
-//
+// Per lane and parameter T_j lives in a register in double precision (the
+// per-trajectory score accumulator), fed with each record's dwell summed in
+// double over its window slices; n_j lives in LDS (u16, lane private).  BT_j
+// is accumulated by events: with D(t) = integral_0^t C du (a double per
+// lane), BT_j = sum over the intervals where node direction k, whose rate is
+// theta_j, is enabled of D(end) - D(start), so when an enabled indicator
+// turns on (off) at t, -D(t) (+D(t)) goes straight to a per-block double
+// accumulator in LDS, and the intervals still open at the trajectory's end
+// close there.  Ended trajectories are folded together: each lane's
+// Gtot * (n_j / theta_j - T_j) + (open intervals) * D is summed over the
+// wave (DPP) and added to the block accumulator, which also takes the
+// -C(t_k) / theta_j term of every jump; the block totals go to the output
+// with one atomic per parameter.  Only T occupies registers per parameter
+// (a lane keeping BT_j, or a running total of its trajectories, as well
+// needs twice as many).
+
 // The trajectory is the forward kernel's: same Philox stream (seed, id,
 // variant), same rates (runtime values through mbg_eval_p), same time and
 // flip arithmetic, so the sample is the one the forward statistics came from.
@@ -48,9 +54,36 @@
 
 // Trajectories of a wave that have ended wait, without stepping, until
 // a.fold_lanes of the wave's lanes have ended (or no lane is still running),
-// and are then folded into the lanes' running totals together: the fold
-// touches every parameter, so doing it for one lane at a time would run it on
-// most steps with one or two lanes active.
+// and are then folded together: the fold touches every parameter and sums
+// over the wave, so doing it for one lane at a time would run it on most
+// steps with one or two lanes active.
+
+namespace mbg_grad
+{
+
+// Sum of v over the wave's 64 lanes, in lane 63 (every lane must call): the
+// inclusive DPP scan of mbg_lane_core.h on the two halves of a double.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double add_dpp_f64(double v)
+{
+	const uint64_t b = __double_as_longlong(v);
+	const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWS, 0xF, false);
+	const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROWS, 0xF, false);
+	return v + __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v)
+{
+	v = add_dpp_f64<0x111, 0xF>(v);
+	v = add_dpp_f64<0x112, 0xF>(v);
+	v = add_dpp_f64<0x114, 0xF>(v);
+	v = add_dpp_f64<0x118, 0xF>(v);
+	v = add_dpp_f64<0x142, 0xA>(v);
+	v = add_dpp_f64<0x143, 0xC>(v);
+	return v;
+}
+
+} // namespace mbg_grad
 
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MBG_GRAD_WAVES)))
 simulate_grad(mbg_grad_args a)
@@ -90,41 +123,62 @@ simulate_grad(mbg_grad_args a)
 	const int lane = __lane_id();
 	const uint32_t B0 = ((uint32_t)a.n_nodes + 3u) >> 2;
 	uint32_t s[W];
+	// enabled up (cu) / down (cd) directions of the current record's state;
+	// all clear before a trajectory's first record
+	uint32_t cu[W], cd[W];
 #pragma unroll
 	for (int w = 0; w < W; ++w)
+	{
 		s[w] = 0u;
-	// T[j]: the current trajectory's integral of c_j; R[j]: the contributions
-	// of the lane's folded trajectories plus the current one's BT_j
-	double T[NP], R[NP];
+		cu[w] = 0u;
+		cd[w] = 0u;
+	}
+	// T[j]: the current trajectory's integral of c_j
+	double T[NP];
 #pragma unroll
 	for (int j = 0; j < NP; ++j)
-	{
 		T[j] = 0.0;
-		R[j] = 0.0;
-	}
 	float t = 0.0f;
 	uint32_t k = 0;
 	uint64_t id = 0;
 	bool active = false, waiting = false, exhausted = false;
-	double C = 0.0, gw = 0.0; // completed windows' G; G of the window in progress
+	// completed windows' G; G of the window in progress; D = integral of C
+	double C = 0.0, gw = 0.0, D = 0.0;
 	uint64_t ntrans = 0;
 
 	mbg_dev::id_chunks queue;
 	for (;;)
 	{
-		// fold the ended trajectories: Gtot * (n_j / theta_j - T_j) joins R_j
+		// fold the ended trajectories: Gtot * (n_j / theta_j - T_j) and the
+		// intervals still open (D at the end) join the block accumulators
 		const uint64_t wm = __ballot(waiting);
 		if (wm && (__popcll(wm) >= a.fold_lanes || __ballot(active) == 0ull))
 		{
-			if (waiting)
-			{
-				const double gt = C + gw;
+			const double gt = C + gw;
 #pragma unroll
-				for (int j = 0; j < NP; ++j)
+			for (int j = 0; j < NP; ++j)
+			{
+				double v = 0.0;
+				if (waiting)
 				{
-					R[j] = fma(gt, fma((double)mycnt[j], invt[j], -T[j]), R[j]);
+					v = fma(gt, fma((double)mycnt[j], invt[j], -T[j]), (double)mbg_grad_on(j, cu, cd) * D);
 					mycnt[j] = 0;
 					T[j] = 0.0;
+				}
+				v = mbg_grad::wave_sum_f64(v);
+				if (lane == 63 && v != 0.0)
+					atomicAdd(acc + j, v);
+				// one parameter at a time: the scheduler would otherwise
+				// interleave all the reductions and keep their values live
+				__builtin_amdgcn_sched_barrier(0);
+			}
+			if (waiting)
+			{
+#pragma unroll
+				for (int w = 0; w < W; ++w)
+				{
+					cu[w] = 0u;
+					cd[w] = 0u;
 				}
 				waiting = false;
 			}
@@ -144,6 +198,7 @@ simulate_grad(mbg_grad_args a)
 					k = 0;
 					C = 0.0;
 					gw = 0.0;
+					D = 0.0;
 					active = true;
 					fresh = true;
 				}
@@ -167,6 +222,26 @@ simulate_grad(mbg_grad_args a)
 			t = mbg_dev::advance_time(false, t, e.total, u1, a.tick, a.max_time, fixed);
 			ntrans++;
 
+			// enabled directions that turned on (-D) or off (+D) at tp; none
+			// while D is 0 (a trajectory's first records)
+#pragma unroll
+			for (int w = 0; w < W; ++w)
+			{
+				const uint32_t nu = en[w] & ~s[w], nd = en[w] & s[w];
+				uint64_t tog = D != 0.0 ? ((uint64_t)(nd ^ cd[w]) << 32 | (nu ^ cu[w])) : 0ull;
+				while (tog)
+				{
+					const int bit = __ffsll((unsigned long long)tog) - 1;
+					tog &= tog - 1ull;
+					const int dir = bit >> 5, node = 32 * w + (bit & 31);
+					const unsigned p = jp[2 * node + dir];
+					if (p != 0xffffu)
+						atomicAdd(acc + p, (((dir ? nd : nu) >> (bit & 31)) & 1u) ? -D : D);
+				}
+				cu[w] = nu;
+				cd[w] = nd;
+			}
+
 			// dwell slices of [tp, t): G weights, window completions, C-weighted dwell
 			const uint64_t obs = mbg_obs_index(s);
 			int w = (int)floorf(tp / a.tick);
@@ -189,7 +264,8 @@ simulate_grad(mbg_grad_args a)
 				++w;
 				b = en;
 			}
-			mbg_grad_dwell(s, en, dsum, cdt, T, R);
+			mbg_grad_dwell(s, en, dsum, T);
+			D += cdt;
 
 			k++;
 			if (t >= a.max_time || k >= a.max_steps)
@@ -224,17 +300,6 @@ simulate_grad(mbg_grad_args a)
 		}
 	}
 
-	// the lanes' running totals to the block accumulators, one atomic per
-	// parameter and wave
-#pragma unroll
-	for (int j = 0; j < NP; ++j)
-	{
-		double v = R[j];
-		for (int off = 32; off > 0; off >>= 1)
-			v += __shfl_xor(v, off);
-		if (lane == 0 && v != 0.0)
-			atomicAdd(acc + j, v);
-	}
 	for (int off = 32; off > 0; off >>= 1)
 		ntrans += __shfl_xor(ntrans, off);
 	if (lane == 0 && ntrans)

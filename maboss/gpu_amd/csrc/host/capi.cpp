@@ -73,6 +73,8 @@ mbg::gen_options to_gen(const mbg_compile_opts* o)
 		g.entropy = mbg::entropy_formula::reference;
 	if (o && o->lane_occupancy > 0)
 		g.lane_waves = std::min(o->lane_occupancy, 8);
+	if (o && o->grad_occupancy > 0)
+		g.grad_waves = std::min(o->grad_occupancy, 8);
 	return g;
 }
 

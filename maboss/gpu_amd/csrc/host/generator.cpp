@@ -750,9 +750,9 @@ generated generate(const model& m, const gen_options& opt)
 // d total / d theta_j in a state is the number of nodes whose current rate
 // is theta_j: node i in the fast form has rate theta_u (up_param) when off
 // with logic on, theta_d (down_param) when on with logic off, else 0.  The
-// emitted mbg_grad_dwell adds the record's dwell (T) and its C-weighted dwell
-// (BT) to the per-parameter double accumulators of the nodes where that
-// indicator is set.
+// emitted mbg_grad_dwell adds the record's dwell to the per-parameter double
+// accumulators (T) of the nodes where that indicator is set; mbg_grad_on
+// counts, for one parameter, the node directions whose indicator is set.
 grad_code generate_grad(const model& m, const generated& g)
 {
 	const int N = m.n_nodes();
@@ -772,9 +772,11 @@ grad_code generate_grad(const model& m, const generated& g)
 	std::ostringstream os;
 	os << "\n#define MBG_GRAD_NP " << gc.n_params << "\n\n";
 	os << "__device__ __forceinline__ void mbg_grad_dwell(const uint32_t* __restrict__ s, const uint32_t* __restrict__ en, "
-		  "double dt, double cdt, double (&T)[MBG_GRAD_NP], double (&BT)[MBG_GRAD_NP])\n{\n";
+		  "double dt, double (&T)[MBG_GRAD_NP])\n{\n";
 	for (int w = 0; w < m.n_words(); w++)
 		os << "\tconst uint32_t sw" << w << " = s[" << w << "], ew" << w << " = en[" << w << "];\n";
+	// per parameter and state word: nodes whose up (um) / down (dm) rate it is
+	std::vector<std::vector<uint32_t>> um(gc.n_params, std::vector<uint32_t>(m.n_words(), 0u)), dm = um;
 	for (int i = 0; i < N; i++)
 	{
 		const node_t& n = m.nodes[i];
@@ -801,19 +803,43 @@ grad_code generate_grad(const model& m, const generated& g)
 		if (ni.up_param >= 0)
 		{
 			gc.jparam[2 * (size_t)i] = (unsigned short)ni.up_param;
+			um[ni.up_param][w] |= 1u << b;
 			os << "\t\t{ const double c = (double)(e & (x ^ 1u)); T[" << ni.up_param << "] = fma(c, dt, T["
-			   << ni.up_param << "]); BT[" << ni.up_param << "] = fma(c, cdt, BT[" << ni.up_param << "]); }\n";
+			   << ni.up_param << "]); }\n";
 		}
 		if (ni.down_param >= 0)
 		{
 			gc.jparam[2 * (size_t)i + 1] = (unsigned short)ni.down_param;
+			dm[ni.down_param][w] |= 1u << b;
 			os << "\t\t{ const double c = (double)(e & x); T[" << ni.down_param << "] = fma(c, dt, T["
-			   << ni.down_param << "]); BT[" << ni.down_param << "] = fma(c, cdt, BT[" << ni.down_param
-			   << "]); }\n";
+			   << ni.down_param << "]); }\n";
 		}
 		os << "\t}\n";
 	}
-	os << "}\n";
+	os << "}\n\n";
+	// c_j = d total / d theta_j in a state, from the words of enabled up (cu)
+	// and down (cd) directions: the number of enabled node directions whose
+	// rate is theta_j.  The masks are a constant table, so the kernel's
+	// unrolled loop over j reads them as immediates.
+	const int W = m.n_words();
+	for (int d = 0; d < 2; d++)
+	{
+		os << "static __device__ const uint32_t mbg_grad_" << (d ? "dm" : "um") << "[" << gc.n_params * W << "] = {";
+		for (int j = 0; j < gc.n_params; j++)
+			for (int w = 0; w < W; w++)
+			{
+				char buf[24];
+				std::snprintf(buf, sizeof(buf), "%s0x%08xu", j + w == 0 ? "" : ", ", d ? dm[j][w] : um[j][w]);
+				os << buf;
+			}
+		os << "};\n";
+	}
+	os << "__device__ __forceinline__ uint32_t mbg_grad_on(int j, const uint32_t* __restrict__ cu, "
+		  "const uint32_t* __restrict__ cd)\n{\n\tuint32_t c = 0u;\n";
+	for (int w = 0; w < W; w++)
+		os << "\tc += __builtin_popcount(cu[" << w << "] & mbg_grad_um[j * " << W << " + " << w
+		   << "]) + __builtin_popcount(cd[" << w << "] & mbg_grad_dm[j * " << W << " + " << w << "]);\n";
+	os << "\treturn c;\n}\n";
 	gc.source = os.str();
 	return gc;
 }

@@ -26,6 +26,8 @@ struct gen_options
 	// Waves per SIMD the lane kernels are compiled for (the register budget:
 	// 512 / waves VGPRs per lane); 0 lets the compiler choose.
 	int lane_waves = 0;
+	// Waves per SIMD of the gradient kernel; 0: by parameter count.
+	int grad_waves = 0;
 };
 
 // What the generator decided about each node; exported for the oracle-side

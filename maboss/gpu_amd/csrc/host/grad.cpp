@@ -154,8 +154,8 @@ void simulate_grad(program& p, const grad_params& gp, const float* rates, const 
 	a.max_time = m.max_time();
 	a.tick = m.time_tick();
 	a.g_lds = g_lds ? 1 : 0;
-	// 8 measured best on grad_30 (5.24 ms; 16: 5.35, 4: 5.33, 32: 5.77)
-	a.fold_lanes = gp.fold_lanes > 0 ? std::min(gp.fold_lanes, 64) : 8;
+	// 16 measured best on grad_30 (3.63 ms; 12: 3.67, 24: 3.67, 8: 3.84, 4: 4.42)
+	a.fold_lanes = gp.fold_lanes > 0 ? std::min(gp.fold_lanes, 64) : 16;
 	a.lds_bytes = (int)lds;
 	a.istate = (const float*)(dtab + en_bytes);
 	a.pv.rate = rates;

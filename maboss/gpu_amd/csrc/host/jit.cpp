@@ -252,12 +252,15 @@ hipFunction_t program::grad_function()
 	const grad_code& gc = grad();
 	scope_timer tm("compiler> gradient kernel");
 	const double t0 = wall_seconds();
-	// Occupancy: the T and BT accumulators are doubles, 4 registers per
-	// parameter; 2 waves per SIMD (256 registers) leave room for the rate
-	// code up to 32 parameters, 1 wave (512 registers) above.
-	const int waves = gc.n_params <= 32 ? 2 : 1;
-	const std::string src = "#define MBG_GRAD_WAVES " + std::to_string(waves) + "\n" + rates_define(mdl->n_nodes())
-							+ gen.source + gc.source
+	// Occupancy: the T accumulators are doubles, 2 registers per parameter,
+	// next to about 170 for the rest of the kernel; 2 waves per SIMD (256
+	// registers) up to 64 parameters, 1 wave (512 registers) above.  Measured
+	// on grad_30 (60 parameters): 2 waves 3.6 ms (a few spilled registers),
+	// 1 wave 5.3 ms, 3 and 4 waves 10-13 ms (most of T spilled).  The rate
+	// tables are read with scalar loads: vector loads would hoist every rate
+	// into a register of its own.
+	const int waves = opts.grad_waves > 0 ? opts.grad_waves : gc.n_params <= 64 ? 2 : 1;
+	const std::string src = "#define MBG_GRAD_WAVES " + std::to_string(waves) + "\n" + gen.source + gc.source
 							+ "\n#include \"mbg_grad_kernel.h\"\n";
 	const uint64_t h = unit_hash(src, { mbg_hdr_numeric, mbg_hdr_abi, mbg_hdr_kernel_args, mbg_hdr_stats_device,
 										mbg_hdr_lane_core, mbg_hdr_grad_kernel });
