@@ -348,7 +348,11 @@ hipFunction_t program::function(int flavour, int win_mode, int fin_mode)
 	// (window-cell inserts), so a kernel that spills more goes down one wave.  Measured on
 	// MI355X: at 30 nodes 4 waves beat the compiler's choice by 15 %; on the
 	// 100-node model 2 waves without spills beat 3 with 11 spilled registers
-	// by 5 %.
+	// by 5 %.  When the window cells and final states are both kept in LDS
+	// the loop has no global atomics in flight for a reload to wait behind,
+	// and up to 64 spilled words are kept: the grad_30 forward (run-time
+	// rates, 30 nodes) runs 7 % faster at 3 waves with 43 spilled words than
+	// at 2 without.
 	const int n_nodes = mdl->n_nodes();
 	int waves = 4;
 	if (opts.lane_waves > 0)
@@ -381,8 +385,9 @@ hipFunction_t program::function(int flavour, int win_mode, int fin_mode)
 			hip_check(hipModuleGetFunction(&fn, mod, "simulate"), "hipModuleGetFunction(simulate)");
 		int scratch = 0;
 		hip_check(hipFuncGetAttribute(&scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fn), "hipFuncGetAttribute");
-		// a few spilled registers (up to 4 words) are kept: their reloads are rare
-		if (scratch <= 16 || waves <= 2 || opts.lane_waves > 0)
+		// a few spilled registers are kept: their reloads are rare
+		const int spill_limit = (!param && win_mode == MBG_HIST_LDS && fin_mode == MBG_HIST_LDS) ? 256 : 16;
+		if (scratch <= spill_limit || waves <= 2 || opts.lane_waves > 0)
 			break;
 		(void)hipModuleUnload(mod);
 		mod = nullptr;
