@@ -61,17 +61,6 @@ const char* const k_compile_flags[] = { "--offload-arch=gfx950",
 										"-Xclang",
 										"-packed-fp32-ops" };
 
-// Run-time rate tables are read with scalar loads (constant address space)
-// above 40 nodes and with vector loads up to 40.  Measured on MI355X: scalar
-// loads make the batched 50-node launch 17 % faster and the 100-node
-// run-time-rates simulate 24 % faster; on a 30-node model with 60 rate
-// parameters they make the simulate 17 % slower (the rates hoisted into SGPRs
-// spill to VGPR lanes) and the gradient kernel 2 % slower.
-std::string rates_define(int n_nodes)
-{
-	return n_nodes <= 40 ? "#define MBG_RATES_VECTOR 1\n" : "";
-}
-
 uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull)
 {
 	const unsigned char* p = (const unsigned char*)data;
@@ -256,9 +245,7 @@ hipFunction_t program::grad_function()
 	// next to about 170 for the rest of the kernel; 2 waves per SIMD (256
 	// registers) up to 64 parameters, 1 wave (512 registers) above.  Measured
 	// on grad_30 (60 parameters): 2 waves 3.6 ms (a few spilled registers),
-	// 1 wave 5.3 ms, 3 and 4 waves 10-13 ms (most of T spilled).  The rate
-	// tables are read with scalar loads: vector loads would hoist every rate
-	// into a register of its own.
+	// 1 wave 5.3 ms, 3 and 4 waves 10-13 ms (most of T spilled).
 	const int waves = opts.grad_waves > 0 ? opts.grad_waves : gc.n_params <= 64 ? 2 : 1;
 	const std::string src = "#define MBG_GRAD_WAVES " + std::to_string(waves) + "\n" + gen.source + gc.source
 							+ "\n#include \"mbg_grad_kernel.h\"\n";
@@ -279,7 +266,7 @@ hipFunction_t program::moments_function(int states_per_lane)
 		return fn_mom[slot];
 	scope_timer tm("compiler> moments kernel");
 	const double t0 = wall_seconds();
-	const std::string src = "#define MBG_MOM_K " + std::to_string(slot ? 1 : 8) + "\n" + rates_define(mdl->n_nodes())
+	const std::string src = "#define MBG_MOM_K " + std::to_string(slot ? 1 : 8) + "\n"
 							+ gen.source + "\n#include \"mbg_moments_kernel.h\"\n";
 	const uint64_t h = unit_hash(src, { mbg_hdr_numeric, mbg_hdr_abi, mbg_hdr_kernel_args, mbg_hdr_stats_device,
 										mbg_hdr_lane_core, mbg_hdr_moments_kernel });
@@ -348,11 +335,7 @@ hipFunction_t program::function(int flavour, int win_mode, int fin_mode)
 	// (window-cell inserts), so a kernel that spills more goes down one wave.  Measured on
 	// MI355X: at 30 nodes 4 waves beat the compiler's choice by 15 %; on the
 	// 100-node model 2 waves without spills beat 3 with 11 spilled registers
-	// by 5 %.  When the window cells and final states are both kept in LDS
-	// the loop has no global atomics in flight for a reload to wait behind,
-	// and up to 64 spilled words are kept: the grad_30 forward (run-time
-	// rates, 30 nodes) runs 7 % faster at 3 waves with 43 spilled words than
-	// at 2 without.
+	// by 5 %.
 	const int n_nodes = mdl->n_nodes();
 	int waves = 4;
 	if (opts.lane_waves > 0)
@@ -365,8 +348,6 @@ hipFunction_t program::function(int flavour, int win_mode, int fin_mode)
 	{
 		std::string src = "#define MBG_KW " + std::to_string(W) + "\n#define MBG_PARAM " + std::to_string(flavour)
 						  + "\n#define MBG_LANE_WAVES " + std::to_string(waves) + "\n";
-		if (flavour != 0)
-			src += rates_define(n_nodes);
 		if (!param)
 			src += "#define MBG_WIN_MODE " + std::to_string(win_mode) + "\n#define MBG_FIN_MODE "
 				   + std::to_string(fin_mode) + "\n";
@@ -385,9 +366,8 @@ hipFunction_t program::function(int flavour, int win_mode, int fin_mode)
 			hip_check(hipModuleGetFunction(&fn, mod, "simulate"), "hipModuleGetFunction(simulate)");
 		int scratch = 0;
 		hip_check(hipFuncGetAttribute(&scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fn), "hipFuncGetAttribute");
-		// a few spilled registers are kept: their reloads are rare
-		const int spill_limit = (!param && win_mode == MBG_HIST_LDS && fin_mode == MBG_HIST_LDS) ? 256 : 16;
-		if (scratch <= spill_limit || waves <= 2 || opts.lane_waves > 0)
+		// a few spilled registers (up to 4 words) are kept: their reloads are rare
+		if (scratch <= 16 || waves <= 2 || opts.lane_waves > 0)
 			break;
 		(void)hipModuleUnload(mod);
 		mod = nullptr;
