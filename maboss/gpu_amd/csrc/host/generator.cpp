@@ -615,9 +615,8 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 		os << "\t\tconst uint32_t ub = (uint32_t)ud, db = (uint32_t)(ud >> 32);\n";
 		os << "\t\tconst uint32_t xm = 0u - ((gx >> j) & 1u), msk = 0u - ((ge >> j) & 1u);\n";
 		os << "\t\tacc2 = acc2 + mbg_u2f((ub ^ ((ub ^ db) & xm)) & msk);\n";
-		// pinned per node up to 40 nodes (4 waves per SIMD, 128 registers);
-		// above, the compiler schedules the group freely
-		os << "\t\tcnt += (acc2 < rr) ? 1 : 0;\n" << (N <= 40 ? "\t\tMBG_PIN2(acc2, cnt);\n" : "") << "\t}\n";
+		// the compiler schedules the group freely (no pins)
+		os << "\t\tcnt += (acc2 < rr) ? 1 : 0;\n\t}\n";
 		os << "\tflip = gsel * " << k_group << " + cnt;\n";
 	}
 	else
