@@ -310,7 +310,7 @@ def bench_grad(mg, netgen, wdir, steps):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="plain_100", choices=sorted(WORKLOAD_TEXT))
     ap.add_argument("--trajectories", type=int, default=-1, help="override the workload's trajectory count")
@@ -378,7 +378,7 @@ def main():
     sorted_out = None
     if li.window_mode == 1 and not a.no_sorted_output:
         pipelined_steps(prog, n, 1, window_mode="sort")
-        s_wall, s_trans, s_sims, s_li = pipelined_steps(prog, n, max(1, min(a.steps, 3)), window_mode="sort")
+        s_wall, s_trans, s_sims, s_li = pipelined_steps(prog, n, max(1, min(a.steps, 10)), window_mode="sort")
         sorted_out = {"window_mode": "sort", "value": s_trans / s_wall, "unit": "transitions/s",
                       "simulate_ms": sum(s_sims) / len(s_sims), "window_pairs": s_li.window_pairs,
                       "note": "window cells appended as (key, value) pairs and sort-reduced instead of hashed"}
@@ -443,7 +443,7 @@ def main():
         "compile_seconds": prog.compile_seconds,
     }
     if not a.no_forms and a.workload == "plain_100":
-        out["discrete"] = bench_discrete(mg, netgen, wdir, max(1, min(a.steps, 3)))
+        out["discrete"] = bench_discrete(mg, netgen, wdir, max(1, min(a.steps, 10)))
     if not a.no_forms:
         out["scale_1000"] = bench_scale(mg, netgen, wdir, 2, not a.no_cpu_baseline, max(a.cpu_seconds, 4.0))
         out["batched"] = bench_batched(mg, netgen, wdir, max(1, min(a.steps, 3)))
