@@ -109,7 +109,7 @@ struct mbg_sim_args
 	const mbg_param_view* pv;       // per-variant tables (batched/gradient); null for plain
 	mbg_wave_tables wt;             // wave-per-trajectory kernel only
 	int wave_lds_off;               // byte offset of the per-wave rate arrays in LDS
-	int pad2;
+	int bent_windows;               // batched form: per-wave LDS entropy sums, windows (0: global atomics)
 
 	// batched form only
 	const struct mbg_batch_variant* bv;  // [n_variants]

@@ -295,18 +295,25 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 	// histogram atomics in flight)
 	__shared__ float ist_tab[4][32 * W];
 	float* const istate = ist_tab[(threadIdx.x >> 6) & 3];
-	// per wave: the window-cell cache, S values (u64) then S keys (u32) in
-	// dynamic LDS; cell index (w << n_obs) | obs within the chunk's variant
+	// per wave in dynamic LDS: the window-cell cache, S values (u64) then S
+	// keys (u32), cell index (w << n_obs) | obs within the chunk's variant;
+	// then EW entropy sums of the chunk's windows, added to the variant's
+	// slice when the chunk is done (every lane of a wave adds to the same few
+	// windows, which as global atomics cost 16 % of the batched launch)
 	extern __shared__ unsigned long long bcache[];
 	const uint32_t S = a.bcache_slots;
+	const uint32_t EW = (uint32_t)a.bent_windows;
 	const int sbits = S ? 31 - __builtin_clz(S) : 0;
-	unsigned long long* const cval = bcache + (size_t)(threadIdx.x >> 6) * (S + S / 2);
+	unsigned long long* const cval = bcache + (size_t)(threadIdx.x >> 6) * (S + S / 2 + EW);
 	uint32_t* const ckey = (uint32_t*)(cval + S);
+	unsigned long long* const went = cval + S + S / 2;
 	for (uint32_t i = (uint32_t)lane; i < S; i += 64u)
 	{
 		cval[i] = 0ull;
 		ckey[i] = MBG_BCACHE_EMPTY;
 	}
+	for (uint32_t i = (uint32_t)lane; i < EW; i += 64u)
+		went[i] = 0ull;
 	MBG_WAVE_SYNC();
 	uint32_t s[W];
 #pragma unroll
@@ -351,7 +358,12 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 			}
 			atomicAdd(a.win_vals + win_off + cell, (unsigned long long)q);
 		};
-		auto fe = [&](int w, uint64_t q) { atomicAdd(a.win_ent + ent_off + w, (unsigned long long)q); };
+		auto fe = [&](int w, uint64_t q) {
+			if (EW)
+				atomicAdd(went + w, (unsigned long long)q);
+			else
+				atomicAdd(a.win_ent + ent_off + w, (unsigned long long)q);
+		};
 		auto emit = [&](bool has, int w, uint64_t o, uint64_t q) {
 			if (has)
 				fw(w, o, q);
@@ -463,6 +475,21 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate_ba
 					atomicAdd(a.win_vals + win_off + k, cval[i]);
 					ckey[i] = MBG_BCACHE_EMPTY;
 					cval[i] = 0ull;
+				}
+			}
+			MBG_WAVE_SYNC();
+		}
+		if (EW)
+		{
+			// the chunk's entropy sums to the variant's slice, the LDS sums emptied
+			MBG_WAVE_SYNC();
+			for (uint32_t i = (uint32_t)lane; i < (uint32_t)windows; i += 64u)
+			{
+				const unsigned long long v = went[i];
+				if (v)
+				{
+					atomicAdd(a.win_ent + ent_off + i, v);
+					went[i] = 0ull;
 				}
 			}
 			MBG_WAVE_SYNC();

@@ -285,7 +285,13 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 	// the per-wave LDS cell cache keys cells by their 32-bit index in a variant
 	const int max_windows = L.windows.empty() ? 0 : *std::max_element(L.windows.begin(), L.windows.end());
 	const bool cache_ok = ((uint64_t)max_windows << n_obs) < 0xffffffffull;
-	auto cache_lds = [&](int block, int slots) { return (size_t)(block / 64) * (size_t)slots * 12; };
+	// per-wave LDS entropy sums when the longest horizon has at most 256 windows
+	// (2 KB per wave: with the largest tuned cell cache, 1024 slots x 12 B, a
+	// block of 4 waves stays within 56 KB)
+	a.bent_windows = max_windows <= 256 ? max_windows : 0;
+	auto cache_lds = [&](int block, int slots) {
+		return (size_t)(block / 64) * ((size_t)slots * 12 + (size_t)a.bent_windows * 8);
+	};
 	auto launch = [&](int block, uint64_t grid, int slots) {
 		a.bcache_slots = (unsigned)slots;
 		fill(p, stream, counters, MBG_NCOUNTERS, 0);
