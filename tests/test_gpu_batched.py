@@ -126,6 +126,23 @@ def test_variants_match_oracle(tmp_path, b50):
     assert info.transitions == total_t
 
 
+def test_long_horizon_batch_vs_oracle(tmp_path, b50):
+    """A batch whose longest horizon has more windows (300) than the per-wave
+    LDS entropy sums hold (256), so its entropy sums go straight to global
+    atomics, next to a short one: every variant still equals the oracle."""
+    bnd, cfg = b50
+    cfg_text = open(cfg).read()
+    prog = program(bnd, cfg)
+    variants = [{"n_trajectories": 600, "max_time": 300.0}, {"n_trajectories": 700, "max_time": 40.0}]
+    print("[gpu] batched_50 300-window horizon", flush=True)
+    results, _ = prog.simulate_batched(variants)
+    for v, spec in enumerate(variants):
+        g = gpu_raw(tmp_path, results[v], f"L{v}")
+        o = oracle_variant(tmp_path, bnd, edited_cfg(cfg_text, max_time=spec["max_time"]), v,
+                           spec["n_trajectories"], f"L{v}")
+        assert_raw_equal(g, o, f"variant {v}")
+
+
 def test_default_variant_equals_plain_run(tmp_path, b50):
     """Variant v with the model's rates == plain kernel with stream tag v."""
     bnd, cfg = b50

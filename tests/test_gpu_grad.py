@@ -96,6 +96,41 @@ def test_backward_matches_oracle(tmp_path, g30):
     assert np.all(err <= 1e-6 * np.abs(o) + 1e-10 * scale), (err / np.maximum(np.abs(o), 1e-300)).max()
 
 
+SHARED_BND = """Node A { logic = !C; rate_up = @logic ? $k : 0; rate_down = @logic ? 0 : $d; }
+Node B { logic = A & !D; rate_up = @logic ? $k : 0; rate_down = @logic ? 0 : $d; }
+Node C { logic = B | E; rate_up = @logic ? $k : 0; rate_down = @logic ? 0 : $k; }
+Node D { logic = !A; rate_up = @logic ? $u : 0; rate_down = @logic ? 0 : $d; }
+Node E { logic = C & D; rate_up = @logic ? 1.5 : 0; rate_down = @logic ? 0 : $u; }
+Node F { logic = E | !B; rate_up = @logic ? $u : 0; rate_down = @logic ? 0 : 0.75; }
+"""
+SHARED_CFG = """$k = 1.25; $d = 0.5; $u = 2.0;
+E.is_internal = 1; F.is_internal = 1; A.istate = 0.3; C.istate = 0.6;
+sample_count = 20000; max_time = 12; time_tick = 0.5; discrete_time = 0; seed_pseudorandom = 11;
+"""
+
+
+def test_backward_shared_parameters_match_oracle(tmp_path):
+    """Parameters each used by several node directions ($k by three up rates
+    and one down rate, $d by three down rates, $u by up and down rates of
+    different nodes), with parameter-free rates beside them: every component
+    equals the oracle's to the same bar as grad_30."""
+    bnd, cfg = tmp_path / "s.bnd", tmp_path / "s.cfg"
+    bnd.write_text(SHARED_BND)
+    cfg.write_text(SHARED_CFG)
+    prog = mg.Model.load(str(bnd), str(cfg)).compile()
+    m = prog.model.info
+    G = np.random.default_rng(5).normal(size=(m.windows, 1 << m.n_observed))
+    n = 20000
+    print(f"[gpu] shared-parameter gradient n={n}", flush=True)
+    got = gpu_grad(prog, G, n)
+    want = oracle_grad(tmp_path, str(bnd), str(cfg), G, n, tag="sh")
+    o = np.array([want[k] for k in prog.model.param_names])
+    scale = np.abs(o).max()
+    assert scale > 0 and len(o) == 3
+    err = np.abs(got - o)
+    assert np.all(err <= 1e-6 * np.abs(o) + 1e-10 * scale), (err / np.maximum(np.abs(o), 1e-300)).max()
+
+
 def test_backward_runtime_rates_and_shards(tmp_path, g30):
     """Rates passed at run time equal rates edited into the model; shards add up."""
     bnd, cfg, prog = g30
