@@ -168,9 +168,6 @@ simulate_grad(mbg_grad_args a)
 				v = mbg_grad::wave_sum_f64(v);
 				if (lane == 63 && v != 0.0)
 					atomicAdd(acc + j, v);
-				// one parameter at a time: the scheduler would otherwise
-				// interleave all the reductions and keep their values live
-				__builtin_amdgcn_sched_barrier(0);
 			}
 			if (waiting)
 			{
