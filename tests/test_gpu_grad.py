@@ -131,6 +131,28 @@ def test_backward_shared_parameters_match_oracle(tmp_path):
     assert np.all(err <= 1e-6 * np.abs(o) + 1e-10 * scale), (err / np.maximum(np.abs(o), 1e-300)).max()
 
 
+def test_backward_two_words_ninety_parameters(tmp_path):
+    """A 45-node network (2 state words) with 90 rate parameters: the kernel
+    compiled for 1 wave per SIMD (above 64 parameters) and the enabled-word
+    events of the second word; every component equals the oracle's."""
+    net = netgen.random_network(6, 45, (1, 3), 40, 0.0, 0.5)
+    net.constants = dict(sample_count=20000, max_time=10, time_tick=0.5, discrete_time=0, seed_pseudorandom=6)
+    bnd, cfg = net.write(str(tmp_path / "g45"))
+    prog = mg.Model.load(bnd, cfg).compile()
+    m = prog.model.info
+    assert m.n_words == 2 and m.n_params == 90
+    G = np.random.default_rng(9).normal(size=(m.windows, 1 << m.n_observed))
+    n = 20000
+    print(f"[gpu] 45-node gradient n={n}", flush=True)
+    got = gpu_grad(prog, G, n)
+    want = oracle_grad(tmp_path, bnd, cfg, G, n, tag="g45")
+    o = np.array([want[k] for k in prog.model.param_names])
+    scale = np.abs(o).max()
+    assert scale > 0
+    err = np.abs(got - o)
+    assert np.all(err <= 1e-6 * np.abs(o) + 1e-10 * scale), (err / np.maximum(np.abs(o), 1e-300)).max()
+
+
 def test_backward_runtime_rates_and_shards(tmp_path, g30):
     """Rates passed at run time equal rates edited into the model; shards add up."""
     bnd, cfg, prog = g30
