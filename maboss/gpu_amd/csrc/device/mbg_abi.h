@@ -66,6 +66,23 @@ struct mbg_param_view
 #define MBG_MASK(x) ((void)0)
 #endif
 
+// Any bitwise function of three words, given by its truth table t: bit
+// (4a + 2b + c) of t is the result for input bits a, b, c (gfx950's
+// v_bitop3_b32).  The host form sums the table's minterms.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MBG_BITOP3(a, b, c, t) __builtin_amdgcn_bitop3_b32((a), (b), (c), (t))
+#else
+static inline uint32_t mbg_bitop3_host(uint32_t a, uint32_t b, uint32_t c, unsigned t)
+{
+	uint32_t r = 0u;
+	for (unsigned i = 0; i < 8; i++)
+		if (t >> i & 1u)
+			r |= (i & 4u ? a : ~a) & (i & 2u ? b : ~b) & (i & 1u ? c : ~c);
+	return r;
+}
+#define MBG_BITOP3(a, b, c, t) mbg_bitop3_host((a), (b), (c), (t))
+#endif
+
 // Pins running sums at a point of the instruction stream (no instruction is
 // emitted): the generated rate code places one after each node, so each
 // node's terms are added before the next node's are computed, rather than

@@ -438,15 +438,26 @@ void emit_eval(std::ostringstream& os, const model& m, const gen_options& opt, b
 	};
 	// the selected rate r (and l) of a fast-form node from the masks xm (x)
 	// and msk (enabled), branch-free: same value as (x == L) ? 0 : (x ? D : U)
+	// With run-time rates ub and db are wave-uniform (scalar registers), and a
+	// vector instruction reads one of them: the select is written as two
+	// three-input ops, ((xm & msk & (ub ^ db)) ^ (msk & ub)), each reading one
+	// (truth tables 0x80 and 0x78), which saves the copy of ub into a vector
+	// register the compiler's own form needs per node.
+	auto select_expr = [&](const char* u, const char* d) {
+		if (param)
+			return std::string("mbg_u2f(MBG_BITOP3(MBG_BITOP3(xm, msk, ") + u + " ^ " + d + ", 0x80), msk, " + u
+				   + ", 0x78))";
+		return std::string("mbg_u2f((") + u + " ^ ((" + u + " ^ " + d + ") & xm)) & msk)";
+	};
 	auto select_rate = [&](emitter& em, int i, bool with_l) {
 		os << "\t\tconst uint32_t ub = " << bits_of(em, nc[i].U, nc[i].up_p, false)
 		   << ", db = " << bits_of(em, nc[i].D, nc[i].dn_p, false) << ";\n";
-		os << "\t\tr = mbg_u2f((ub ^ ((ub ^ db) & xm)) & msk);\n";
+		os << "\t\tr = " << select_expr("ub", "db") << ";\n";
 		if (with_l)
 		{
 			os << "\t\tconst uint32_t ul = " << bits_of(em, nc[i].U, nc[i].up_p, true)
 			   << ", dl = " << bits_of(em, nc[i].D, nc[i].dn_p, true) << ";\n";
-			os << "\t\tl = mbg_u2f((ul ^ ((ul ^ dl) & xm)) & msk);\n";
+			os << "\t\tl = " << select_expr("ul", "dl") << ";\n";
 		}
 	};
 	// a node outside the fast form: its rate expressions as written
