@@ -259,11 +259,13 @@ def bench_batched(mg, netgen, wdir, steps):
     t0 = time.perf_counter()
     _, hinfo = prog.simulate_batched(variants)
     one["host_inclusive_transitions_per_s"] = hinfo.transitions / (time.perf_counter() - t0)
+    # one variant per launch, in chunks of 64 ids (one trajectory per lane): the most waves a single
+    # variant of 1e3-1e5 trajectories can keep busy, whatever chunk size the whole batch was tuned to
     sdev, strans = 0.0, 0
     t0 = time.perf_counter()
     for v in variants:
         _, vi = prog.simulate_batched([v], device_only=True, use_graph=False, block_size=info.block_size,
-                                      blocks_per_cu=info.blocks_per_cu, traj_per_lane=info.traj_per_lane,
+                                      blocks_per_cu=info.blocks_per_cu, traj_per_lane=1,
                                       lds_cells=info.lds_cells or -1)
         sdev += vi.kernel_ms / 1e3
         strans += vi.transitions

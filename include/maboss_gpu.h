@@ -235,7 +235,7 @@ void mbg_batch_params_default(mbg_batch_params* p);
 /* Simulate n_variants variants in one launch.  results[v] receives variant
  * v's statistics (free each with mbg_result_free); info may be NULL.  The
  * autotuner times every block size (64, 128, 256) x trajectories per lane
- * (1, 2, 4) x LDS cell cache (none, 256, 1024 slots) on the batch and keeps
+ * (1, 2, 4, 6, 8, 12) x LDS cell cache (none, 256, 1024 slots) on the batch and keeps
  * the fastest; results do not depend on the choice (integer sums). */
 int mbg_simulate_batched(mbg_program* p, const mbg_batch_params* params, int32_t n_variants,
 						 const mbg_variant* variants, void* stream, mbg_result** results, mbg_batch_info* info);

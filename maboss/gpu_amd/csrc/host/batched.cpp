@@ -363,7 +363,7 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 				if (sl && !cache_ok)
 					continue;
 				for (int b : { 64, 128, 256 })
-					for (int t : { 1, 2, 4 })
+					for (int t : { 1, 2, 4, 6, 8, 12 })
 					{
 						const int pc = occupancy(b, sl);
 						const uint64_t g = enqueue(b, pc, t);
