@@ -379,10 +379,10 @@ __device__ void fp_insert(unsigned int* table, uint64_t mask, const uint32_t (&k
 }
 
 // Block-private fixed-point cache in LDS, same slot layout as the global
-// table; one lane of a wave at a time (as fp_insert).  False when the probes
-// find no slot.
+// table; one lane of a wave at a time (as fp_insert).  Adds cnt trajectories;
+// false when the probes find no slot.
 template <int KW>
-__device__ bool fp_lds_insert(unsigned int* t, int nslots, const uint32_t (&key)[KW])
+__device__ bool fp_lds_insert(unsigned int* t, int nslots, const uint32_t (&key)[KW], unsigned int cnt)
 {
 	constexpr int stride = 2 + KW;
 	uint64_t h = 0x9e3779b97f4a7c15ull;
@@ -401,7 +401,7 @@ __device__ bool fp_lds_insert(unsigned int* t, int nslots, const uint32_t (&key)
 				e[2 + w] = key[w];
 			__threadfence_block();
 			atomicExch(e, 2u);
-			atomicAdd(e + 1, 1u);
+			atomicAdd(e + 1, cnt);
 			return true;
 		}
 		for (int spin = 0; tag == 1u && spin < (1 << 16); ++spin)
@@ -414,7 +414,7 @@ __device__ bool fp_lds_insert(unsigned int* t, int nslots, const uint32_t (&key)
 			same &= __hip_atomic_load(e + 2 + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == key[w];
 		if (same)
 		{
-			atomicAdd(e + 1, 1u);
+			atomicAdd(e + 1, cnt);
 			return true;
 		}
 		slot = (slot + 1u) & (uint32_t)(nslots - 1);
@@ -464,13 +464,13 @@ struct lane_hist_t
 		hash_add(a.tail_table, a.tail_mask, key, 1ull, a.counters + MBG_CTR_TAIL_OVERFLOW);
 	}
 
-	// One more trajectory ending in fixed point key (one lane of a wave at a time).
+	// cnt more trajectories ending in fixed point key (one lane of a wave at a time).
 	template <int KW>
-	__device__ void fixed(const uint32_t (&key)[KW]) const
+	__device__ void fixed(const uint32_t (&key)[KW], unsigned int cnt = 1u) const
 	{
-		if (a.fp_lds_slots && fp_lds_insert<KW>((unsigned int*)(lds + a.fp_lds_off), a.fp_lds_slots, key))
+		if (a.fp_lds_slots && fp_lds_insert<KW>((unsigned int*)(lds + a.fp_lds_off), a.fp_lds_slots, key, cnt))
 			return;
-		fp_insert<KW>(a.fp_table, a.fp_mask, key, 1u, a.counters);
+		fp_insert<KW>(a.fp_table, a.fp_mask, key, cnt, a.counters);
 	}
 
 	__device__ void fin(uint64_t obs) const

@@ -168,6 +168,16 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 	uint64_t id = 0, obs = 0; // obs: observed index of s, kept up to date at each flip
 	bool active = false, exhausted = false;
 	pending pend;
+	// Models of up to 64 nodes: the lane's last fixed point and the
+	// trajectories that ended in it since, inserted when the lane ends in
+	// another one and at the end (trajectories of such models often end in
+	// one attractor, and each insert runs one lane of the wave at a time).
+	constexpr bool fp_pend = W <= 2;
+	uint32_t pfs[W];
+#pragma unroll
+	for (int w = 0; w < W; ++w)
+		pfs[w] = 0u;
+	uint32_t pcnt = 0;
 	// the block's {transitions, finished, truncated} counters, after the
 	// waves' item-owner bytes; a trajectory adds its steps when it ends
 	unsigned long long* const bctr = (unsigned long long*)((uint8_t*)(lds + a.own_lds_off) + 64 * (blockDim.x >> 6));
@@ -241,16 +251,56 @@ extern "C" __global__ void __launch_bounds__(256) MBG_LANE_OCCUPANCY simulate(mb
 		{
 			if (ended && t >= a.max_time)
 				hist.fin(obs);
-			uint64_t fm = __ballot(ended && fixed);
-			while (fm)
+			if (fp_pend)
 			{
-				const int l = __ffsll((unsigned long long)fm) - 1;
-				if (lane == l)
-					hist.fixed<W>(s);
-				fm &= fm - 1ull;
+				const bool fx = ended && fixed;
+				bool same = fx && pcnt != 0u;
+#pragma unroll
+				for (int w = 0; w < W; ++w)
+					same = same && pfs[w] == s[w];
+				uint64_t fm = __ballot(fx && !same && pcnt != 0u);
+				while (fm)
+				{
+					const int l = __ffsll((unsigned long long)fm) - 1;
+					if (lane == l)
+						hist.fixed<W>(pfs, pcnt);
+					fm &= fm - 1ull;
+				}
+				if (same)
+					pcnt++;
+				else if (fx)
+				{
+#pragma unroll
+					for (int w = 0; w < W; ++w)
+						pfs[w] = s[w];
+					pcnt = 1u;
+				}
+			}
+			else
+			{
+				uint64_t fm = __ballot(ended && fixed);
+				while (fm)
+				{
+					const int l = __ffsll((unsigned long long)fm) - 1;
+					if (lane == l)
+						hist.fixed<W>(s);
+					fm &= fm - 1ull;
+				}
 			}
 			if (ended)
 				active = false;
+		}
+	}
+
+	if (fp_pend)
+	{
+		uint64_t fm = __ballot(pcnt != 0u);
+		while (fm)
+		{
+			const int l = __ffsll((unsigned long long)fm) - 1;
+			if (lane == l)
+				hist.fixed<W>(pfs, pcnt);
+			fm &= fm - 1ull;
 		}
 	}
 
