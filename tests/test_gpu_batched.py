@@ -172,7 +172,7 @@ def test_graph_replays_match_eager(tmp_path, b50):
         prog.simulate(n_trajectories=500)  # eager work between replays
 
 
-@pytest.mark.parametrize("block,tpl", [(64, 1), (256, 4), (0, 0)])
+@pytest.mark.parametrize("block,tpl", [(64, 1), (256, 4), (128, 6), (64, 8), (256, 12), (0, 0)])
 def test_launch_shape_invariance(tmp_path, b50, block, tpl):
     bnd, cfg = b50
     prog = program(bnd, cfg)

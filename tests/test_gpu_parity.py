@@ -134,6 +134,19 @@ def test_many_distinct_fixed_points(tmp_path, kernel, order):
     assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=3000, extra=order))
 
 
+@pytest.mark.parametrize("n_mem,n_dyn,min_fixed", [(50, 10, 2900), (4, 3, 12)])
+def test_fixed_point_counts_kept_per_lane(tmp_path, n_mem, n_dyn, min_fixed):
+    """Models of up to 64 nodes: a lane counts repeats of its last fixed point
+    in registers and inserts when it ends in another one and when it stops.
+    60 nodes, nearly every trajectory its own fixed point (an insert at every
+    end); 7 nodes, 16 attractors (lanes switch between few points)."""
+    bnd, cfg = write_memory_model(tmp_path, n_mem=n_mem, n_dyn=n_dyn)
+    g, r = gpu_raw(tmp_path, bnd, cfg, kernel="lane", n_trajectories=3000)
+    assert r.info.kernel == 0
+    assert len(read_raw(g)["P"]) >= min_fixed
+    assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=3000))
+
+
 def write_slow_model(tmp_path):
     """Six slow togglers: a dwell lasts ~5 time units, ~100 windows of 0.05, so
     the windows between a record's first and last ones fill many 64-lane
