@@ -309,6 +309,14 @@ def bench_grad(mg, netgen, wdir, steps):
             "backward_valu_frac_at_forward_ops": valu_frac(info, bt / bwd)}
 
 
+def _step(what, t0=[None]):
+    """One progress line on stderr as each block of the run ends."""
+    now = time.perf_counter()
+    if t0[0] is None:
+        t0[0] = now
+    print(f"bench: {what} ({now - t0[0]:.1f} s)", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -362,6 +370,7 @@ def main():
     prog = model.compile()
     n = a.trajectories if a.trajectories > 0 else info.sample_count
 
+    _step("model compiled")
     if a.warmup:
         pipelined_steps(prog, n, a.warmup)
     # the headline: K host-inclusive steps
@@ -376,6 +385,7 @@ def main():
     fi = full.info
     lat_s = time.perf_counter() - t1
     del full
+    _step("headline, device-only and latency steps")
 
     sorted_out = None
     if li.window_mode == 1 and not a.no_sorted_output:
@@ -446,12 +456,17 @@ def main():
     }
     if not a.no_forms and a.workload == "plain_100":
         out["discrete"] = bench_discrete(mg, netgen, wdir, max(1, min(a.steps, 10)))
+        _step("discrete")
     if not a.no_forms:
         out["scale_1000"] = bench_scale(mg, netgen, wdir, 2, not a.no_cpu_baseline, max(a.cpu_seconds, 4.0))
+        _step("scale_1000")
         out["batched"] = bench_batched(mg, netgen, wdir, max(1, min(a.steps, 3)))
+        _step("batched")
         out["grad"] = bench_grad(mg, netgen, wdir, max(1, min(a.steps, 3)))
+        _step("grad")
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(bnd, cfg, a.cpu_seconds, cpu_threads())
+        _step("cpu baseline")
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
         b1, c1 = netgen.write_workload("plain_small", wdir)
         cb1 = cpu_baseline(b1, c1, 0.0, cpu_threads(), min_ids=10000)

@@ -337,6 +337,11 @@ typedef struct mbg_result_info
 	int32_t pad2;
 	uint64_t window_spills;          /* hashed window mode: cells whose probe passed 256 slots, kept in a
 	                                    spill list and summed with the table's cells after the run */
+	uint32_t window_record_bytes;    /* bytes per window cell copied to the host: 12 when the cells
+	                                    travel packed (key and value in one 96-bit record), else 16 */
+	int32_t pad3;
+	uint64_t window_escapes;         /* packed cells whose value did not fit its record and was copied
+	                                    in a side list */
 } mbg_result_info;
 
 int mbg_result_get_info(const mbg_result* r, mbg_result_info* out);

@@ -142,6 +142,9 @@ class ResultInfo(ctypes.Structure):
         ("simulate_ms", ctypes.c_double), ("window_pairs", ctypes.c_uint64),
         ("moment_spills", ctypes.c_uint64), ("has_errors", ctypes.c_int32), ("pad2", ctypes.c_int32),
         ("window_spills", ctypes.c_uint64),
+        ("window_record_bytes", ctypes.c_uint32),
+        ("pad3", ctypes.c_int32),
+        ("window_escapes", ctypes.c_uint64),
     ]
 
 

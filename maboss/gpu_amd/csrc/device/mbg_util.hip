@@ -13,10 +13,14 @@
 //                    count those trajectories per (first full window, observed
 //                    state) and this kernel adds count * q(w) to each window w,
 //                    which is exactly the sum of the per-window additions.
-// mbg_table_compact -- occupied slots of a hash table as (key, value) pairs.
+// mbg_table_compact -- occupied slots of a hash table as (key, value) pairs,
+//                    and how many values are at least esc_min (the cells the
+//                    packed copy sends through its side list).
 // mbg_fp_compact   -- published slots of the fixed-point table as records.
 // mbg_rate_view    -- parameter view of run-time rates held in device memory.
 // mbg_cells_to_probs -- dense window cells to probabilities (double).
+// mbg_pack_cells   -- key-sorted window cells as 12-byte records for the copy
+//                    to the host.
 #include <hip/hip_runtime.h>
 
 #include "mbg_kernel_args.h"
@@ -90,20 +94,31 @@ extern "C" __global__ void __launch_bounds__(256) mbg_table_compact(const unsign
 																	unsigned long long cap, unsigned long long* keys,
 																	unsigned long long* vals,
 																	unsigned long long out_cap,
-																	unsigned long long* count)
+																	unsigned long long* count,
+																	unsigned long long esc_min,
+																	unsigned long long* esc_count)
 {
-	__shared__ unsigned int wtot[4];
+	__shared__ unsigned int wtot[4], wesc[4];
 	__shared__ unsigned long long base;
 	const ulonglong2* slots = (const ulonglong2*)table;
 	const uint64_t first = (uint64_t)blockIdx.x * blockDim.x * k_compact_per_thread + threadIdx.x;
 	ulonglong2 e[k_compact_per_thread];
-	unsigned int mine = 0;
+	unsigned int mine = 0, esc = 0;
 #pragma unroll
 	for (int j = 0; j < k_compact_per_thread; ++j)
 	{
 		const uint64_t i = first + (uint64_t)j * blockDim.x;
 		e[j] = i < cap ? slots[i] : make_ulonglong2(MBG_EMPTY_KEY, 0ull);
-		mine += (e[j].x != MBG_EMPTY_KEY && e[j].y != 0ull) ? 1u : 0u;
+		const bool used = e[j].x != MBG_EMPTY_KEY && e[j].y != 0ull;
+		mine += used ? 1u : 0u;
+		esc += (used && esc_min && e[j].y >= esc_min) ? 1u : 0u;
+	}
+	if (esc_min)
+	{
+		// values the packed copy sends through its side list (one atomic per block)
+#pragma unroll
+		for (int off = 32; off > 0; off >>= 1)
+			esc += __shfl_xor(esc, off);
 	}
 	// inclusive scan over the wave, then the waves' totals in order
 	const int lane = __lane_id(), wave = threadIdx.x >> 6;
@@ -117,6 +132,8 @@ extern "C" __global__ void __launch_bounds__(256) mbg_table_compact(const unsign
 	}
 	if (lane == 63)
 		wtot[wave] = incl;
+	if (lane == 0)
+		wesc[wave] = esc;
 	__syncthreads();
 	if (threadIdx.x == 0)
 	{
@@ -128,6 +145,11 @@ extern "C" __global__ void __launch_bounds__(256) mbg_table_compact(const unsign
 			tot += t;
 		}
 		base = tot ? atomicAdd(count, (unsigned long long)tot) : 0ull;
+		unsigned int esc_tot = 0;
+		for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+			esc_tot += wesc[w];
+		if (esc_tot)
+			atomicAdd(esc_count, (unsigned long long)esc_tot);
 	}
 	__syncthreads();
 	uint64_t o = base + wtot[wave] + incl - mine;
@@ -197,4 +219,113 @@ extern "C" __global__ void __launch_bounds__(256) mbg_cells_to_probs(const unsig
 	for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
 		 i += (unsigned long long)gridDim.x * blockDim.x)
 		out[i] = (double)cells[i] / denom;
+}
+
+// Key-sorted window cells (keys below 2^kbits, 1 <= kbits <= 60) as 12-byte
+// records for the copy to the host: record i is the 96-bit integer
+// key | field << kbits, stored as three 32-bit words, least significant
+// first.  With esc_min = 0 (kbits <= 31, so the 96 - kbits bit field holds
+// any 64-bit value) the field is the value.  Otherwise the field has
+// vb = 96 - kbits bits and esc_min = 2^(vb - 1): a value below esc_min is the
+// field itself, and a larger one goes to the side list xval at a position p
+// the field records as esc_min + p.  Positions are handed out per block (one
+// atomic reserves the block's range), so the list's order varies from run to
+// run while every record decodes to the same value; *xcount ends at the
+// number of escaped values, which the caller knows beforehand (the table
+// compaction counts them) and sizes xval with.  A thread packs 4 consecutive
+// cells with 16-byte loads and stores.
+extern "C" __global__ void __launch_bounds__(256) mbg_pack_cells(const unsigned long long* keys,
+																 const unsigned long long* vals, unsigned long long n,
+																 unsigned int kbits, unsigned long long esc_min,
+																 unsigned int* rec, unsigned long long* xval,
+																 unsigned long long xcap, unsigned long long* xcount)
+{
+	__shared__ unsigned int wtot[4];
+	__shared__ unsigned long long base;
+	const uint64_t c0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+	unsigned long long k[4], v[4];
+	if (c0 + 3 < n)
+	{
+		const ulonglong2 ka = *(const ulonglong2*)(keys + c0), kb = *(const ulonglong2*)(keys + c0 + 2);
+		const ulonglong2 va = *(const ulonglong2*)(vals + c0), vb = *(const ulonglong2*)(vals + c0 + 2);
+		k[0] = ka.x, k[1] = ka.y, k[2] = kb.x, k[3] = kb.y;
+		v[0] = va.x, v[1] = va.y, v[2] = vb.x, v[3] = vb.y;
+	}
+	else
+	{
+#pragma unroll
+		for (int j = 0; j < 4; ++j)
+		{
+			k[j] = c0 + j < n ? keys[c0 + j] : 0ull;
+			v[j] = c0 + j < n ? vals[c0 + j] : 0ull;
+		}
+	}
+	bool x[4];
+	unsigned int mine = 0;
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
+	{
+		x[j] = c0 + j < n && esc_min && v[j] >= esc_min;
+		mine += x[j] ? 1u : 0u;
+	}
+	// the block's escapes: wave scan of the per-thread counts, waves' totals in order
+	const int lane = __lane_id(), wave = threadIdx.x >> 6;
+	unsigned int incl = mine;
+#pragma unroll
+	for (int off = 1; off < 64; off <<= 1)
+	{
+		const unsigned int t = __shfl_up(incl, off);
+		if (lane >= off)
+			incl += t;
+	}
+	if (lane == 63)
+		wtot[wave] = incl;
+	__syncthreads();
+	if (threadIdx.x == 0)
+	{
+		unsigned int tot = 0;
+		for (int q = 0; q < (int)(blockDim.x >> 6); ++q)
+		{
+			const unsigned int t = wtot[q];
+			wtot[q] = tot;
+			tot += t;
+		}
+		base = tot ? atomicAdd(xcount, (unsigned long long)tot) : 0ull;
+	}
+	__syncthreads();
+	uint64_t o = base + wtot[wave] + incl - mine;
+	unsigned int w[12];
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
+	{
+		unsigned long long f = v[j];
+		if (x[j])
+		{
+			if (o < xcap)
+				xval[o] = v[j];
+			f = esc_min + o;
+			o++;
+		}
+		const unsigned long long lo = k[j] | (f << kbits);
+		w[3 * j] = (unsigned int)lo;
+		w[3 * j + 1] = (unsigned int)(lo >> 32);
+		w[3 * j + 2] = (unsigned int)(f >> (64 - kbits));
+	}
+	if (c0 + 3 < n)
+	{
+		uint4* out = (uint4*)(rec + c0 * 3);
+		out[0] = make_uint4(w[0], w[1], w[2], w[3]);
+		out[1] = make_uint4(w[4], w[5], w[6], w[7]);
+		out[2] = make_uint4(w[8], w[9], w[10], w[11]);
+	}
+	else
+	{
+		for (int j = 0; j < 4; ++j)
+			if (c0 + j < n)
+			{
+				rec[(c0 + j) * 3] = w[3 * j];
+				rec[(c0 + j) * 3 + 1] = w[3 * j + 1];
+				rec[(c0 + j) * 3 + 2] = w[3 * j + 2];
+			}
+	}
 }

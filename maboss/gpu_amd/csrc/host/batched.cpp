@@ -531,13 +531,7 @@ std::vector<std::unique_ptr<result>> simulate_batched(program& p, const batch_pa
 		}
 	}
 	auto sub_view = [](const cell_view& all, uint64_t first, uint64_t end) {
-		cell_view c = all;
-		const uint64_t lo = all.lower_bound(first), hi = all.lower_bound(end);
-		c.key = all.key + lo;
-		c.val = all.val + lo;
-		c.n = hi - lo;
-		c.sub = first;
-		return c;
+		return all.slice(all.lower_bound(first), all.lower_bound(end), first);
 	};
 	const uint64_t per = 1ull << n_obs;
 	for (int v = 0; v < L.nv; v++)

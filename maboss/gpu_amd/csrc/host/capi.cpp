@@ -532,6 +532,8 @@ int mbg_result_get_info(const mbg_result* rr, mbg_result_info* out)
 		out->window_pairs = r.window_pairs;
 		out->moment_spills = r.moment_spills;
 		out->window_spills = r.window_spills;
+		out->window_record_bytes = r.window_record_bytes;
+		out->window_escapes = r.window_escapes;
 		out->has_errors = r.has_moments ? 1 : 0;
 		out->window_mode = r.window_mode;
 		out->final_mode = r.final_mode;

@@ -73,16 +73,37 @@ private:
 struct cell_view
 {
 	std::shared_ptr<const void> hold;
+	// plain form: keys and values in two arrays
 	const uint64_t* key = nullptr;
 	const uint64_t* val = nullptr;
+	// packed form (rec set): 12-byte records of three 32-bit words, the
+	// 96-bit integer key | field << kbits.  With esc_min = 0 the field is the
+	// value; otherwise the field has 96 - kbits bits, and a field f >= esc_min
+	// stands for the value xval[f - esc_min] of the side list
+	const uint32_t* rec = nullptr;
+	unsigned kbits = 0;
+	uint64_t esc_min = 0;
+	const uint64_t* xval = nullptr;
+	uint64_t nx = 0;
+	uint64_t first = 0; // packed form: record index of this view's cell 0
 	uint64_t n = 0;
 	uint64_t sub = 0;
-	uint64_t k(uint64_t i) const { return key[i] - sub; }
-	uint64_t v(uint64_t i) const { return val[i]; }
+	uint64_t k(uint64_t i) const { return (rec ? packed_key(first + i) : key[i]) - sub; }
+	uint64_t v(uint64_t i) const { return rec ? packed_val(first + i) : val[i]; }
 	// index of the first cell whose key is >= kk
 	uint64_t lower_bound(uint64_t kk) const;
+	// cells [lo, hi) of this view, their keys reduced by s
+	cell_view slice(uint64_t lo, uint64_t hi, uint64_t s) const;
 	// cells from owned vectors
 	static cell_view from(std::vector<uint64_t> keys, std::vector<uint64_t> vals);
+
+private:
+	uint64_t packed_lo(uint64_t r) const
+	{
+		return (uint64_t)rec[3 * r] | ((uint64_t)rec[3 * r + 1] << 32);
+	}
+	uint64_t packed_key(uint64_t r) const { return packed_lo(r) & ((1ull << kbits) - 1ull); }
+	uint64_t packed_val(uint64_t r) const;
 };
 
 // Tables of the gradient kernel (enable masks, initial-state probabilities,
@@ -117,6 +138,7 @@ struct out_slot
 	device_buffer fk, fv, fka, fva; // final states, compacted
 	device_buffer fpc;            // fixed points, compacted records {count, words}
 	device_buffer cnt;            // [4] counts of the compactions
+	device_buffer prec, px;       // window cells packed for the copy: records; escape count + side list
 	cleared_buf cl_ent, cl_cnt;
 	hipEvent_t copied = nullptr;  // the last copy out of this slot
 	bool pending = false;         // copied has been recorded
@@ -166,7 +188,7 @@ struct program
 	// utility kernels (slot fill, tail expansion, table compaction), precompiled
 	hipModule_t mod_util = nullptr;
 	hipFunction_t fn_fill_slots = nullptr, fn_expand_tails = nullptr, fn_table_compact = nullptr,
-				  fn_fp_compact = nullptr, fn_rate_view = nullptr, fn_cells_to_probs = nullptr;
+				  fn_fp_compact = nullptr, fn_rate_view = nullptr, fn_cells_to_probs = nullptr, fn_pack_cells = nullptr;
 	void load_util();
 	// waits for every copy in flight (before buffers are released)
 	void drain_copies();
@@ -288,6 +310,8 @@ struct result
 	double kernel_ms = 0, total_ms = 0, simulate_ms = 0;
 	uint64_t window_pairs = 0;
 	uint64_t window_spills = 0; // hashed window cells that went through the spill list
+	uint32_t window_record_bytes = 16; // bytes per window cell copied to the host (12 packed, 16 plain)
+	uint64_t window_escapes = 0;       // packed cells whose value went through the side list
 	int window_mode = 0, final_mode = 0;
 	int kernel = 0; // 0 lane, 1 wave
 	uint64_t window_capacity = 0;

@@ -74,12 +74,42 @@ uint64_t cell_view::lower_bound(uint64_t kk) const
 	while (lo < hi)
 	{
 		const uint64_t mid = lo + (hi - lo) / 2;
-		if (key[mid] - sub < kk)
+		if (k(mid) < kk)
 			lo = mid + 1;
 		else
 			hi = mid;
 	}
 	return lo;
+}
+
+uint64_t cell_view::packed_val(uint64_t r) const
+{
+	uint64_t f = (packed_lo(r) >> kbits) | ((uint64_t)rec[3 * r + 2] << (64 - kbits));
+	if (!esc_min)
+		return f;
+	const unsigned vb = 96u - kbits;
+	if (vb < 64)
+		f &= (1ull << vb) - 1ull;
+	if (f < esc_min)
+		return f;
+	if (f - esc_min >= nx)
+		throw error(MBG_E_INTERNAL, "packed window cell points past the side list");
+	return xval[f - esc_min];
+}
+
+cell_view cell_view::slice(uint64_t lo, uint64_t hi, uint64_t s) const
+{
+	cell_view c = *this;
+	if (rec)
+		c.first = first + lo;
+	else
+	{
+		c.key = key + lo;
+		c.val = val + lo;
+	}
+	c.n = hi - lo;
+	c.sub = s;
+	return c;
 }
 
 cell_view cell_view::from(std::vector<uint64_t> keys, std::vector<uint64_t> vals)

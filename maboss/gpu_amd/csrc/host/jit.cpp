@@ -294,6 +294,7 @@ void program::load_util()
 	hip_check(hipModuleGetFunction(&fn_rate_view, mod_util, "mbg_rate_view"), "hipModuleGetFunction(mbg_rate_view)");
 	hip_check(hipModuleGetFunction(&fn_cells_to_probs, mod_util, "mbg_cells_to_probs"),
 			  "hipModuleGetFunction(mbg_cells_to_probs)");
+	hip_check(hipModuleGetFunction(&fn_pack_cells, mod_util, "mbg_pack_cells"), "hipModuleGetFunction(mbg_pack_cells)");
 }
 
 hipFunction_t program::wave_function()

@@ -237,11 +237,13 @@ void launch_util(hipFunction_t fn, unsigned grid, void** args, hipStream_t strea
 }
 
 // Occupied slots {key, value} of a hash table as (key, value) pairs; their
-// number is added to *count.
+// number is added to *count, and the number of values >= esc_min (when not 0)
+// to *esc_count.
 void table_compact(program& p, const unsigned long long* table, uint64_t cap, unsigned long long* keys,
-				   unsigned long long* vals, uint64_t out_cap, unsigned long long* count, hipStream_t stream)
+				   unsigned long long* vals, uint64_t out_cap, unsigned long long* count, hipStream_t stream,
+				   uint64_t esc_min = 0, unsigned long long* esc_count = nullptr)
 {
-	void* args[] = { &table, &cap, &keys, &vals, &out_cap, &count };
+	void* args[] = { &table, &cap, &keys, &vals, &out_cap, &count, &esc_min, &esc_count };
 	const unsigned g = (unsigned)std::max<uint64_t>(1, (cap + 256 * 16 - 1) / (256 * 16)); // 16 slots per thread
 	launch_util(p.fn_table_compact, g, args, stream, "hipModuleLaunchKernel(mbg_table_compact)");
 }
@@ -253,6 +255,17 @@ void fp_compact(program& p, const unsigned int* table, uint64_t cap, int stride,
 	void* args[] = { &table, &cap, &stride, &out, &out_cap, &count };
 	const unsigned g = (unsigned)std::min<uint64_t>(4096, std::max<uint64_t>(1, (cap + 255) / 256));
 	launch_util(p.fn_fp_compact, g, args, stream, "hipModuleLaunchKernel(mbg_fp_compact)");
+}
+
+// Key-sorted window cells as 12-byte records (values >= esc_min, when not 0,
+// to the side list xval, their number added to *xcount).
+void pack_cells(program& p, const unsigned long long* keys, const unsigned long long* vals, uint64_t n,
+				unsigned kbits, uint64_t esc_min, unsigned int* rec, unsigned long long* xval, uint64_t xcap,
+				unsigned long long* xcount, hipStream_t stream)
+{
+	void* args[] = { &keys, &vals, &n, &kbits, &esc_min, &rec, &xval, &xcap, &xcount };
+	const unsigned g = (unsigned)((n + 1023) / 1024); // 4 cells per thread
+	launch_util(p.fn_pack_cells, g, args, stream, "hipModuleLaunchKernel(mbg_pack_cells)");
 }
 
 struct run_events
@@ -367,7 +380,7 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 		hip_check(hipStreamCreateWithFlags(&p.copy_stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
 	if (!p.mailbox)
 		p.mailbox = p.pool->get(4096);
-	auto* mb = (unsigned long long*)p.mailbox.get(); // [0, 16) counters, [16, 20) compaction counts
+	auto* mb = (unsigned long long*)p.mailbox.get(); // [0, 16) counters, [16, 22) compaction counts
 
 	// The run's output slot: its previous copy to the host must be complete
 	// before this run writes into it or grows its buffers (a host-side wait:
@@ -458,6 +471,14 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 		hip_check(hipMemsetAsync(ptr, 0, bytes, stream), "hipMemsetAsync");
 	};
 
+	// Window cells travel to the host as 12-byte records when their keys take
+	// at most 60 bits (the copy bounds a step whose statistics are large):
+	// up to 31 key bits leave the record room for any 64-bit value; above
+	// that, values of 2^(95 - key bits) or more go to a side list, which the
+	// hashed table's compaction counts so every copy size is known at the
+	// run's one mid-point synchronisation.
+	const unsigned pk_bits = (unsigned)std::max(n_obs + bits_for((uint64_t)windows - 1), 1);
+	const uint64_t esc_min = pk_bits <= 31 ? 0ull : 1ull << (95 - pk_bits);
 	for (int attempt = 0;; attempt++)
 	{
 		// ---- storage
@@ -709,16 +730,19 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 		}
 
 		// ---- the occupied cells of every table, compacted into the output slot
-		auto* cnt = (unsigned long long*)slot.cnt.ensure(4 * 8);
+		// counts: [0] window cells, [1] final states, [2] fixed points, [3] unique
+		// sorted cells, [4] window values for the packed copy's side list, [5]
+		// the pack kernel's side-list cursor
+		auto* cnt = (unsigned long long*)slot.cnt.ensure(6 * 8);
 		if (!sp.device_only)
 		{
-			use(slot.cl_cnt, cnt, 4 * 8, [&] { memset0(cnt, 4 * 8); });
+			use(slot.cl_cnt, cnt, 6 * 8, [&] { memset0(cnt, 6 * 8); });
 			if (pl.win_mode == MBG_HIST_HASH)
 			{
 				// room after the table's cells for the spilled ones
 				auto* tk = (unsigned long long*)slot.pk.ensure((pl.win_cap + spill_cap) * 8);
 				auto* tv = (unsigned long long*)slot.pv.ensure((pl.win_cap + spill_cap) * 8);
-				table_compact(p, win_keys, pl.win_cap, tk, tv, pl.win_cap, cnt + 0, stream);
+				table_compact(p, win_keys, pl.win_cap, tk, tv, pl.win_cap, cnt + 0, stream, esc_min, cnt + 4);
 			}
 			else if (pl.win_mode != MBG_HIST_SORT)
 			{
@@ -744,12 +768,12 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 		hip_check(hipMemcpyAsync(mb, counters, MBG_NCOUNTERS * 8, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
 		if (!sp.device_only)
 		{
-			hip_check(hipMemcpyAsync(mb + 16, cnt, 4 * 8, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+			hip_check(hipMemcpyAsync(mb + 16, cnt, 6 * 8, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
 			// the sorted mode's unique count (cnt[3]) is written and read after the sort
 			if (pl.win_mode != MBG_HIST_SORT)
 			{
-				memset0(cnt, 4 * 8);
-				slot.cl_cnt = cleared_buf { cnt, 4 * 8 };
+				memset0(cnt, 6 * 8);
+				slot.cl_cnt = cleared_buf { cnt, 6 * 8 };
 			}
 		}
 		clear_used();
@@ -879,11 +903,29 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 			auto* va = (unsigned long long*)slot.fva.ensure(n_fin * 8);
 			sort_pairs(p.b_sort_tmp, fk, fv, ka, va, n_fin, std::max(n_obs, 1), stream, &fk, &fv);
 		}
+		// ---- window cells as 12-byte records for the copy: the hashed table's
+		// cells (unless spilled cells were merged into them after the count)
+		// or other modes' cells whose values always fit
+		const bool packed = n_cells && pk_bits <= 60
+							&& ((pl.win_mode == MBG_HIST_HASH && !n_spill) || (pl.win_mode != MBG_HIST_SORT && !esc_min));
+		const uint64_t n_x = packed && esc_min ? mb[20] : 0;
+		unsigned int* rec = nullptr;
+		unsigned long long* xv = nullptr;
+		if (packed)
+		{
+			rec = (unsigned int*)slot.prec.ensure(n_cells * 12);
+			xv = (unsigned long long*)slot.px.ensure(std::max<uint64_t>(n_x, 1) * 8);
+			pack_cells(p, wk, wv, n_cells, pk_bits, esc_min, rec, xv, n_x, cnt + 5, stream);
+		}
+		res->window_record_bytes = packed ? 12 : 16;
+		res->window_escapes = n_x;
 		hip_check(hipEventRecord(ev->e1, stream), "hipEventRecord");
 		p.mark_run_done(stream);
 
 		// ---- copy out on the copy stream, into one pinned block
-		const size_t o_wv = align256(n_cells * 8), o_ent = o_wv + align256(n_cells * 8);
+		// packed: records, side list, the pack kernel's side-list count
+		const size_t o_wv = packed ? align256(n_cells * 12) : align256(n_cells * 8);
+		const size_t o_ent = packed ? o_wv + align256(n_x * 8) + 256 : o_wv + align256(n_cells * 8);
 		const size_t o_fk = o_ent + align256((size_t)windows * 8), o_fv = o_fk + align256(n_fin * 8);
 		const size_t o_fp = o_fv + align256(n_fin * 8), total = o_fp + align256(n_fp * (fp_stride - 1) * 4);
 		std::shared_ptr<char> blk = p.pool->get(total);
@@ -894,8 +936,19 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 			if (bytes)
 				hip_check(hipMemcpyAsync(h + off, src, bytes, hipMemcpyDeviceToHost, cs), "hipMemcpyAsync");
 		};
-		d2h(0, wk, n_cells * 8);
-		d2h(o_wv, wv, n_cells * 8);
+		if (packed)
+		{
+			d2h(0, rec, n_cells * 12);
+			d2h(o_wv, xv, n_x * 8);
+			d2h(o_wv + align256(n_x * 8), cnt + 5, 8);
+			// the cursor cleared for the slot's next run, after its copy
+			hip_check(hipMemsetAsync(cnt + 5, 0, 8, cs), "hipMemsetAsync");
+		}
+		else
+		{
+			d2h(0, wk, n_cells * 8);
+			d2h(o_wv, wv, n_cells * 8);
+		}
 		d2h(o_ent, ent, (size_t)windows * 8);
 		d2h(o_fk, fk, n_fin * 8);
 		d2h(o_fv, fv, n_fin * 8);
@@ -911,7 +964,8 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 		slot.pending = true;
 
 		result* r = res.get();
-		r->on_ready = [r, ev, blk, h, n_cells, n_fin, n_fp, o_wv, o_ent, o_fk, o_fv, o_fp, fp_stride, t0, W]() {
+		r->on_ready = [r, ev, blk, h, n_cells, n_fin, n_fp, o_wv, o_ent, o_fk, o_fv, o_fp, fp_stride, t0, W, packed,
+					   n_x, pk_bits, esc_min]() {
 			hip_check(hipEventSynchronize(ev->copied), "hipEventSynchronize");
 			float ms = 0, sim_ms = 0;
 			hip_check(hipEventElapsedTime(&ms, ev->e0, ev->e1), "hipEventElapsedTime");
@@ -919,8 +973,24 @@ std::unique_ptr<result> run_impl(program& p, const sim_params& sp, hipStream_t s
 			r->kernel_ms = ms;
 			r->simulate_ms = sim_ms;
 			r->win.hold = blk;
-			r->win.key = (const uint64_t*)h;
-			r->win.val = (const uint64_t*)(h + o_wv);
+			if (packed)
+			{
+				uint64_t placed = 0;
+				std::memcpy(&placed, h + o_wv + align256(n_x * 8), 8);
+				if (placed != n_x)
+					throw error(MBG_E_INTERNAL, "packed window cells: the side list holds " + std::to_string(placed)
+													+ " values, the compaction counted " + std::to_string(n_x));
+				r->win.rec = (const uint32_t*)h;
+				r->win.kbits = pk_bits;
+				r->win.esc_min = esc_min;
+				r->win.xval = (const uint64_t*)(h + o_wv);
+				r->win.nx = n_x;
+			}
+			else
+			{
+				r->win.key = (const uint64_t*)h;
+				r->win.val = (const uint64_t*)(h + o_wv);
+			}
 			r->win.n = n_cells;
 			std::memcpy(r->ent.data(), h + o_ent, r->ent.size() * 8);
 			r->fin.hold = blk;

@@ -84,6 +84,22 @@ def test_workloads_bit_exact(tmp_path, wl, name, n):
         assert r.info.window_mode == 1  # 50 observed nodes: hashed window cells
 
 
+def test_packed_cells_with_escapes_bit_exact(tmp_path, wl):
+    """Window cells copied as 12-byte records: plain_100's keys take 57 bits,
+    leaving 39 for the value, so cells above 128 trajectory-ticks go through
+    the side list; every cell still equals the oracle's."""
+    bnd, cfg = wl["plain_100"]
+    n = 30000
+    g, r = gpu_raw(tmp_path, bnd, cfg, n_trajectories=n)
+    assert r.info.window_record_bytes == 12
+    assert r.info.window_escapes > 0
+    assert_same(g, oracle_raw(tmp_path, bnd, cfg, n=n))
+    # the sorted-pairs mode copies plain 16-byte cells: same result
+    s, rs = gpu_raw(tmp_path, bnd, cfg, "s", n_trajectories=n, window_mode="sort")
+    assert rs.info.window_record_bytes == 16
+    assert_same(g, s)
+
+
 @pytest.mark.parametrize("name,n", [("plain_small", 4000), ("plain_100", 2000), ("plain_100_discrete", 2000),
                                     ("grad_30", 3000), ("scale_1000", 300)])
 def test_wave_kernel_bit_exact(tmp_path, wl, name, n):
