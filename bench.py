@@ -428,6 +428,7 @@ def main():
         "config": {"workload": f"{a.workload}: {WORKLOAD_TEXT[a.workload]}", "trajectories": n,
                    "nodes": N, "state_words": W, "observed_nodes": nobs, "windows": info.windows,
                    "form": "plain", "transitions_per_step": per_step, "window_cells_per_step": li.n_window_cells,
+                   "window_record_bytes": li.window_record_bytes, "window_escapes_per_step": li.window_escapes,
                    "timed": "first kernel to the last statistic in pinned host memory; steps pipelined "
                             "(a step's copy overlaps the next step's kernels)"},
         "roofline": {"bound": "valu" if valu_bound else "hbm",
